@@ -1,0 +1,182 @@
+"""Benchmark of the LBM time step on one MI355X.
+
+Workload (the headline configuration): D3Q19 TRT (Lambda 3/16, tau 0.8),
+192^3 periodic, one squirmer of radius 12 (B1 = 1e-3, beta from --beta,
+position and orientation uniform from the seed), Ladd moving-wall bounce-back
+with momentum-exchange force/torque, populations stored as FP16 deviations
+with FP32 collision.  One step = one tick! of the reference.
+
+Prints ONE JSON line: MLUPS (all nodes x steps / s), the roofline of the
+stream-collide kernel (algorithmic bytes 2*Q*sizeof(storage) per node over its
+HIP-event-timed average duration), the CPU baseline (the oracle, a C
+restatement of the reference step, on this host's cores), FP64/FP32 variants
+of the same step, and the FP16-vs-FP64 errors after --accuracy-steps steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
+
+METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
+HBM_PEAK_GBS = 8000.0
+BYTES = {"f64": 8, "f32": 4, "f16": 2}
+
+
+def workload_kwargs(storage: str, n: int):
+    return dict(dims=3, velocities="D3Q19", latticeParameter=1.0, relaxationTimeRatio=0.8,
+                kineticParameter=3 / 16, xlims=(-n // 2, n // 2 - 1), storage=storage)
+
+
+def squirmer_kwargs(seed: int, n: int, r: float, beta: float):
+    rng = np.random.default_rng(seed)
+    lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
+    pos = rng.uniform(lo, hi, 3)
+    e = rng.standard_normal(3)
+    return dict(radius=r, position=pos, swimmingSpeed=2e-3 / 3, beta=beta, swimmingDirection=e / np.linalg.norm(e),
+                coupleForces=True, coupleTorques=True)
+
+
+def build_model(lh, storage, args):
+    m = lh.modelInit(**workload_kwargs(storage, args.n))
+    lh.addSquirmer(m, **squirmer_kwargs(args.seed, args.n, args.radius, args.beta))
+    return m
+
+
+def timed(lh, m, steps, warmup):
+    m.run(warmup)
+    m.sync()
+    m.set_timing(True)
+    t0 = time.perf_counter()
+    m.run(steps)
+    m.sync()
+    t1 = time.perf_counter()
+    m.set_timing(False)
+    kms, sms = m.last_run_times()
+    return t1 - t0, kms, sms
+
+
+def cpu_baseline(args):
+    """The oracle (FP64 C restatement of the reference step, OpenMP over
+    nodes) on the same workload, timed for a bounded number of steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import harness
+    kw = workload_kwargs("f64", args.n)
+    o = harness.oracle_model(**kw)
+    o.add("squirmer", **squirmer_kwargs(args.seed, args.n, args.radius, args.beta))
+    o.run(1)  # first touch
+    steps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while steps < args.cpu_max_steps and t < args.cpu_seconds:
+        o.run(1)
+        steps += 1
+        t = time.perf_counter() - t0
+    nodes = args.n ** 3
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nodes * steps / t / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the same 192^3 squirmer workload in FP64 ({t:.1f} s), "
+                      f"oracle/lbm_oracle.c with OpenMP over nodes"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--storage", default="f16", choices=list(BYTES))
+    ap.add_argument("--n", type=int, default=192)
+    ap.add_argument("--radius", type=float, default=12.0)
+    ap.add_argument("--beta", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--accuracy-steps", type=int, default=5000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import lbmengine_hip as lh
+    nodes = args.n ** 3
+    Q = 19
+
+    m = build_model(lh, args.storage, args)
+    wall, kms, sms = timed(lh, m, args.steps, args.warmup)
+    mlups = nodes * args.steps / wall / 1e6
+    bytes_per_launch = 2 * Q * BYTES[args.storage] * nodes
+    achieved = bytes_per_launch / (kms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_step_{args.storage}_{args.n}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.storage,
+        "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
+        "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
+                               f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
+                   "nodes": nodes, "block": list(m.get_block())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "step_kernel", "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
+                     "bytes_per_launch": bytes_per_launch},
+    }
+    del m
+
+    if not args.no_variants:
+        var = {}
+        for st in ("f64", "f32", "f16"):
+            if st == args.storage:
+                var[st] = {"MLUPS": out["value"], "kernel_ms": out["roofline"]["kernel_ms"]}
+                continue
+            mv = build_model(lh, st, args)
+            w, k, s = timed(lh, mv, max(args.steps // 4, 100), args.warmup // 4)
+            var[st] = {"MLUPS": round(nodes * max(args.steps // 4, 100) / w / 1e6, 1), "kernel_ms": round(k, 5),
+                       "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            del mv
+        var["speedup_f16_over_f64"] = round(var["f16"]["MLUPS"] / var["f64"]["MLUPS"], 2)
+        out["variants"] = var
+
+        # FP16/FP32 deviation storage against FP64 after the same number of ticks
+        acc = {}
+        ref = build_model(lh, "f64", args)
+        ref.run(args.accuracy_steps)
+        ref.sync()
+        u64 = ref.fluidVelocity
+        v64 = ref.particles[0].velocity
+        del ref
+        for st in ("f32", "f16"):
+            mm = build_model(lh, st, args)
+            mm.run(args.accuracy_steps)
+            mm.sync()
+            u = mm.fluidVelocity
+            v = mm.particles[0].velocity
+            acc[st] = {"u_rel_l2": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                       "speed_rel": float(abs(np.linalg.norm(v) - np.linalg.norm(v64)) / np.linalg.norm(v64)),
+                       "velocity_rel": float(np.linalg.norm(v - v64) / np.linalg.norm(v64))}
+            del mm
+        acc["steps"] = args.accuracy_steps
+        acc["swimming_speed_f64"] = float(np.linalg.norm(v64))
+        out["accuracy_vs_f64"] = acc
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * lbm_hip.h — C ABI of liblbm_hip.so, the MI355X (gfx950) time step of a
+ * lattice-Boltzmann engine for squirmers (LBMengine.jl semantics).
+ *
+ * The reference has no FFI: its only seam is the backend directory chosen
+ * at module load (src/LBMengine.jl:12-24), where every backend defines the
+ * same generic functions over its own `LBMmodel`.  The entry points below
+ * are what a third backend directory would `ccall` in place of those
+ * functions; each one cites the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns int: 0 (LBM_OK) on success, a negative LBM_E*
+ *    code on failure; lbm_last_error() then returns a thread-local message.
+ *    No C++ exception or abort crosses this boundary.
+ *  - Arrays use the reference's Julia layout: column-major, x fastest, the
+ *    velocity (or vector-component) index slowest; q is 0-based here where
+ *    the reference is 1-based.  Host buffers are caller-owned and copied
+ *    synchronously; device memory is owned by the handle.
+ *  - A handle is used from one host thread at a time.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_EINVAL -1     /* invalid argument or configuration            */
+#define LBM_ENOMEM -2     /* device or host allocation failed             */
+#define LBM_EDEVICE -3    /* HIP runtime error                            */
+#define LBM_ESTATE -4     /* operation not valid in the current state     */
+#define LBM_ERANGE -5     /* a particle left the region the step supports */
+
+#define LBM_COLLISION_BGK 0 /* :bgk  gpu/fluids.jl:69-71  */
+#define LBM_COLLISION_TRT 1 /* :trt  gpu/fluids.jl:72-88  */
+
+#define LBM_STORAGE_F64 0   /* populations f_i stored in FP64                  */
+#define LBM_STORAGE_F32 1   /* deviations f_i - w_i rho0 stored in FP32        */
+#define LBM_STORAGE_F16 2   /* deviations scaled by 2^14/w_i stored in FP16    */
+
+#define LBM_SEMANTICS_GPU 0 /* follow src/gpu/ (Rodrigues rotation of e, ...)  */
+#define LBM_SEMANTICS_CPU 1 /* follow src/cpu/ (swimming direction not rotated) */
+
+#define LBM_FORCE_NONE 0
+#define LBM_FORCE_UNIFORM 1 /* constant Guo body force, zeroed on walls         */
+#define LBM_FORCE_FIELD 2   /* per-node Guo body force (lbm_set_force_field)   */
+
+#define LBM_PARTICLE_BEAD 0     /* addBead!      gpu/initMethods.jl:7-89   */
+#define LBM_PARTICLE_SQUIRMER 1 /* addSquirmer!  gpu/initMethods.jl:91-222 */
+
+#define LBM_MAX_PARTICLES 8
+
+/* Resolved model parameters: the output of modelInit's unit and scheme
+ * resolution (gpu/initMethods.jl:440-702), done by the host front end. */
+typedef struct lbm_config {
+  int32_t dims;          /* 2 or 3                                        */
+  int32_t q;             /* 9 (D2Q9, dims 2) or 19 (D3Q19, dims 3)        */
+  int64_t n[3];          /* nodes per dimension; n[2] = 1 when dims == 2  */
+  int32_t collision;     /* LBM_COLLISION_*                               */
+  int32_t storage;       /* LBM_STORAGE_*                                 */
+  int32_t semantics;     /* LBM_SEMANTICS_*                               */
+  int32_t walled;        /* bit k: dimension k has a wall layer at index 0
+                            and n[k]-1 (walledDimensions, misc.jl:7-31)   */
+  int32_t force;         /* LBM_FORCE_*                                   */
+  int32_t reserved0;
+  double force_uniform[3]; /* used when force == LBM_FORCE_UNIFORM        */
+  double dt;             /* timeStep                                      */
+  double dx;             /* latticeParameter (the GPU backend forces dx == dt) */
+  double tau;            /* relaxationTime (BGK) or relaxationTimePlus (TRT) */
+  double tau_minus;      /* relaxationTimeMinus (TRT only)                */
+  double rho0;           /* shift of the deviation storage (FP32/FP16)    */
+  const double *coords[3]; /* node coordinates along each dimension, length
+                              n[k]; the values of Julia's range(xlims..., step=dx) */
+} lbm_config;
+
+/* One rigid spherical particle, parameters resolved by addSquirmer!/addBead!
+ * (mass from the origin-centred sphere, B1 from swimmingSpeed, ...). */
+typedef struct lbm_particle {
+  int32_t kind;            /* LBM_PARTICLE_*                              */
+  int32_t couple_forces;   /* coupleForces                                */
+  int32_t couple_torques;  /* coupleTorques                               */
+  int32_t reserved0;
+  double radius;
+  double position[3];
+  double velocity[3];
+  double angular_velocity[3]; /* in 2D only [0] is used (scalar omega)   */
+  double swimming_direction[3];
+  double B1, B2;
+  double inverse_mass;
+  double inverse_moment_of_inertia;
+} lbm_particle;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char *lbm_last_error(void);
+
+/* Library version string. */
+const char *lbm_version(void);
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int lbm_device_count(int32_t *count);
+
+/* Replaces the device allocation half of modelInit (gpu/initMethods.jl:440-702):
+ * allocates the populations for cfg and the step's working state.  The
+ * populations are undefined until lbm_init_uniform or lbm_set_populations. */
+int lbm_create(const lbm_config *cfg, lbm_handle **out);
+
+int lbm_destroy(lbm_handle *h);
+
+/* findInitialConditions for uniform rho and u (gpu/initMethods.jl:417-438,
+ * 657-668): f_i = feq_i(rho, u - dt F / (2 rho)), with rho = 0 on walls. */
+int lbm_init_uniform(lbm_handle *h, double rho, const double *u);
+
+/* Per-node Guo force density, host FP64 (N..., dims); zeroed on walls as
+ * modelInit does (gpu/initMethods.jl:643-647).  Requires LBM_FORCE_FIELD. */
+int lbm_set_force_field(lbm_handle *h, const double *force);
+
+/* model.distributions <- f / f <- model.distributions, host FP64 (N..., Q). */
+int lbm_set_populations(lbm_handle *h, const double *f);
+int lbm_get_populations(lbm_handle *h, double *f);
+
+/* hydroVariablesUpdate! (gpu/fluids.jl:7-37) evaluated on the current
+ * populations: rho (N...) and u (N..., dims), host FP64.  Either pointer
+ * may be NULL. */
+int lbm_get_hydro(lbm_handle *h, double *rho, double *u);
+
+/* addSquirmer!/addBead! after parameter resolution, including the initial
+ * moveParticles!(; initialSetup = true) (gpu/initMethods.jl:91-222,
+ * gpu/particles.jl:7-92).  *id receives the 0-based particle index. */
+int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id);
+
+/* Current state of particle id (position, velocity, angular velocity,
+ * swimming direction; the other fields echo what was added). */
+int lbm_get_particle(lbm_handle *h, int32_t id, lbm_particle *out);
+
+/* Momentum-exchange force and torque of the last step, before the
+ * integrator consumed them (forceInput/torqueInput, gpu/fluids.jl:193-204). */
+int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3],
+                          double torque[3]);
+
+/* Boundary links of particle id in the reference's findall order over its
+ * streamingInvasionRegions (gpu/aux.jl:187-198, gpu/fluids.jl:181): link
+ * (node, j) means node is fluid and node + c_j is solid.  Writes at most cap
+ * entries; *count receives the total. */
+int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir,
+                  int64_t cap, int64_t *count);
+
+/* LBMpropagate!(model; ticks = nsteps) (gpu/fluids.jl:228-264): nsteps calls
+ * of tick! (gpu/fluids.jl:132-226).  Asynchronous on the handle's stream. */
+int lbm_run(lbm_handle *h, int64_t nsteps);
+
+/* Wait for the handle's stream; reports a particle that left the supported
+ * region (LBM_ERANGE) or a HIP error. */
+int lbm_sync(lbm_handle *h);
+
+/* model.tick and model.time. */
+int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time);
+
+/* Block shape of the step kernel (threads along x, y, z; product 64..1024),
+ * normally chosen by the autotuner; 0 resets to the default. */
+int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz);
+
+/* Current block shape of the step kernel. */
+int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz);
+
+/* Measurement (no counterpart in the reference, which has no timing):
+ * with timing on, lbm_run records HIP events around every stream-collide
+ * kernel on the handle's stream, waits for the run and keeps the averages. */
+int lbm_set_timing(lbm_handle *h, int on);
+
+/* Average device time per step kernel over the last timed lbm_run, in ms
+ * (0 if no timed run happened). */
+int lbm_last_run_ms(lbm_handle *h, double *ms);
+
+/* Averages of the last timed lbm_run: stream-collide kernel time and whole
+ * step time (all kernels of a tick), in ms.  Either pointer may be NULL. */
+int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

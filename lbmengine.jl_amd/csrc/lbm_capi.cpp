@@ -1,0 +1,705 @@
+// lbm_capi.cpp — host runtime of liblbm_hip.so: the extern "C" entry points
+// declared in include/lbm_hip.h.  It owns the device state of one lattice
+// (ping-pong population buffers, coordinate tables, particle records,
+// momentum-exchange partials), launches the step on its own stream, and
+// optionally replays the step loop from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_dispatch.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(LBM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                  \
+  do {                                       \
+    hipError_t e_ = (call);                  \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+size_t elem_size(int storage) {
+  return storage == LBM_STORAGE_F64 ? 8 : (storage == LBM_STORAGE_F32 ? 4 : 2);
+}
+
+}  // namespace
+
+struct lbm_handle {
+  lbm_config cfg{};
+  int dims = 0, Q = 0;
+  int32_t n[3] = {1, 1, 1};
+  int64_t N = 0;
+  double x0[3] = {0, 0, 0};
+  std::vector<double> coords_h[3];
+  double *coords_d[3] = {nullptr, nullptr, nullptr};
+  void *f[2] = {nullptr, nullptr};
+  int cur = 0;
+  void *F = nullptr;  // force field (double or float)
+  DevParticle *parts_d = nullptr;
+  std::vector<DevParticle> parts_h;  // parameters as added
+  double *partial = nullptr;
+  size_t partial_cap = 0;
+  PartLaunch pl{};
+  int link_blocks = 0;
+  Tiling tiling{};
+  hipStream_t stream = nullptr;
+  int64_t tick = 0;
+  double time = 0.0;
+  bool populated = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t run_start = nullptr, run_stop = nullptr;
+  int64_t timed_steps = 0;
+  double step_kernel_ms = 0.0, step_ms = 0.0;
+  // graph of an even number of steps starting from buffer 0
+  hipGraphExec_t graph = nullptr;
+  int64_t graph_steps = 0;
+};
+
+namespace {
+
+BoxGeom geom(const lbm_handle *h) {
+  BoxGeom g;
+  g.nx = h->n[0];
+  g.ny = h->n[1];
+  g.nz = h->n[2];
+  g.walled = h->cfg.walled;
+  g.N = h->N;
+  return g;
+}
+
+CollParams coll(const lbm_handle *h) {
+  CollParams c;
+  c.dt = h->cfg.dt;
+  c.tau = h->cfg.tau;
+  c.tau_minus = h->cfg.collision == LBM_COLLISION_TRT ? h->cfg.tau_minus : h->cfg.tau;
+  c.ls = h->cfg.dx / h->cfg.dt;
+  c.invC2s = 3.0 / (c.ls * c.ls);
+  c.invC4s = 9.0 / (c.ls * c.ls * c.ls * c.ls);
+  c.rho0 = h->cfg.rho0;
+  for (int k = 0; k < 3; k++) c.F[k] = h->cfg.force == LBM_FORCE_UNIFORM ? h->cfg.force_uniform[k] : 0.0;
+  c.force_mode = h->cfg.force;
+  return c;
+}
+
+Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
+  Tiling t;
+  t.bx = bx;
+  t.by = by;
+  t.bz = bz;
+  t.tx = (h->n[0] + bx - 1) / bx;
+  t.ty = (h->n[1] + by - 1) / by;
+  t.tz = (h->n[2] + bz - 1) / bz;
+  t.ntiles = t.tx * t.ty * t.tz;
+  return t;
+}
+
+Tiling default_tiling(const lbm_handle *h) {
+  // A block spans whole x rows where rows are short, so the x-shifted pushes
+  // of a row land in cache lines the same block writes.
+  int bx = h->n[0] <= 256 ? h->n[0] : 256;
+  int by = std::max(1, 512 / bx);
+  by = std::min(by, h->n[1]);
+  return make_tiling(h, bx, by, 1);
+}
+
+hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
+  StepArgs a;
+  a.fin = h->f[in];
+  a.fout = h->f[in ^ 1];
+  a.Ffield = h->F;
+  a.g = geom(h);
+  a.c = coll(h);
+  a.t = h->tiling;
+  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx * h->tiling.by * h->tiling.bz), h->stream};
+  hipError_t e;
+  if (e0) hipEventRecord(e0, h->stream);
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+    case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+    default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+  }
+  if (e != hipSuccess) return e;
+  if (e1) hipEventRecord(e1, h->stream);
+  if (h->parts_h.empty()) return hipSuccess;
+  LinkArgs la;
+  la.fin = h->f[in];
+  la.fout = h->f[in ^ 1];
+  la.Ffield = h->F;
+  la.g = a.g;
+  la.c = a.c;
+  la.parts = h->parts_d;
+  for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
+  la.partial = h->partial;
+  la.pl = h->pl;
+  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+    case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+    default: e = launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+  }
+  if (e != hipSuccess) return e;
+  IntegrateArgs ia;
+  ia.parts = h->parts_d;
+  ia.partial = h->partial;
+  ia.np = (int32_t)h->parts_h.size();
+  ia.nblocks = h->link_blocks;
+  ia.dims = h->dims;
+  ia.semantics = h->cfg.semantics;
+  ia.dt = h->cfg.dt;
+  ia.dx = h->cfg.dx;
+  for (int k = 0; k < 3; k++) {
+    ia.x0[k] = h->x0[k];
+    ia.n[k] = h->n[k];
+  }
+  return launch_integrate(ia, h->stream);
+}
+
+void drop_graph(lbm_handle *h) {
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  h->graph = nullptr;
+  h->graph_steps = 0;
+}
+
+int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
+  a.g = geom(h);
+  a.c = coll(h);
+  a.Ffield = h->F;
+  const int threads = 256;
+  Launch l{dim3((unsigned)((h->N + threads - 1) / threads)), dim3(threads), h->stream};
+  hipError_t e = hipErrorInvalidValue;
+#define LBM_AUX(SK)                                                     \
+  switch (which) {                                                      \
+    case 0: e = launch_init_uniform<SK>(h->Q, a, l); break;             \
+    case 1: e = launch_set_populations<SK>(h->Q, a, l); break;          \
+    case 2: e = launch_get_populations<SK>(h->Q, a, l); break;          \
+    case 3: e = launch_hydro<SK>(h->Q, a, l); break;                    \
+  }
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: LBM_AUX(kF64); break;
+    case LBM_STORAGE_F32: LBM_AUX(kF32); break;
+    default: LBM_AUX(kF16); break;
+  }
+#undef LBM_AUX
+  if (e != hipSuccess) return hip_fail(e, "aux kernel launch");
+  return LBM_OK;
+}
+
+int refresh_particle_launch(lbm_handle *h) {
+  const int np = (int)h->parts_h.size();
+  PartLaunch &pl = h->pl;
+  pl = PartLaunch{};
+  pl.np = np;
+  if (h->dims == 3) {
+    pl.lx = 8;
+    pl.ly = 8;
+    pl.lz = 4;
+  } else {
+    pl.lx = 16;
+    pl.ly = 16;
+    pl.lz = 1;
+  }
+  pl.semantics = h->cfg.semantics;
+  double coeff = 1.0;
+  for (int k = 0; k < h->dims; k++) coeff *= h->cfg.dx;
+  if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
+  pl.coeff = coeff;
+  pl.cutoff = 2.0 * h->cfg.dx;
+  const int32_t bl[3] = {pl.lx, pl.ly, pl.lz};
+  int blocks = 0;
+  for (int p = 0; p < np; p++) {
+    pl.first_block[p] = blocks;
+    int nb = 1;
+    for (int k = 0; k < 3; k++) {
+      const int32_t e = k < h->dims ? std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]) : 1;
+      pl.ext[p][k] = e;
+      pl.tiles[p][k] = (e + bl[k] - 1) / bl[k];
+      nb *= pl.tiles[p][k];
+    }
+    blocks += nb;
+  }
+  pl.first_block[np] = blocks;
+  h->link_blocks = blocks;
+  const size_t need = (size_t)std::max(blocks, 1) * std::max(np, 1) * 6;
+  if (need > h->partial_cap) {
+    if (h->partial) hipFree(h->partial);
+    h->partial = nullptr;
+    HIP_TRY(hipMalloc(&h->partial, need * sizeof(double)), "hipMalloc partials");
+    h->partial_cap = need;
+  }
+  return LBM_OK;
+}
+
+int check_particles(lbm_handle *h) {
+  if (h->parts_h.empty()) return LBM_OK;
+  std::vector<DevParticle> tmp(h->parts_h.size());
+  HIP_TRY(hipMemcpy(tmp.data(), h->parts_d, tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
+          "copy particles");
+  for (size_t p = 0; p < tmp.size(); p++)
+    if (tmp[p].err)
+      return fail(LBM_ERANGE, "particle " + std::to_string(p) +
+                                  " came within two nodes of the lattice edge; the unwrapped sphere "
+                                  "of the reference is only supported away from the faces");
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+
+const char *lbm_version(void) { return "lbm_hip 0.1 (gfx950)"; }
+
+int lbm_device_count(int32_t *count) {
+  if (!count) return fail(LBM_EINVAL, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return LBM_OK;
+}
+
+int lbm_destroy(lbm_handle *h) {
+  if (!h) return LBM_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  for (int k = 0; k < 3; k++)
+    if (h->coords_d[k]) hipFree(h->coords_d[k]);
+  for (int b = 0; b < 2; b++)
+    if (h->f[b]) hipFree(h->f[b]);
+  if (h->F) hipFree(h->F);
+  if (h->parts_d) hipFree(h->parts_d);
+  if (h->partial) hipFree(h->partial);
+  for (auto e : h->ev) hipEventDestroy(e);
+  if (h->run_start) hipEventDestroy(h->run_start);
+  if (h->run_stop) hipEventDestroy(h->run_stop);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return LBM_OK;
+}
+
+int lbm_create(const lbm_config *cfg, lbm_handle **out) {
+  if (!cfg || !out) return fail(LBM_EINVAL, "null argument");
+  *out = nullptr;
+  if (!((cfg->dims == 2 && cfg->q == 9) || (cfg->dims == 3 && cfg->q == 19)))
+    return fail(LBM_EINVAL, "lattice must be D2Q9 (dims 2) or D3Q19 (dims 3)");
+  if (cfg->collision != LBM_COLLISION_BGK && cfg->collision != LBM_COLLISION_TRT)
+    return fail(LBM_EINVAL, "collision must be BGK or TRT");
+  if (cfg->storage < LBM_STORAGE_F64 || cfg->storage > LBM_STORAGE_F16)
+    return fail(LBM_EINVAL, "storage must be F64, F32 or F16");
+  if (cfg->semantics != LBM_SEMANTICS_GPU && cfg->semantics != LBM_SEMANTICS_CPU)
+    return fail(LBM_EINVAL, "semantics must be GPU or CPU");
+  if (cfg->force < LBM_FORCE_NONE || cfg->force > LBM_FORCE_FIELD)
+    return fail(LBM_EINVAL, "force mode must be NONE, UNIFORM or FIELD");
+  if (!(cfg->dt > 0) || !(cfg->dx > 0) || !(cfg->tau > 0))
+    return fail(LBM_EINVAL, "dt, dx and tau must be positive");
+  if (cfg->collision == LBM_COLLISION_TRT && !(cfg->tau_minus > 0))
+    return fail(LBM_EINVAL, "tau_minus must be positive for TRT");
+  if (cfg->walled & ~((1 << cfg->dims) - 1)) return fail(LBM_EINVAL, "walled names a dimension the lattice lacks");
+  int64_t N = 1;
+  for (int k = 0; k < cfg->dims; k++) {
+    if (cfg->n[k] < 1 || cfg->n[k] > (1 << 30)) return fail(LBM_EINVAL, "bad lattice size");
+    if (((cfg->walled >> k) & 1) && cfg->n[k] < 3) return fail(LBM_EINVAL, "a walled dimension needs >= 3 nodes");
+    if (!cfg->coords[k]) return fail(LBM_EINVAL, "missing coordinate table");
+    N *= cfg->n[k];
+  }
+  if (N * (int64_t)cfg->q >= ((int64_t)1 << 31) || N >= ((int64_t)1 << 31))
+    return fail(LBM_EINVAL, "lattice too large for one box (N*Q must stay below 2^31)");
+  lbm_handle *h = new (std::nothrow) lbm_handle;
+  if (!h) return fail(LBM_ENOMEM, "out of host memory");
+  h->cfg = *cfg;
+  h->dims = cfg->dims;
+  h->Q = cfg->q;
+  for (int k = 0; k < 3; k++) h->n[k] = k < cfg->dims ? (int32_t)cfg->n[k] : 1;
+  h->N = N;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  auto bail = [&](int code) {
+    lbm_destroy(h);
+    return code;
+  };
+  for (int k = 0; k < 3; k++) {
+    if (k < cfg->dims)
+      h->coords_h[k].assign(cfg->coords[k], cfg->coords[k] + cfg->n[k]);
+    else
+      h->coords_h[k].assign(1, 0.0);
+    h->x0[k] = h->coords_h[k][0];
+    if (hipMalloc(&h->coords_d[k], h->coords_h[k].size() * sizeof(double)) != hipSuccess)
+      return bail(fail(LBM_ENOMEM, "hipMalloc coordinates"));
+    if (hipMemcpy(h->coords_d[k], h->coords_h[k].data(), h->coords_h[k].size() * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(LBM_EDEVICE, "copy coordinates"));
+  }
+  h->cfg.coords[0] = h->cfg.coords[1] = h->cfg.coords[2] = nullptr;
+  const size_t bytes = (size_t)N * cfg->q * elem_size(cfg->storage);
+  for (int b = 0; b < 2; b++)
+    if (hipMalloc(&h->f[b], bytes) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc populations"));
+  if (cfg->force == LBM_FORCE_FIELD) {
+    const size_t fb = (size_t)N * cfg->dims * (cfg->storage == LBM_STORAGE_F64 ? 8 : 4);
+    if (hipMalloc(&h->F, fb) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc force field"));
+    if (hipMemset(h->F, 0, fb) != hipSuccess) return bail(fail(LBM_EDEVICE, "clear force field"));
+  }
+  if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES) != hipSuccess)
+    return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
+  h->tiling = default_tiling(h);
+  if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
+    return bail(fail(LBM_EDEVICE, "hipEventCreate"));
+  *out = h;
+  return LBM_OK;
+}
+
+int lbm_init_uniform(lbm_handle *h, double rho, const double *u) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.rho_init = rho;
+  for (int k = 0; k < 3; k++) a.u_init[k] = (u && k < h->dims) ? u[k] : 0.0;
+  int rc = aux_launch(h, 0, a);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream), "init");
+  h->tick = 0;
+  h->time = 0.0;
+  h->populated = true;
+  return LBM_OK;
+}
+
+int lbm_set_force_field(lbm_handle *h, const double *force) {
+  if (!h || !force) return fail(LBM_EINVAL, "null argument");
+  if (h->cfg.force != LBM_FORCE_FIELD) return fail(LBM_ESTATE, "the model was created without a force field");
+  const int64_t N = h->N;
+  const int d = h->dims;
+  std::vector<double> F(force, force + (size_t)N * d);
+  for (int64_t n = 0; n < N; n++) {
+    const int x = n % h->n[0], y = (n / h->n[0]) % h->n[1], z = n / ((int64_t)h->n[0] * h->n[1]);
+    const int id[3] = {x, y, z};
+    bool wall = false;
+    for (int k = 0; k < d; k++)
+      if (((h->cfg.walled >> k) & 1) && (id[k] == 0 || id[k] == h->n[k] - 1)) wall = true;
+    if (wall)
+      for (int k = 0; k < d; k++) F[(size_t)k * N + n] = 0.0;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (h->cfg.storage == LBM_STORAGE_F64) {
+    HIP_TRY(hipMemcpy(h->F, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice), "copy force");
+  } else {
+    std::vector<float> Ff(F.begin(), F.end());
+    HIP_TRY(hipMemcpy(h->F, Ff.data(), Ff.size() * sizeof(float), hipMemcpyHostToDevice), "copy force");
+  }
+  return LBM_OK;
+}
+
+int lbm_set_populations(lbm_handle *h, const double *f) {
+  if (!h || !f) return fail(LBM_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  const size_t bytes = (size_t)h->N * h->Q * sizeof(double);
+  double *tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes), "hipMalloc staging");
+  hipError_t e = hipMemcpy(tmp, f, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(tmp);
+    return hip_fail(e, "copy populations");
+  }
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.f64 = tmp;
+  int rc = aux_launch(h, 1, a);
+  e = hipStreamSynchronize(h->stream);
+  hipFree(tmp);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "set populations");
+  h->populated = true;
+  return LBM_OK;
+}
+
+int lbm_get_populations(lbm_handle *h, double *f) {
+  if (!h || !f) return fail(LBM_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  const size_t bytes = (size_t)h->N * h->Q * sizeof(double);
+  double *tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes), "hipMalloc staging");
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.out64 = tmp;
+  int rc = aux_launch(h, 2, a);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (!rc && e == hipSuccess) e = hipMemcpy(f, tmp, bytes, hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "get populations");
+  return LBM_OK;
+}
+
+int lbm_get_hydro(lbm_handle *h, double *rho, double *u) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  const size_t N = (size_t)h->N;
+  double *tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, N * (1 + h->dims) * sizeof(double)), "hipMalloc staging");
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.out64 = tmp;
+  a.uout = tmp + N;
+  int rc = aux_launch(h, 3, a);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (!rc && e == hipSuccess && rho) e = hipMemcpy(rho, tmp, N * sizeof(double), hipMemcpyDeviceToHost);
+  if (!rc && e == hipSuccess && u) e = hipMemcpy(u, tmp + N, N * h->dims * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "get hydro");
+  return LBM_OK;
+}
+
+int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
+  if (!h || !p) return fail(LBM_EINVAL, "null argument");
+  if ((int)h->parts_h.size() >= LBM_MAX_PARTICLES) return fail(LBM_EINVAL, "too many particles");
+  if (p->kind != LBM_PARTICLE_BEAD && p->kind != LBM_PARTICLE_SQUIRMER)
+    return fail(LBM_EINVAL, "particle kind must be BEAD or SQUIRMER");
+  if (!(p->radius >= h->cfg.dx))
+    return fail(LBM_EINVAL, "particle radius must be at least one lattice spacing");
+  for (int k = 0; k < h->dims; k++)
+    if (!(2.0 * p->radius + 6.0 * h->cfg.dx < h->n[k] * h->cfg.dx))
+      return fail(LBM_EINVAL, "particle diameter must be smaller than the box");
+  DevParticle d{};
+  d.kind = p->kind;
+  d.couple_forces = p->couple_forces != 0;
+  d.couple_torques = p->couple_torques != 0;
+  d.radius = p->radius;
+  d.B1 = p->B1;
+  d.B2 = p->B2;
+  d.inv_mass = p->inverse_mass;
+  d.inv_I = p->inverse_moment_of_inertia;
+  for (int k = 0; k < 3; k++) {
+    const bool in = k < h->dims;
+    d.R[k] = in ? p->position[k] : 0.0;
+    d.V[k] = in ? p->velocity[k] : 0.0;
+    d.W[k] = (h->dims == 3 || k == 0) ? p->angular_velocity[k] : 0.0;
+    d.e[k] = in ? p->swimming_direction[k] : 0.0;
+    d.Rg[k] = d.R[k];
+    d.nR[k] = d.R[k];
+    d.nV[k] = d.V[k];
+    d.nW[k] = d.W[k];
+    d.ne[k] = d.e[k];
+  }
+  int32_t err = 0;
+  link_box(d.Rg, d.radius, h->x0, h->cfg.dx, h->n, h->dims, d.lo, d.hi, &err);
+  if (err)
+    return fail(LBM_ERANGE, "particle must lie at least two nodes inside every face of the lattice");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  const int idx = (int)h->parts_h.size();
+  HIP_TRY(hipMemcpy(h->parts_d + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
+  h->parts_h.push_back(d);
+  int rc = refresh_particle_launch(h);
+  if (rc) {
+    h->parts_h.pop_back();
+    return rc;
+  }
+  drop_graph(h);
+  if (id) *id = idx;
+  return LBM_OK;
+}
+
+int lbm_get_particle(lbm_handle *h, int32_t id, lbm_particle *out) {
+  if (!h || !out) return fail(LBM_EINVAL, "null argument");
+  if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  DevParticle d;
+  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  std::memset(out, 0, sizeof *out);
+  out->kind = d.kind;
+  out->couple_forces = d.couple_forces;
+  out->couple_torques = d.couple_torques;
+  out->radius = d.radius;
+  for (int k = 0; k < 3; k++) {
+    out->position[k] = d.R[k];
+    out->velocity[k] = d.V[k];
+    out->angular_velocity[k] = d.W[k];
+    out->swimming_direction[k] = d.e[k];
+  }
+  out->B1 = d.B1;
+  out->B2 = d.B2;
+  out->inverse_mass = d.inv_mass;
+  out->inverse_moment_of_inertia = d.inv_I;
+  return LBM_OK;
+}
+
+int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3], double torque[3]) {
+  if (!h || !force || !torque) return fail(LBM_EINVAL, "null argument");
+  if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  DevParticle d;
+  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  for (int k = 0; k < 3; k++) {
+    force[k] = d.force[k];
+    torque[k] = d.torque[k];
+  }
+  return LBM_OK;
+}
+
+int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_t cap, int64_t *count) {
+  if (!h || !count) return fail(LBM_EINVAL, "null argument");
+  if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
+  if (cap > 0 && (!node || !dir)) return fail(LBM_EINVAL, "null output buffer");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  DevParticle d;
+  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  LinkMaskArgs a;
+  a.g = geom(h);
+  a.part = h->parts_d + id;
+  for (int k = 0; k < 3; k++) {
+    a.coords[k] = h->coords_d[k];
+    a.lo[k] = d.lo[k];
+    a.ext[k] = d.hi[k] - d.lo[k] + 1;
+  }
+  const int64_t tot = (int64_t)a.ext[0] * a.ext[1] * a.ext[2];
+  uint32_t *mask = nullptr;
+  HIP_TRY(hipMalloc(&mask, tot * sizeof(uint32_t)), "hipMalloc link mask");
+  a.mask = mask;
+  Launch l{dim3((unsigned)((tot + 255) / 256)), dim3(256), h->stream};
+  hipError_t e = launch_link_mask(h->Q, a, l);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  std::vector<uint32_t> m(tot);
+  if (e == hipSuccess) e = hipMemcpy(m.data(), mask, tot * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  hipFree(mask);
+  if (e != hipSuccess) return hip_fail(e, "link mask");
+  // findall over (N..., Q): node index fastest, direction slowest; the box
+  // does not wrap, so box order (x fastest) is node order.
+  int64_t c = 0;
+  for (int j = 0; j < h->Q; j++)
+    for (int z = 0; z < a.ext[2]; z++)
+      for (int y = 0; y < a.ext[1]; y++)
+        for (int x = 0; x < a.ext[0]; x++) {
+          const int64_t t = x + (int64_t)a.ext[0] * (y + (int64_t)a.ext[1] * z);
+          if ((m[t] >> j) & 1) {
+            if (c < cap) {
+              node[c] = (a.lo[0] + x) + (int64_t)h->n[0] * ((a.lo[1] + y) + (int64_t)h->n[1] * (a.lo[2] + z));
+              dir[c] = j;
+            }
+            c++;
+          }
+        }
+  *count = c;
+  return LBM_OK;
+}
+
+int lbm_run(lbm_handle *h, int64_t nsteps) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
+  if (nsteps == 0) return LBM_OK;
+  if (h->timing) {
+    if ((int64_t)h->ev.size() < 2 * nsteps) {
+      const size_t old = h->ev.size();
+      h->ev.resize(2 * nsteps);
+      for (size_t i = old; i < h->ev.size(); i++)
+        HIP_TRY(hipEventCreate(&h->ev[i]), "hipEventCreate");
+    }
+    h->timed_steps = nsteps;
+  }
+  HIP_TRY(hipEventRecord(h->run_start, h->stream), "event record");
+  for (int64_t s = 0; s < nsteps; s++) {
+    hipEvent_t e0 = h->timing ? h->ev[2 * s] : nullptr;
+    hipEvent_t e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
+    hipError_t e = step_once(h, h->cur, e0, e1);
+    if (e != hipSuccess) return hip_fail(e, "step launch");
+    h->cur ^= 1;
+    h->tick += 1;
+    h->time += h->cfg.dt;
+  }
+  HIP_TRY(hipEventRecord(h->run_stop, h->stream), "event record");
+  if (h->timing) {
+    HIP_TRY(hipEventSynchronize(h->run_stop), "sync");
+    double sum = 0.0;
+    for (int64_t s = 0; s < nsteps; s++) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * s], h->ev[2 * s + 1]), "event time");
+      sum += ms;
+    }
+    float tot = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&tot, h->run_start, h->run_stop), "event time");
+    h->step_kernel_ms = sum / (double)nsteps;
+    h->step_ms = tot / (double)nsteps;
+  }
+  return LBM_OK;
+}
+
+int lbm_sync(lbm_handle *h) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  return check_particles(h);
+}
+
+int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (tick) *tick = h->tick;
+  if (time) *time = h->time;
+  return LBM_OK;
+}
+
+int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (bx == 0 && by == 0 && bz == 0) {
+    h->tiling = default_tiling(h);
+  } else {
+    const int64_t t = (int64_t)bx * by * bz;
+    if (bx < 1 || by < 1 || bz < 1 || t < 64 || t > 512)
+      return fail(LBM_EINVAL, "block shape must have 64..512 threads");
+    h->tiling = make_tiling(h, bx, by, bz);
+  }
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_last_run_ms(lbm_handle *h, double *ms) {
+  if (!h || !ms) return fail(LBM_EINVAL, "null argument");
+  *ms = h->step_kernel_ms;
+  return LBM_OK;
+}
+
+// Extra entry points used by the host front end and the benchmark.
+
+// Turn per-step kernel timing (HIP events around every step kernel) on/off.
+int lbm_set_timing(lbm_handle *h, int on) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  h->timing = on != 0;
+  return LBM_OK;
+}
+
+// Average device time per step kernel and per whole step of the last timed run.
+int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (kernel_ms) *kernel_ms = h->step_kernel_ms;
+  if (step_ms) *step_ms = h->step_ms;
+  return LBM_OK;
+}
+
+// Current block shape of the step kernel.
+int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (bx) *bx = h->tiling.bx;
+  if (by) *by = h->tiling.by;
+  if (bz) *bz = h->tiling.bz;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,823 @@
+// lbm_step.hpp — the time step of LBMengine.jl as HIP kernels for gfx950.
+//
+// One step = tick! (gpu/fluids.jl:132-226):
+//   step_kernel   every node: load f_i(x) (one coalesced read per q),
+//                 hydro moments, BGK/TRT collision + Guo forcing, then push
+//                 f*_i(x) to x + c_i; static walls handled analytically
+//                 (zeroed wall layers, halfway bounce-back into own slots).
+//   link_kernel   nodes of each particle's link box only: halfway
+//                 bounce-back at particle links, the Ladd moving-wall term
+//                 with the squirmer slip, and the momentum exchange reduced
+//                 per block in FP64 (wavefront shuffles, then LDS).
+//   (integrate_kernel in lbm_particles.hip: fixed-order final reduction
+//                 and moveParticles!.)
+//
+// Populations are SoA in the reference's Julia layout, f[q*N + n] with
+// n = x + nx*(y + ny*z).  The stored state is the reference's
+// model.distributions after streaming, so the step reads each node's own
+// populations (aligned) and scatters them (push).  FP64 storage holds f;
+// FP32 holds f - w_i rho0; FP16 holds (f - w_i rho0) * 2^14 / w_i.
+#pragma once
+
+#include "lbm_common.hpp"
+
+namespace lbm {
+
+enum StoreKind { kF64 = 0, kF32 = 1, kF16 = 2 };
+
+template <int S>
+struct Store;
+template <>
+struct Store<kF64> {
+  using T = double;  // storage element
+  using R = double;  // compute type
+};
+template <>
+struct Store<kF32> {
+  using T = float;
+  using R = float;
+};
+template <>
+struct Store<kF16> {
+  using T = _Float16;
+  using R = float;
+};
+
+// Stored element -> value in the compute type (deviation for FP32/FP16).
+template <int S, int Q>
+__device__ __forceinline__ typename Store<S>::R decode(typename Store<S>::T v, int i) {
+  if constexpr (S == kF16)
+    return static_cast<float>(v) * static_cast<float>(Lattice<Q>::w(i) / kF16Scale);
+  else
+    return v;
+}
+
+template <int S, int Q>
+__device__ __forceinline__ typename Store<S>::T encode(typename Store<S>::R v, int i) {
+  if constexpr (S == kF16)
+    return static_cast<_Float16>(v * static_cast<float>(kF16Scale * Lattice<Q>::invw(i)));
+  else
+    return v;
+}
+
+// Population f_i in FP64 from a stored element.
+template <int S, int Q>
+__device__ __forceinline__ double to_f64(typename Store<S>::T v, int i, double rho0) {
+  if constexpr (S == kF64)
+    return v;
+  else
+    return Lattice<Q>::w(i) * rho0 + static_cast<double>(decode<S, Q>(v, i));
+}
+
+// Stored element from a population f_i in FP64.
+template <int S, int Q>
+__device__ __forceinline__ typename Store<S>::T from_f64(double f, int i, double rho0) {
+  if constexpr (S == kF64)
+    return f;
+  else
+    return encode<S, Q>(static_cast<float>(f - Lattice<Q>::w(i) * rho0), i);
+}
+
+struct StepArgs {
+  const void *fin;
+  void *fout;
+  const void *Ffield;  // (N, d): double for FP64 storage, float otherwise
+  BoxGeom g;
+  CollParams c;
+  Tiling t;
+};
+
+// ------------------------------------------------------------ collision
+// FP64 storage: the reference's formulas in the reference's order of
+// operations, without FMA contraction, so the step reproduces the FP64
+// CPU restatement to the last bit wherever both sum in the same order.
+template <int Q, int COLL, int FM>
+__device__ __forceinline__ void collide_ref(const double (&f)[Q], const double *Fn,
+                                            const CollParams &c, double (&fs)[Q],
+                                            double &rho_out) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  // hydroVariablesUpdate! (gpu/fluids.jl:7-37)
+  double rho = f[0];
+#pragma unroll
+  for (int q = 1; q < Q; q++) rho = rho + f[q];
+  double mom[3];
+  const int cq0[3] = {L::cx[0], L::cy[0], L::cz[0]};
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    double acc = f[0] * static_cast<double>(cq0[k]);
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+      const int ck = k == 0 ? L::cx[q] : (k == 1 ? L::cy[q] : L::cz[q]);
+      acc = acc + f[q] * static_cast<double>(ck);
+    }
+    mom[k] = acc;
+  }
+  double F[3] = {0.0, 0.0, 0.0};
+  if constexpr (FM != LBM_FORCE_NONE) {
+#pragma unroll
+    for (int k = 0; k < D; k++) F[k] = Fn[k];
+    const double half_dt = 0.5 * c.dt;
+#pragma unroll
+    for (int k = 0; k < D; k++) mom[k] = mom[k] + half_dt * F[k];
+  }
+  double u[3];
+#pragma unroll
+  for (int k = 0; k < D; k++) u[k] = rho != 0.0 ? mom[k] / rho : 0.0;
+  rho_out = rho;
+  // equilibriumDistribution (gpu/fluids.jl:39-52)
+  double uu = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) uu = uu + u[k] * u[k];
+  double fe[Q], udc_[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double ci[3] = {L::cx[q] * c.ls, L::cy[q] * c.ls, L::cz[q] * c.ls};
+    double udc = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) udc = udc + u[k] * ci[k];
+    udc_[q] = udc;
+    const double first = udc * c.invC2s + (udc * udc) * (0.5 * c.invC4s);
+    const double second = first - uu * (0.5 * c.invC2s);
+    fe[q] = L::w(q) * ((second + 1.0) * rho);
+  }
+  // collisionStep (gpu/fluids.jl:58-101) with guoForcingTerm (:103-127)
+  const double cb = -c.dt / c.tau;
+  const double cm = -c.dt / c.tau_minus;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    double om;
+    if constexpr (COLL == LBM_COLLISION_BGK) {
+      om = cb * (f[q] - fe[q]);
+    } else {
+      const int o = L::opp[q];
+      const double fp = 0.5 * (f[q] + f[o]), fm = 0.5 * (f[q] - f[o]);
+      const double ep = 0.5 * (fe[q] + fe[o]), em = 0.5 * (fe[q] - fe[o]);
+      om = cb * (fp - ep) + cm * (fm - em);
+    }
+    if constexpr (FM != LBM_FORCE_NONE) {
+      const double ci[3] = {L::cx[q] * c.ls, L::cy[q] * c.ls, L::cz[q] * c.ls};
+      const double udc = udc_[q];
+      double second[3];
+#pragma unroll
+      for (int k = 0; k < D; k++) second[k] = (udc * ci[k]) * c.invC4s;
+      double S;
+      if constexpr (COLL == LBM_COLLISION_BGK) {
+        double inter = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) inter = inter + ((-u[k] + ci[k]) * c.invC2s + second[k]) * F[k];
+        S = L::w(q) * c.dt * (1.0 - c.dt / (2.0 * c.tau)) * inter;
+      } else {
+        double inter = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) inter = inter + ((-u[k]) * c.invC2s + second[k]) * F[k];
+        const double plus = (1.0 - c.dt / (2.0 * c.tau)) * inter;
+        double inter2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) inter2 = inter2 + F[k] * (ci[k] * c.invC2s);
+        const double minus = (1.0 - c.dt / (2.0 * c.tau_minus)) * inter2;
+        S = L::w(q) * c.dt * (plus + minus);
+      }
+      om = om + S;
+    }
+    fs[q] = om + f[q];
+  }
+}
+
+// FP32/FP16 storage: the same collision in deviation variables d_i =
+// f_i - w_i rho0, computed in FP32 (Sum_i w_i = 1, Sum_i w_i c_i = 0 make
+// the shift drop out of every moment).
+template <int Q, int COLL, int FM>
+__device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn,
+                                            const CollParams &c, float (&ds)[Q],
+                                            float &rho_out) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const float ls = static_cast<float>(c.ls);
+  const float invC2s = static_cast<float>(c.invC2s);
+  const float invC4s = static_cast<float>(c.invC4s);
+  float drho = 0.0f;
+#pragma unroll
+  for (int q = 0; q < Q; q++) drho += d[q];
+  float j[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if (L::cx[q]) j[0] += L::cx[q] > 0 ? d[q] : -d[q];
+    if (L::cy[q]) j[1] += L::cy[q] > 0 ? d[q] : -d[q];
+    if (L::cz[q]) j[2] += L::cz[q] > 0 ? d[q] : -d[q];
+  }
+  float F[3] = {0.0f, 0.0f, 0.0f};
+  if constexpr (FM != LBM_FORCE_NONE) {
+#pragma unroll
+    for (int k = 0; k < D; k++) F[k] = Fn[k];
+    const float half_dt = static_cast<float>(0.5 * c.dt);
+#pragma unroll
+    for (int k = 0; k < D; k++) j[k] += half_dt * F[k];
+  }
+  const float rho = static_cast<float>(c.rho0) + drho;
+  const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
+  float u[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
+  rho_out = rho;
+  float uu = 0.0f;
+#pragma unroll
+  for (int k = 0; k < D; k++) uu += u[k] * u[k];
+  const float dt = static_cast<float>(c.dt);
+  const float wp = static_cast<float>(c.dt / c.tau);
+  const float wm = COLL == LBM_COLLISION_TRT ? static_cast<float>(c.dt / c.tau_minus) : wp;
+  const float gp = 1.0f - 0.5f * wp, gm = 1.0f - 0.5f * wm;
+  float uF = 0.0f;
+  if constexpr (FM != LBM_FORCE_NONE) {
+#pragma unroll
+    for (int k = 0; k < D; k++) uF += u[k] * F[k];
+  }
+  const float base = -0.5f * invC2s * uu;  // -u.u/(2 c_s^2)
+  // Rest population (self-conjugate)
+  {
+    const float w0 = static_cast<float>(L::w(0));
+    const float dfe = w0 * (drho + rho * base);
+    float S = 0.0f;
+    if constexpr (FM != LBM_FORCE_NONE) S = w0 * dt * gp * (-uF * invC2s);
+    ds[0] = d[0] - wp * (d[0] - dfe) + S;
+  }
+  // Conjugate pairs (q, q̄) with q < q̄: symmetric and antisymmetric parts.
+#pragma unroll
+  for (int q = 1; q < Q; q++) {
+    const int o = L::opp[q];
+    if (o < q) continue;
+    const float w = static_cast<float>(L::w(q));
+    const float cu = ls * (L::cx[q] * u[0] + L::cy[q] * u[1] + (D == 3 ? L::cz[q] * u[2] : 0.0f));
+    const float esym = w * (drho + rho * (0.5f * invC4s * cu * cu + base));
+    const float eanti = w * rho * invC2s * cu;
+    float Ssym = 0.0f, Santi = 0.0f;
+    if constexpr (FM != LBM_FORCE_NONE) {
+      const float cF =
+          ls * (L::cx[q] * F[0] + L::cy[q] * F[1] + (D == 3 ? L::cz[q] * F[2] : 0.0f));
+      Ssym = w * dt * gp * (-uF * invC2s + cu * cF * invC4s);
+      Santi = w * dt * gm * (cF * invC2s);
+    }
+    const float psym = 0.5f * (d[q] + d[o]) - esym;
+    const float panti = 0.5f * (d[q] - d[o]) - eanti;
+    const float dsym = -wp * psym + Ssym;
+    const float danti = -wm * panti + Santi;
+    ds[q] = d[q] + dsym + danti;
+    ds[o] = d[o] + dsym - danti;
+  }
+}
+
+// ------------------------------------------------------- node geometry
+struct NodeIdx {
+  int x, y, z;
+};
+
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+__device__ __forceinline__ bool is_wall_node(const BoxGeom &g, int x, int y, int z) {
+  return ((g.walled & 1) && (x == 0 || x == g.nx - 1)) ||
+         ((g.walled & 2) && (y == 0 || y == g.ny - 1)) ||
+         ((g.walled & 4) && (z == 0 || z == g.nz - 1));
+}
+
+// -------------------------------------------------------------- step
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
+  using L = Lattice<Q>;
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  constexpr int D = L::D;
+  const BoxGeom &g = a.g;
+  const int tile = xcd_remap(blockIdx.x, a.t.ntiles);
+  const int tpx = tile % a.t.tx;
+  const int tpy = (tile / a.t.tx) % a.t.ty;
+  const int tpz = tile / (a.t.tx * a.t.ty);
+  const int lid = threadIdx.x;
+  const int x = tpx * a.t.bx + lid % a.t.bx;
+  const int y = tpy * a.t.by + (lid / a.t.bx) % a.t.by;
+  const int z = tpz * a.t.bz + lid / (a.t.bx * a.t.by);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const int64_t N = g.N;
+  const int n = x + g.nx * (y + g.ny * z);
+  const T *__restrict__ fin = static_cast<const T *>(a.fin);
+  T *__restrict__ fout = static_cast<T *>(a.fout);
+
+  if (is_wall_node(g, x, y, z)) {
+    // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
+    return;
+  }
+
+  R fs[Q];
+  {
+    R f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
+    R Fn[3] = {0, 0, 0};
+    if constexpr (FM == LBM_FORCE_UNIFORM) {
+#pragma unroll
+      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
+    } else if constexpr (FM == LBM_FORCE_FIELD) {
+      const R *Ff = static_cast<const R *>(a.Ffield);
+#pragma unroll
+      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
+    }
+    R rho;
+    if constexpr (S == kF64)
+      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    else
+      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+  }
+
+  // Periodic neighbour offsets.
+  const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+  const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+  const int ox[3] = {xm, x, xp};
+  const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
+  const int nxy = g.nx * g.ny;
+  const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
+
+  // Flags of fluid nodes next to a wall layer.
+  const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+  const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+  const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+  const bool near = xl | xh | yl | yh | zl | zh;
+
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+    const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
+    bool push = true, own = false;
+    if (near) {
+      // x + c_q inside a wall layer: the wall keeps zero, no push.
+      push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
+               (cz < 0 && zl) || (cz > 0 && zh));
+      // x - c_q inside a wall layer: halfway bounce-back into own slot q
+      // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
+      own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
+            (cz > 0 && zl) || (cz < 0 && zh);
+    }
+    if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
+    if (own) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
+  }
+}
+
+// ------------------------------------------------------------ particles
+struct PartLaunch {
+  int32_t np;
+  int32_t ext[LBM_MAX_PARTICLES][3];   // link box extent per particle (nodes)
+  int32_t tiles[LBM_MAX_PARTICLES][3]; // tiles per dimension per particle
+  int32_t first_block[LBM_MAX_PARTICLES + 1];
+  int32_t lx, ly, lz;                  // link-kernel block shape
+  int32_t semantics;
+  double coeff;                        // dx^d / dt (gpu) or dx^d (cpu)
+  double cutoff;                       // 2 dx
+};
+
+struct LinkArgs {
+  const void *fin;
+  void *fout;
+  const void *Ffield;
+  BoxGeom g;
+  CollParams c;
+  const DevParticle *parts;
+  const double *coords[3];
+  double *partial;  // [block][np][6]
+  PartLaunch pl;
+};
+
+// (X + (-R))_1^2 + ... < r^2 at node (x, y, z) (gpu/kernels.jl:344-353)
+template <int D>
+__device__ __forceinline__ bool solid_at(const double *const coords[3], const double Rg[3],
+                                         double r2, int x, int y, int z) {
+#pragma clang fp contract(off)
+  const double a = coords[0][x] + (-Rg[0]);
+  const double b = coords[1][y] + (-Rg[1]);
+  double s = a * a + b * b;
+  if constexpr (D == 3) {
+    const double c = coords[2][z] + (-Rg[2]);
+    s = s + c * c;
+  }
+  return s < r2;
+}
+
+// Node velocity of particle p at the solid node (x, y, z):
+// getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
+// (cpu/particles.jl:20-43), from the node-velocity snapshot.
+template <int D>
+__device__ __forceinline__ void node_velocity(const double *const coords[3], const DevParticle &p,
+                                              int semantics, double cutoff, int x, int y, int z,
+                                              double (&out)[3]) {
+#pragma clang fp contract(off)
+  const int idx[3] = {x, y, z};
+  double r[3] = {0.0, 0.0, 0.0};
+  if (semantics == LBM_SEMANTICS_GPU) {
+#pragma unroll
+    for (int k = 0; k < D; k++) r[k] = coords[k][idx[k]] + (-p.nR[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; k++) r[k] = coords[k][idx[k]] - p.nR[k];
+  }
+  double nn = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) nn = nn + r[k] * r[k];
+  nn = sqrt(nn);
+  const double *V = p.nV, *W = p.nW, *e = p.ne;
+  double bulk[3];
+  if (semantics == LBM_SEMANTICS_GPU) {
+    if constexpr (D == 2) {
+      bulk[0] = V[0] - W[0] * r[1];
+      bulk[1] = V[1] + W[0] * r[0];
+    } else {
+      bulk[0] = V[0] + W[1] * r[2] - W[2] * r[1];
+      bulk[1] = V[1] - W[0] * r[2] + W[2] * r[0];
+      bulk[2] = V[2] + W[0] * r[1] - W[1] * r[0];
+    }
+    if (nn == 0.0) nn = 1.0;
+    if (p.kind == LBM_PARTICLE_BEAD) {
+#pragma unroll
+      for (int k = 0; k < D; k++) out[k] = bulk[k];
+      return;
+    }
+    const double inv = 1.0 / nn;
+    double unit[3], edr = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) unit[k] = inv * r[k];
+#pragma unroll
+    for (int k = 0; k < D; k++) edr = edr + unit[k] * e[k];
+    const double first = p.B1 + p.B2 * edr;
+#pragma unroll
+    for (int k = 0; k < D; k++) out[k] = first * (edr * unit[k] + (-e[k])) + bulk[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; k++) out[k] = 0.0;
+    // nodes farther than 2 dx from the surface keep zero velocity
+    // (cpu/particles.jl:25); link nodes never are
+    if (fabs(nn - p.radius) > cutoff) return;
+    if constexpr (D == 2) {
+      bulk[0] = V[0] + (0.0 * 0.0 - W[0] * r[1]);
+      bulk[1] = V[1] + (-0.0 * 0.0 + W[0] * r[0]);
+    } else {
+      bulk[0] = V[0] + (W[1] * r[2] - W[2] * r[1]);
+      bulk[1] = V[1] + (-W[0] * r[2] + W[2] * r[0]);
+      bulk[2] = V[2] + (W[0] * r[1] - W[1] * r[0]);
+    }
+    if (p.kind == LBM_PARTICLE_BEAD || nn == 0.0) {
+#pragma unroll
+      for (int k = 0; k < D; k++) out[k] = bulk[k];
+      return;
+    }
+    double fr[3], edr = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) fr[k] = r[k] / nn;
+#pragma unroll
+    for (int k = 0; k < D; k++) edr = edr + e[k] * fr[k];
+    const double first = p.B1 + p.B2 * edr;
+#pragma unroll
+    for (int k = 0; k < D; k++) out[k] = first * (edr * fr[k] - e[k]) + bulk[k];
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Bounce-back, Ladd and momentum exchange at the particle links.  Runs after
+// step_kernel on the same buffers: the pushes step_kernel made into link
+// slots (from solid nodes) are overwritten here with f*_{q̄}(x) plus the
+// Ladd terms, as the reference overwrites `streamed[cIR]` (gpu/fluids.jl:
+// 157-207).  f*(x) is recomputed from the unchanged input buffer with the
+// same collision code, so it equals the value step_kernel pushed.
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
+  using L = Lattice<Q>;
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  constexpr int D = L::D;
+  const BoxGeom &g = a.g;
+  const PartLaunch &pl = a.pl;
+  // which particle box and which tile of it
+  int pb = 0;
+  while (pb + 1 < pl.np && (int)blockIdx.x >= pl.first_block[pb + 1]) pb++;
+  const int tb = blockIdx.x - pl.first_block[pb];
+  const int tpx = tb % pl.tiles[pb][0];
+  const int tpy = (tb / pl.tiles[pb][0]) % pl.tiles[pb][1];
+  const int tpz = tb / (pl.tiles[pb][0] * pl.tiles[pb][1]);
+  const int lid = threadIdx.x;
+  const DevParticle &P = a.parts[pb];
+  const int x = P.lo[0] + tpx * pl.lx + lid % pl.lx;
+  const int y = P.lo[1] + tpy * pl.ly + (lid / pl.lx) % pl.ly;
+  const int z = (D == 3 ? P.lo[2] : 0) + tpz * pl.lz + lid / (pl.lx * pl.ly);
+
+  double acc[LBM_MAX_PARTICLES][6];
+#pragma unroll
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[p][k] = 0.0;
+
+  bool active = x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2]) && x < g.nx &&
+                y < g.ny && z < g.nz && !is_wall_node(g, x, y, z);
+  // a node inside several boxes is handled by the first one
+  for (int p = 0; p < pb && active; p++) {
+    const DevParticle &Pp = a.parts[p];
+    if (x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
+        (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2])))
+      active = false;
+  }
+
+  uint32_t links[LBM_MAX_PARTICLES];
+  uint32_t any = 0;
+  if (active) {
+    for (int p = 0; p < pl.np; p++) {
+      const DevParticle &Pp = a.parts[p];
+      uint32_t m = 0;
+      const bool inbox = x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
+                         (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2]));
+      if (inbox) {
+        const double r2 = Pp.radius * Pp.radius;
+        if (!solid_at<D>(a.coords, Pp.Rg, r2, x, y, z)) {
+#pragma unroll
+          for (int q = 1; q < Q; q++) {
+            const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
+                      sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
+            if (solid_at<D>(a.coords, Pp.Rg, r2, sx, sy, sz)) m |= 1u << q;
+          }
+        }
+      }
+      links[p] = m;
+      any |= m;
+    }
+  }
+
+  if (any) {
+    const int64_t N = g.N;
+    const int n = x + g.nx * (y + g.ny * z);
+    const T *__restrict__ fin = static_cast<const T *>(a.fin);
+    T *__restrict__ fout = static_cast<T *>(a.fout);
+    R fs[Q];
+    R rho;
+    {
+      R f[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
+      R Fn[3] = {0, 0, 0};
+      if constexpr (FM == LBM_FORCE_UNIFORM) {
+#pragma unroll
+        for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
+      } else if constexpr (FM == LBM_FORCE_FIELD) {
+        const R *Ff = static_cast<const R *>(a.Ffield);
+#pragma unroll
+        for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
+      }
+      if constexpr (S == kF64)
+        collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+      else
+        collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    }
+    const double X[3] = {a.coords[0][x], a.coords[1][y], D == 3 ? a.coords[2][z] : 0.0};
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+      if (!((any >> q) & 1)) continue;
+      const int o = L::opp[q];
+      R val = fs[o];
+      const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
+      const double lc = 2.0 * L::w(q) * a.c.invC2s;
+      const double shift = S == kF64 ? 0.0 : 2.0 * L::w(q) * a.c.rho0;
+      const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
+                sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
+      for (int p = 0; p < pl.np; p++) {
+        if (!((links[p] >> q) & 1)) continue;
+        const DevParticle &Pp = a.parts[p];
+        double ub[3] = {0.0, 0.0, 0.0};
+        node_velocity<D>(a.coords, Pp, pl.semantics, pl.cutoff, sx, sy, sz, ub);
+        double udc = 0.0;
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+          for (int k = 0; k < D; k++) udc = udc + ub[k] * ci[k];
+          if constexpr (S == kF64)
+            val = val + lc * static_cast<double>(rho) * udc;
+          else
+            val = val + static_cast<R>(lc * static_cast<double>(rho) * udc);
+        }
+        if (Pp.couple_forces || Pp.couple_torques) {
+#pragma clang fp contract(off)
+          const double s = static_cast<double>(fs[o]) + static_cast<double>(val) + shift;
+          const double sc = a.pl.coeff * s;
+          double arm[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+          for (int k = 0; k < D; k++) arm[k] = X[k] + (-Pp.R[k]);
+          if (Pp.couple_forces) {
+#pragma unroll
+            for (int k = 0; k < D; k++) acc[p][k] -= sc * ci[k];
+          }
+          if (Pp.couple_torques) {
+            if constexpr (D == 2) {
+              acc[p][3] -= sc * (arm[0] * ci[1] - arm[1] * ci[0]);
+            } else {
+              acc[p][3] -= sc * (arm[1] * ci[2] - arm[2] * ci[1]);
+              acc[p][4] -= sc * (-arm[0] * ci[2] + arm[2] * ci[0]);
+              acc[p][5] -= sc * (arm[0] * ci[1] - arm[1] * ci[0]);
+            }
+          }
+        }
+      }
+      fout[q * N + n] = encode<S, Q>(val, q);
+    }
+  }
+
+  // Block reduction of the momentum exchange, one partial per particle.
+  __shared__ double red[8][6];
+  const int wave = lid >> 6, lane = lid & 63;
+  const int nwaves = (blockDim.x + 63) >> 6;
+  for (int p = 0; p < pl.np; p++) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double v = wave_sum(acc[p][k]);
+      if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (lid < 6) {
+      double s = 0.0;
+      for (int w = 0; w < nwaves; w++) s += red[w][lid];
+      a.partial[((int64_t)blockIdx.x * pl.np + p) * 6 + lid] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------- aux kernels
+struct AuxArgs {
+  void *f;              // populations (storage)
+  const double *f64;    // FP64 host-layout buffer (set)
+  double *out64;        // FP64 output (get populations / rho)
+  double *uout;         // FP64 velocity output
+  const void *Ffield;
+  BoxGeom g;
+  CollParams c;
+  double rho_init, u_init[3];
+};
+
+// findInitialConditions (gpu/initMethods.jl:417-438) with uniform rho, u.
+template <int Q, int S>
+__global__ void init_uniform_kernel(AuxArgs a) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  using T = typename Store<S>::T;
+  constexpr int D = L::D;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+  const bool wall = is_wall_node(a.g, x, y, z);
+  const double rho = wall ? 0.0 : a.rho_init;
+  double u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < D; k++) u[k] = wall ? 0.0 : a.u_init[k];
+  if (a.c.force_mode != LBM_FORCE_NONE) {
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      double Fk;
+      if (a.c.force_mode == LBM_FORCE_UNIFORM)
+        Fk = wall ? 0.0 : a.c.F[k];
+      else if constexpr (S == kF64)
+        Fk = static_cast<const double *>(a.Ffield)[k * a.g.N + n];
+      else
+        Fk = static_cast<const float *>(a.Ffield)[k * a.g.N + n];
+      const double ct = rho != 0.0 ? Fk / rho : 0.0;
+      u[k] = u[k] - 0.5 * a.c.dt * ct;
+    }
+  }
+  T *f = static_cast<T *>(a.f);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
+    double udc = 0.0, uu = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) udc = udc + u[k] * ci[k];
+#pragma unroll
+    for (int k = 0; k < D; k++) uu = uu + u[k] * u[k];
+    const double first = udc * a.c.invC2s + (udc * udc) * (0.5 * a.c.invC4s);
+    const double second = first - uu * (0.5 * a.c.invC2s) + 1.0;
+    f[q * a.g.N + n] = from_f64<S, Q>(second * (L::w(q) * rho), q, a.c.rho0);
+  }
+}
+
+template <int Q, int S>
+__global__ void set_populations_kernel(AuxArgs a) {
+  using T = typename Store<S>::T;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  T *f = static_cast<T *>(a.f);
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q * a.g.N + n] = from_f64<S, Q>(a.f64[q * a.g.N + n], q, a.c.rho0);
+}
+
+template <int Q, int S>
+__global__ void get_populations_kernel(AuxArgs a) {
+  using T = typename Store<S>::T;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  const T *f = static_cast<const T *>(a.f);
+  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+  const bool wall = S != kF64 && is_wall_node(a.g, x, y, z);
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+    a.out64[q * a.g.N + n] = wall ? 0.0 : to_f64<S, Q>(f[q * a.g.N + n], q, a.c.rho0);
+}
+
+// hydroVariablesUpdate! (gpu/fluids.jl:7-37) on the current populations,
+// evaluated in FP64 whatever the storage.
+template <int Q, int S>
+__global__ void hydro_kernel(AuxArgs a) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  using T = typename Store<S>::T;
+  constexpr int D = L::D;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  const T *fp = static_cast<const T *>(a.f);
+  double f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = to_f64<S, Q>(fp[q * a.g.N + n], q, a.c.rho0);
+  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+  if (S != kF64 && is_wall_node(a.g, x, y, z)) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = 0.0;
+  }
+  double rho = f[0];
+#pragma unroll
+  for (int q = 1; q < Q; q++) rho = rho + f[q];
+  double mom[3];
+  const int cq0[3] = {L::cx[0], L::cy[0], L::cz[0]};
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    double acc = f[0] * static_cast<double>(cq0[k]);
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+      const int ck = k == 0 ? L::cx[q] : (k == 1 ? L::cy[q] : L::cz[q]);
+      acc = acc + f[q] * static_cast<double>(ck);
+    }
+    mom[k] = acc;
+  }
+  if (a.c.force_mode != LBM_FORCE_NONE) {
+    const double half_dt = 0.5 * a.c.dt;
+    const bool wall = is_wall_node(a.g, x, y, z);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      double Fk;
+      if (a.c.force_mode == LBM_FORCE_UNIFORM)
+        Fk = wall ? 0.0 : a.c.F[k];
+      else if constexpr (S == kF64)
+        Fk = static_cast<const double *>(a.Ffield)[k * a.g.N + n];
+      else
+        Fk = static_cast<const float *>(a.Ffield)[k * a.g.N + n];
+      mom[k] = mom[k] + half_dt * Fk;
+    }
+  }
+  if (a.out64) a.out64[n] = rho;
+  if (a.uout) {
+#pragma unroll
+    for (int k = 0; k < D; k++) a.uout[k * a.g.N + n] = rho != 0.0 ? mom[k] / rho : 0.0;
+  }
+}
+
+// Link bits of one particle over its box (for lbm_get_links):
+// bit j of mask[local node] set when node is fluid and node + c_j solid.
+struct LinkMaskArgs {
+  BoxGeom g;
+  const DevParticle *part;
+  const double *coords[3];
+  uint32_t *mask;
+  int32_t lo[3], ext[3];
+};
+
+template <int Q>
+__global__ void link_mask_kernel(LinkMaskArgs a) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tot = (int64_t)a.ext[0] * a.ext[1] * a.ext[2];
+  if (t >= tot) return;
+  const int x = wrapi(a.lo[0] + (int)(t % a.ext[0]), a.g.nx);
+  const int y = wrapi(a.lo[1] + (int)((t / a.ext[0]) % a.ext[1]), a.g.ny);
+  const int z = D == 3 ? wrapi(a.lo[2] + (int)(t / ((int64_t)a.ext[0] * a.ext[1])), a.g.nz) : 0;
+  const DevParticle &P = *a.part;
+  const double r2 = P.radius * P.radius;
+  uint32_t m = 0;
+  if (!solid_at<D>(a.coords, P.Rg, r2, x, y, z)) {
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+      const int sx = wrapi(x + L::cx[j], a.g.nx), sy = wrapi(y + L::cy[j], a.g.ny),
+                sz = D == 3 ? wrapi(z + L::cz[j], a.g.nz) : 0;
+      if (solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz)) m |= 1u << j;
+    }
+  }
+  a.mask[t] = m;
+}
+
+}  // namespace lbm

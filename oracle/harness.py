@@ -1,0 +1,190 @@
+"""ctypes driver of oracle/liblbm_oracle.so — TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker.  It takes the same resolved model as the hip backend
+(lbmengine_hip.resolve_model / resolve_particle, the host-side restatement
+of modelInit and addSquirmer!) and exposes the same read accessors as
+lbmengine_hip.LBMmodel, so a parity test builds both from one set of
+keyword arguments and compares field by field.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liblbm_oracle.so"
+sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
+
+from lbmengine_hip import _capi  # noqa: E402
+from lbmengine_hip.model import (DEFAULT, ResolvedModel, propagate_ticks, resolve_model,  # noqa: E402
+                                 resolve_particle)
+
+_LIB = None
+_FN: dict = {}
+
+
+def build_oracle() -> Path:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return ORACLE_LIB
+
+
+def load():
+    global _LIB, _FN
+    if _LIB is None:
+        if not ORACLE_LIB.exists():
+            build_oracle()
+        _LIB = C.CDLL(str(ORACLE_LIB))
+        _FN = _capi.bind(_LIB, "lbm_", "lbmo_")
+        fn = _LIB.lbmo_get_solid
+        fn.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]
+        fn.restype = C.c_int
+        _FN["lbm_get_solid"] = fn
+    return _FN
+
+
+def _check(rc):
+    if rc != 0:
+        raise _capi.LBMError(rc, _FN["lbm_last_error"]().decode())
+
+
+class _OParticle:
+    def __init__(self, m: "OracleModel", pid: int):
+        self._m, self.id = m, pid
+
+    def _state(self):
+        p = _capi.LbmParticle()
+        _check(self._m._fn["lbm_get_particle"](self._m._h, self.id, C.byref(p)))
+        return p
+
+    @property
+    def position(self):
+        return np.array(self._state().position[: self._m.dims])
+
+    @property
+    def velocity(self):
+        return np.array(self._state().velocity[: self._m.dims])
+
+    @property
+    def angularVelocity(self):
+        w = self._state().angular_velocity
+        return float(w[0]) if self._m.dims == 2 else np.array(w[:3])
+
+    @property
+    def swimmingDirection(self):
+        return np.array(self._state().swimming_direction[: self._m.dims])
+
+    def load(self):
+        F, T = np.zeros(3), np.zeros(3)
+        _check(self._m._fn["lbm_get_particle_load"](self._m._h, self.id, _capi.as_dptr(F), _capi.as_dptr(T)))
+        return F[: self._m.dims], (T[0] if self._m.dims == 2 else T)
+
+    def links(self):
+        fn, h = self._m._fn, self._m._h
+        cnt = C.c_int64(0)
+        _check(fn["lbm_get_links"](h, self.id, None, None, 0, C.byref(cnt)))
+        node = np.zeros(cnt.value, dtype=np.int64)
+        dirs = np.zeros(cnt.value, dtype=np.int32)
+        _check(fn["lbm_get_links"](h, self.id, node.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   dirs.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value, C.byref(cnt)))
+        return node, dirs
+
+    def solid(self):
+        rm = self._m.resolved
+        out = np.zeros(rm.N, dtype=np.uint8)
+        _check(self._m._fn["lbm_get_solid"](self._m._h, self.id, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out.reshape(tuple(rm.n[: rm.dims]), order="F").astype(bool)
+
+
+class OracleModel:
+    """The oracle run of a resolved model (FP64, CPU)."""
+
+    def __init__(self, rm: ResolvedModel):
+        self._fn = load()
+        self.resolved = rm
+        cfg, keep = rm.config()
+        h = C.c_void_p()
+        _check(self._fn["lbm_create"](C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.particles = []
+        if rm.force_mode == _capi.FORCE_FIELD:
+            F = np.asfortranarray(rm.force_field, dtype=np.float64)
+            _check(self._fn["lbm_set_force_field"](h, _capi.as_dptr(F)))
+        if rm.uniform_init():
+            u = np.zeros(3)
+            u[: rm.dims] = rm.init_u
+            _check(self._fn["lbm_init_uniform"](h, float(rm.init_rho), _capi.as_dptr(u)))
+        else:
+            f0 = np.asfortranarray(rm.initial_populations())
+            _check(self._fn["lbm_set_populations"](h, _capi.as_dptr(f0)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._fn["lbm_destroy"](h)
+            self._h = None
+
+    @property
+    def dims(self):
+        return self.resolved.dims
+
+    def add(self, kind: str, **kw):
+        p = resolve_particle(self.resolved, kind=kind, **kw)
+        pid = C.c_int32()
+        _check(self._fn["lbm_add_particle"](self._h, C.byref(p), C.byref(pid)))
+        self.particles.append(_OParticle(self, pid.value))
+
+    def run(self, ticks: int):
+        _check(self._fn["lbm_run"](self._h, int(ticks)))
+
+    @property
+    def tick(self):
+        t, tm = C.c_int64(), C.c_double()
+        _check(self._fn["lbm_get_tick"](self._h, C.byref(t), C.byref(tm)))
+        return t.value
+
+    @property
+    def time(self):
+        t, tm = C.c_int64(), C.c_double()
+        _check(self._fn["lbm_get_tick"](self._h, C.byref(t), C.byref(tm)))
+        return tm.value
+
+    @property
+    def distributions(self):
+        rm = self.resolved
+        f = np.zeros(tuple(rm.n[: rm.dims]) + (rm.Q,), order="F")
+        _check(self._fn["lbm_get_populations"](self._h, _capi.as_dptr(f)))
+        return f
+
+    @distributions.setter
+    def distributions(self, f):
+        f = _capi.fortran_f64(f)
+        _check(self._fn["lbm_set_populations"](self._h, _capi.as_dptr(f)))
+
+    def _hydro(self):
+        rm = self.resolved
+        shp = tuple(rm.n[: rm.dims])
+        rho = np.zeros(shp, order="F")
+        u = np.zeros(shp + (rm.dims,), order="F")
+        _check(self._fn["lbm_get_hydro"](self._h, _capi.as_dptr(rho), _capi.as_dptr(u)))
+        return rho, u
+
+    @property
+    def massDensity(self):
+        return self._hydro()[0]
+
+    @property
+    def fluidVelocity(self):
+        return self._hydro()[1]
+
+
+def oracle_model(**kw) -> OracleModel:
+    return OracleModel(resolve_model(**kw))
+
+
+__all__ = ["OracleModel", "oracle_model", "build_oracle", "DEFAULT", "propagate_ticks"]
