@@ -129,7 +129,7 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   a.g = geom(h);
   a.c = coll(h);
   a.t = h->tiling;
-  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx * h->tiling.by * h->tiling.bz), h->stream};
+  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
   hipError_t e;
   if (e0) hipEventRecord(e0, h->stream);
   switch (h->cfg.storage) {
@@ -166,6 +166,8 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   ia.semantics = h->cfg.semantics;
   ia.dt = h->cfg.dt;
   ia.dx = h->cfg.dx;
+  ia.coeff = h->pl.coeff;
+  ia.ls = h->cfg.dx / h->cfg.dt;
   for (int k = 0; k < 3; k++) {
     ia.x0[k] = h->x0[k];
     ia.n[k] = h->n[k];
@@ -238,7 +240,7 @@ int refresh_particle_launch(lbm_handle *h) {
   }
   pl.first_block[np] = blocks;
   h->link_blocks = blocks;
-  const size_t need = (size_t)std::max(blocks, 1) * std::max(np, 1) * 6;
+  const size_t need = (size_t)std::max(blocks, 1) * std::max(np, 1) * 6 * 2;  // double-double
   if (need > h->partial_cap) {
     if (h->partial) hipFree(h->partial);
     h->partial = nullptr;

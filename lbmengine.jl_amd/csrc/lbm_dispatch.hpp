@@ -18,6 +18,8 @@ struct IntegrateArgs {
   int32_t np, nblocks;
   int32_t dims, semantics;
   double dt, dx;
+  double coeff;  // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
+  double ls;     // lattice speed dx / dt
   double x0[3];
   int32_t n[3];
 };

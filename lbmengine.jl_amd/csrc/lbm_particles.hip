@@ -96,20 +96,30 @@ __device__ static void move_cpu(DevParticle &p, const double dP[3], const double
 }
 
 __global__ void __launch_bounds__(256) integrate_kernel(IntegrateArgs a) {
-  __shared__ double red[256];
+  __shared__ DD red[256];
   __shared__ double tot[LBM_MAX_PARTICLES][6];
   const int t = threadIdx.x;
   for (int p = 0; p < a.np; p++) {
     for (int k = 0; k < 6; k++) {
-      double s = 0.0;
-      for (int b = t; b < a.nblocks; b += blockDim.x) s += a.partial[((int64_t)b * a.np + p) * 6 + k];
+      // fixed-order double-double reduction of the link-kernel partials
+      DD s{0.0, 0.0};
+      for (int b = t; b < a.nblocks; b += blockDim.x) {
+        const double *src = a.partial + (((int64_t)b * a.np + p) * 6 + k) * 2;
+        s = dd_merge(s, DD{src[0], src[1]});
+      }
       red[t] = s;
       __syncthreads();
       for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
-        if (t < w) red[t] += red[t + w];
+        if (t < w) red[t] = dd_merge(red[t], red[t + w]);
         __syncthreads();
       }
-      if (t == 0) tot[p][k] = red[0];
+      if (t == 0) {
+#pragma clang fp contract(off)
+        // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
+        // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
+        const double v = red[0].hi + red[0].lo;
+        tot[p][k] = -a.coeff * (v * a.ls);
+      }
       __syncthreads();
     }
   }
@@ -117,7 +127,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(IntegrateArgs a) {
   for (int p = 0; p < a.np; p++) {
     DevParticle &P = a.parts[p];
     const double F[3] = {tot[p][0], tot[p][1], a.dims == 3 ? tot[p][2] : 0.0};
-    const double T[3] = {tot[p][3], tot[p][4], tot[p][5]};
+    const double T[3] = {tot[p][3], a.dims == 3 ? tot[p][4] : 0.0, a.dims == 3 ? tot[p][5] : 0.0};
     bool moved, find_nv;
     if (a.semantics == LBM_SEMANTICS_GPU)
       move_gpu(P, F, T, a.dims, a.dt, moved, find_nv);

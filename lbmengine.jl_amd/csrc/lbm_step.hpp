@@ -292,10 +292,9 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
   const int tpx = tile % a.t.tx;
   const int tpy = (tile / a.t.tx) % a.t.ty;
   const int tpz = tile / (a.t.tx * a.t.ty);
-  const int lid = threadIdx.x;
-  const int x = tpx * a.t.bx + lid % a.t.bx;
-  const int y = tpy * a.t.by + (lid / a.t.bx) % a.t.by;
-  const int z = tpz * a.t.bz + lid / (a.t.bx * a.t.by);
+  const int x = tpx * a.t.bx + threadIdx.x;
+  const int y = tpy * a.t.by + threadIdx.y;
+  const int z = tpz * a.t.bz + threadIdx.z;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
   const int64_t N = g.N;
   const int n = x + g.nx * (y + g.ny * z);
@@ -346,12 +345,21 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
   const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
   const bool near = xl | xh | yl | yh | zl | zh;
 
+  if (!near) {
+    // away from walls every population streams: no per-direction branches
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1];
+      fout[q * N + t] = encode<S, Q>(fs[q], q);
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
     const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
     bool push = true, own = false;
-    if (near) {
+    {
       // x + c_q inside a wall layer: the wall keeps zero, no push.
       push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
                (cz < 0 && zl) || (cz > 0 && zh));
@@ -481,9 +489,36 @@ __device__ __forceinline__ void node_velocity(const double *const coords[3], con
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Double-double accumulation (Knuth's TwoSum) of the momentum-exchange sums:
+// they are evaluated essentially exactly and rounded once, so the result does
+// not depend on the order in which links, waves and blocks are combined.
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ void dd_add(DD &a, double x) {
+#pragma clang fp contract(off)
+  const double s = a.hi + x;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (x - bb);
+  a.hi = s;
+  a.lo = a.lo + err;
+}
+
+__device__ __forceinline__ DD dd_merge(DD a, DD b) {
+#pragma clang fp contract(off)
+  DD r{a.hi, a.lo + b.lo};
+  dd_add(r, b.hi);
+  return r;
+}
+
+__device__ __forceinline__ DD wave_sum(DD v) {
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  for (int off = 32; off >= 1; off >>= 1) {
+    DD o{__shfl_xor(v.hi, off), __shfl_xor(v.lo, off)};
+    // both lanes of a pair must form the same sum: order the operands
+    v = (threadIdx.x & off) ? dd_merge(o, v) : dd_merge(v, o);
+  }
   return v;
 }
 
@@ -514,11 +549,12 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
   const int y = P.lo[1] + tpy * pl.ly + (lid / pl.lx) % pl.ly;
   const int z = (D == 3 ? P.lo[2] : 0) + tpz * pl.lz + lid / (pl.lx * pl.ly);
 
-  double acc[LBM_MAX_PARTICLES][6];
+  // per particle: sum of s c_q (force) and of (s (x - R)) x c_q (torque)
+  DD acc[LBM_MAX_PARTICLES][6];
 #pragma unroll
   for (int p = 0; p < LBM_MAX_PARTICLES; p++)
 #pragma unroll
-    for (int k = 0; k < 6; k++) acc[p][k] = 0.0;
+    for (int k = 0; k < 6; k++) acc[p][k] = DD{0.0, 0.0};
 
   bool active = x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2]) && x < g.nx &&
                 y < g.ny && z < g.nz && !is_wall_node(g, x, y, z);
@@ -607,22 +643,30 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
         }
         if (Pp.couple_forces || Pp.couple_torques) {
 #pragma clang fp contract(off)
+          // s = f*_{q̄}(x) + f_q(x) (gpu/fluids.jl:193); the loads are
+          // -coeff * sum(s) * c_q and -coeff * cross(sum(s (x - R)), c_q),
+          // formed from these exact sums by the integrator
           const double s = static_cast<double>(fs[o]) + static_cast<double>(val) + shift;
-          const double sc = a.pl.coeff * s;
-          double arm[3] = {0.0, 0.0, 0.0};
+          const int c0 = L::cx[q], c1 = L::cy[q], c2 = L::cz[q];
+          double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-          for (int k = 0; k < D; k++) arm[k] = X[k] + (-Pp.R[k]);
+          for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-Pp.R[k]));
           if (Pp.couple_forces) {
-#pragma unroll
-            for (int k = 0; k < D; k++) acc[p][k] -= sc * ci[k];
+            if (c0) dd_add(acc[p][0], c0 > 0 ? s : -s);
+            if (c1) dd_add(acc[p][1], c1 > 0 ? s : -s);
+            if (D == 3 && c2) dd_add(acc[p][2], c2 > 0 ? s : -s);
           }
           if (Pp.couple_torques) {
             if constexpr (D == 2) {
-              acc[p][3] -= sc * (arm[0] * ci[1] - arm[1] * ci[0]);
+              if (c1) dd_add(acc[p][3], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) dd_add(acc[p][3], c0 > 0 ? -pr[1] : pr[1]);
             } else {
-              acc[p][3] -= sc * (arm[1] * ci[2] - arm[2] * ci[1]);
-              acc[p][4] -= sc * (-arm[0] * ci[2] + arm[2] * ci[0]);
-              acc[p][5] -= sc * (arm[0] * ci[1] - arm[1] * ci[0]);
+              if (c2) dd_add(acc[p][3], c2 > 0 ? pr[1] : -pr[1]);
+              if (c1) dd_add(acc[p][3], c1 > 0 ? -pr[2] : pr[2]);
+              if (c2) dd_add(acc[p][4], c2 > 0 ? -pr[0] : pr[0]);
+              if (c0) dd_add(acc[p][4], c0 > 0 ? pr[2] : -pr[2]);
+              if (c1) dd_add(acc[p][5], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) dd_add(acc[p][5], c0 > 0 ? -pr[1] : pr[1]);
             }
           }
         }
@@ -631,21 +675,24 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
     }
   }
 
-  // Block reduction of the momentum exchange, one partial per particle.
-  __shared__ double red[8][6];
+  // Block reduction of the momentum exchange, one double-double partial per
+  // particle and component: [block][np][6][hi, lo].
+  __shared__ DD red[8][6];
   const int wave = lid >> 6, lane = lid & 63;
   const int nwaves = (blockDim.x + 63) >> 6;
   for (int p = 0; p < pl.np; p++) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      const double v = wave_sum(acc[p][k]);
+      const DD v = wave_sum(acc[p][k]);
       if (lane == 0) red[wave][k] = v;
     }
     __syncthreads();
     if (lid < 6) {
-      double s = 0.0;
-      for (int w = 0; w < nwaves; w++) s += red[w][lid];
-      a.partial[((int64_t)blockIdx.x * pl.np + p) * 6 + lid] = s;
+      DD s = red[0][lid];
+      for (int w = 1; w < nwaves; w++) s = dd_merge(s, red[w][lid]);
+      double *dst = a.partial + (((int64_t)blockIdx.x * pl.np + p) * 6 + lid) * 2;
+      dst[0] = s.hi;
+      dst[1] = s.lo;
     }
     __syncthreads();
   }

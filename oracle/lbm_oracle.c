@@ -53,6 +53,23 @@ static int fail(int code, const char *fmt, ...) {
 
 const char *lbmo_last_error(void) { return g_err; }
 
+/* Double-double accumulation (Knuth's TwoSum): the momentum-exchange sums are
+ * evaluated essentially exactly and rounded once, so their value does not
+ * depend on the order in which links are visited. */
+typedef struct {
+  double hi, lo;
+} dd_t;
+
+static inline void dd_add(dd_t *a, double x) {
+  double s = a->hi + x;
+  double bb = s - a->hi;
+  double err = (a->hi - (s - bb)) + (x - bb);
+  a->hi = s;
+  a->lo = a->lo + err;
+}
+
+static inline double dd_value(dd_t a) { return a.hi + a.lo; }
+
 typedef struct {
   lbm_particle p;      /* parameters and current state                 */
   uint8_t *solid;      /* particle.boundaryConditionsParams.solidRegion */
@@ -63,6 +80,7 @@ typedef struct {
   double mom[OMAXQ + 1][3];  /* momentumInput per q (cpu)              */
   double amom[OMAXQ + 1][3]; /* angularMomentumInput per q (cpu)       */
   double last_force[3], last_torque[3];
+  dd_t fsum[3], tsum[3]; /* sum over links of s c_i and of (s (x - R)) x c_i */
 } opart;
 
 typedef struct lbmo_model {
@@ -537,34 +555,53 @@ static void tick(lbmo_model *m) {
         st[x] = st[x] + lc * m->rho[x] * udc;
       }
       if (!(pp->p.couple_forces || pp->p.couple_torques)) continue;
-      double ssum = 0.0, aux[3] = {0, 0, 0};
+      /* forceInput -= coeff * sum(s) * c_q and torqueInput -= coeff *
+       * cross(sum(s (x - R)), c_q) (gpu/fluids.jl:193-204): the sums over
+       * links (and over q) are accumulated in double-double; the products
+       * s * (x - R)_k are rounded once each, as in the reference. */
       for (int64_t x = 0; x < m->N; x++) {
         if (!cbn[x]) continue;
-        double s = fso[x] + st[x];
-        ssum = ssum + s;
-        for (int k = 0; k < d; k++)
-          aux[k] = aux[k] + s * (m->X[(int64_t)k * m->N + x] + (-pp->p.position[k]));
+        const double s = fso[x] + st[x];
+        double p[3] = {0, 0, 0};
+        for (int k = 0; k < d; k++) p[k] = s * (m->X[(int64_t)k * m->N + x] + (-pp->p.position[k]));
+        if (pp->p.couple_forces)
+          for (int k = 0; k < d; k++)
+            if (m->c[q][k]) dd_add(&pp->fsum[k], m->c[q][k] > 0 ? s : -s);
+        if (pp->p.couple_torques) {
+          /* cross(p, c): (p1 c2 - p2 c1, -p0 c2 + p2 c0, p0 c1 - p1 c0); in 2D
+           * the scalar p0 c1 - p1 c0 */
+          const int *c = m->c[q];
+          if (d == 2) {
+            if (c[1]) dd_add(&pp->tsum[0], c[1] > 0 ? p[0] : -p[0]);
+            if (c[0]) dd_add(&pp->tsum[0], c[0] > 0 ? -p[1] : p[1]);
+          } else {
+            if (c[2]) dd_add(&pp->tsum[0], c[2] > 0 ? p[1] : -p[1]);
+            if (c[1]) dd_add(&pp->tsum[0], c[1] > 0 ? -p[2] : p[2]);
+            if (c[2]) dd_add(&pp->tsum[1], c[2] > 0 ? -p[0] : p[0]);
+            if (c[0]) dd_add(&pp->tsum[1], c[0] > 0 ? p[2] : -p[2]);
+            if (c[1]) dd_add(&pp->tsum[2], c[1] > 0 ? p[0] : -p[0]);
+            if (c[0]) dd_add(&pp->tsum[2], c[0] > 0 ? -p[1] : p[1]);
+          }
+        }
       }
-      double tq[3] = {0, 0, 0};
-      if (d == 2) {
-        tq[0] = aux[0] * ci[1] - aux[1] * ci[0];
-      } else {
-        tq[0] = aux[1] * ci[2] - aux[2] * ci[1];
-        tq[1] = -aux[0] * ci[2] + aux[2] * ci[0];
-        tq[2] = aux[0] * ci[1] - aux[1] * ci[0];
-      }
-      const int nw = d == 2 ? 1 : 3;
+    }
+  }
+  /* round the exact sums once: load = -coeff * sum (ci = c * dx/dt) */
+  for (int p = 0; p < m->np; p++) {
+    opart *pp = &m->parts[p];
+    const int nw = d == 2 ? 1 : 3;
+    for (int k = 0; k < 3; k++) {
+      const double F = k < d ? -coeff * (dd_value(pp->fsum[k]) * m->ls) : 0.0;
+      const double T = k < nw ? -coeff * (dd_value(pp->tsum[k]) * m->ls) : 0.0;
       if (gpu) {
-        if (pp->p.couple_forces)
-          for (int k = 0; k < d; k++) pp->force[k] = pp->force[k] - coeff * ssum * ci[k];
-        if (pp->p.couple_torques)
-          for (int k = 0; k < nw; k++) pp->torque[k] = pp->torque[k] - coeff * tq[k];
+        if (pp->p.couple_forces) pp->force[k] = F;
+        if (pp->p.couple_torques) pp->torque[k] = T;
       } else {
-        if (pp->p.couple_forces)
-          for (int k = 0; k < d; k++) pp->mom[q][k] = pp->mom[q][k] - coeff * ssum * ci[k];
-        if (pp->p.couple_torques)
-          for (int k = 0; k < nw; k++) pp->amom[q][k] = pp->amom[q][k] - coeff * tq[k];
+        if (pp->p.couple_forces) pp->mom[0][k] = F;
+        if (pp->p.couple_torques) pp->amom[0][k] = T;
       }
+      pp->fsum[k].hi = pp->fsum[k].lo = 0.0;
+      pp->tsum[k].hi = pp->tsum[k].lo = 0.0;
     }
   }
   double *t = m->f;
