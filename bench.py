@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--block", default="", help="bx,by,bz of the step kernel instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -112,14 +113,28 @@ def main():
     Q = 19
 
     m = build_model(lh, args.storage, args)
+    if args.block:
+        m.set_block(*[int(v) for v in args.block.split(",")])
+        tuned_ms = float("nan")
+    else:
+        block, tuned_ms = m.autotune(20)
     wall, kms, sms = timed(lh, m, args.steps, args.warmup)
     mlups = nodes * args.steps / wall / 1e6
+    # the same run length again from captured hipGraphs (no per-kernel events)
+    m.sync()
+    t0 = time.perf_counter()
+    m.run(args.steps)
+    m.sync()
+    graph_mlups = nodes * args.steps / (time.perf_counter() - t0) / 1e6
     bytes_per_launch = 2 * Q * BYTES[args.storage] * nodes
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_step_{args.storage}_{args.n}.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    # achievable ceiling: a plain device copy moving the same bytes per launch
+    copy_ms = lh._capi.measure_copy_ms(bytes_per_launch // 2, 10)
+    copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
@@ -127,11 +142,13 @@ def main():
         "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
-                   "nodes": nodes, "block": list(m.get_block())},
+                   "nodes": nodes, "block": list(m.get_block()), "autotuned_kernel_ms": round(tuned_ms, 5)},
+        "graph_MLUPS": round(graph_mlups, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch,
+                     "copy_ceiling_GBs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
     }
     del m
 
@@ -142,6 +159,7 @@ def main():
                 var[st] = {"MLUPS": out["value"], "kernel_ms": out["roofline"]["kernel_ms"]}
                 continue
             mv = build_model(lh, st, args)
+            mv.autotune(10)
             w, k, s = timed(lh, mv, max(args.steps // 4, 100), args.warmup // 4)
             var[st] = {"MLUPS": round(nodes * max(args.steps // 4, 100) / w / 1e6, 1), "kernel_ms": round(k, 5),
                        "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

@@ -167,6 +167,18 @@ int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz);
 /* Current block shape of the step kernel. */
 int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz);
 
+/* Autotuner: times the stream-collide kernel over candidate block shapes
+ * (reading the current populations, writing a scratch buffer, so the state
+ * is untouched), keeps the fastest and returns it with its time per launch.
+ * No counterpart in the reference (CUDA.jl launches 8x8x8 blocks,
+ * gpu/aux.jl:12-21). */
+int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms);
+
+/* lbm_run replays captured hipGraphs of `ticks_per_graph` ticks (even) and
+ * launches the remainder directly; 0 launches every tick directly.  Default
+ * 32.  The result is the same either way. */
+int lbm_set_graph(lbm_handle *h, int32_t ticks_per_graph);
+
 /* Measurement (no counterpart in the reference, which has no timing):
  * with timing on, lbm_run records HIP events around every stream-collide
  * kernel on the handle's stream, waits for the run and keeps the averages. */
@@ -179,6 +191,12 @@ int lbm_last_run_ms(lbm_handle *h, double *ms);
 /* Averages of the last timed lbm_run: stream-collide kernel time and whole
  * step time (all kernels of a tick), in ms.  Either pointer may be NULL. */
 int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms);
+
+/* Measurement: average time (ms) of a plain device-to-device copy of `bytes`
+ * bytes (a multiple of 16), over `reps` copies — the achievable HBM
+ * bandwidth beside the step, and a kernel of known traffic for scaling
+ * profiler byte counters.  No counterpart in the reference. */
+int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms);
 
 #ifdef __cplusplus
 }

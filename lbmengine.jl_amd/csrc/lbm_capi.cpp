@@ -56,6 +56,7 @@ struct lbm_handle {
   std::vector<DevParticle> parts_h;  // parameters as added
   double *partial = nullptr;
   size_t partial_cap = 0;
+  uint32_t *ticket = nullptr;  // link-kernel blocks finished in the current step
   PartLaunch pl{};
   int link_blocks = 0;
   Tiling tiling{};
@@ -69,9 +70,9 @@ struct lbm_handle {
   hipEvent_t run_start = nullptr, run_stop = nullptr;
   int64_t timed_steps = 0;
   double step_kernel_ms = 0.0, step_ms = 0.0;
-  // graph of an even number of steps starting from buffer 0
-  hipGraphExec_t graph = nullptr;
-  int64_t graph_steps = 0;
+  // captured blocks of graph_k ticks, one per starting ping-pong buffer
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int64_t graph_k = 32;
 };
 
 namespace {
@@ -149,15 +150,9 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   la.parts = h->parts_d;
   for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
   la.partial = h->partial;
+  la.ticket = h->ticket;
   la.pl = h->pl;
-  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
-  switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
-    case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
-    default: e = launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
-  }
-  if (e != hipSuccess) return e;
-  IntegrateArgs ia;
+  IntegrateArgs &ia = la.ia;
   ia.parts = h->parts_d;
   ia.partial = h->partial;
   ia.np = (int32_t)h->parts_h.size();
@@ -172,13 +167,40 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
     ia.x0[k] = h->x0[k];
     ia.n[k] = h->n[k];
   }
-  return launch_integrate(ia, h->stream);
+  // one launch: links of every particle box; its last block integrates
+  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: return launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
+    case LBM_STORAGE_F32: return launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
+    default: return launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
+  }
 }
 
 void drop_graph(lbm_handle *h) {
-  if (h->graph) hipGraphExecDestroy(h->graph);
-  h->graph = nullptr;
-  h->graph_steps = 0;
+  for (int b = 0; b < 2; b++) {
+    if (h->graph_exec[b]) hipGraphExecDestroy(h->graph_exec[b]);
+    h->graph_exec[b] = nullptr;
+  }
+}
+
+// Capture k ticks starting from buffer `start` into an executable graph.  The
+// kernels take every pointer by value and read the particle state from
+// device memory, so the graph replays correctly as long as the handle's
+// buffers, particles and block shape stay as they are (drop_graph otherwise).
+int capture_graph(lbm_handle *h, int start, int64_t k, hipGraphExec_t *out) {
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  hipError_t e = hipSuccess;
+  for (int64_t i = 0; i < k && e == hipSuccess; i++) e = step_once(h, start ^ (int)(i & 1), nullptr, nullptr);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(e != hipSuccess ? e : e2, "capture step graph");
+  }
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "instantiate step graph");
+  return LBM_OK;
 }
 
 int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
@@ -225,6 +247,7 @@ int refresh_particle_launch(lbm_handle *h) {
   if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
   pl.coeff = coeff;
   pl.cutoff = 2.0 * h->cfg.dx;
+  pl.shell2 = (1.5 * h->cfg.dx) * (1.5 * h->cfg.dx);
   const int32_t bl[3] = {pl.lx, pl.ly, pl.lz};
   int blocks = 0;
   for (int p = 0; p < np; p++) {
@@ -290,6 +313,7 @@ int lbm_destroy(lbm_handle *h) {
   if (h->F) hipFree(h->F);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
+  if (h->ticket) hipFree(h->ticket);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -364,6 +388,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   }
   if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
+  if (hipMalloc(&h->ticket, sizeof(uint32_t)) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc ticket"));
+  if (hipMemset(h->ticket, 0, sizeof(uint32_t)) != hipSuccess) return bail(fail(LBM_EDEVICE, "clear ticket"));
   h->tiling = default_tiling(h);
   if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
     return bail(fail(LBM_EDEVICE, "hipEventCreate"));
@@ -619,9 +645,25 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
     h->timed_steps = nsteps;
   }
   HIP_TRY(hipEventRecord(h->run_start, h->stream), "event record");
-  for (int64_t s = 0; s < nsteps; s++) {
-    hipEvent_t e0 = h->timing ? h->ev[2 * s] : nullptr;
-    hipEvent_t e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
+  int64_t s = 0;
+  if (!h->timing && h->graph_k > 0) {
+    // replay captured blocks of graph_k ticks (graph_k is even, so a block
+    // leaves the ping-pong parity where it found it)
+    while (nsteps - s >= h->graph_k) {
+      hipGraphExec_t &g = h->graph_exec[h->cur];
+      if (!g) {
+        int rc = capture_graph(h, h->cur, h->graph_k, &g);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(g, h->stream), "graph launch");
+      for (int64_t k = 0; k < h->graph_k; k++) h->time += h->cfg.dt;
+      h->tick += h->graph_k;
+      s += h->graph_k;
+    }
+  }
+  for (int64_t i = 0; s < nsteps; s++, i++) {
+    hipEvent_t e0 = h->timing ? h->ev[2 * i] : nullptr;
+    hipEvent_t e1 = h->timing ? h->ev[2 * i + 1] : nullptr;
     hipError_t e = step_once(h, h->cur, e0, e1);
     if (e != hipSuccess) return hip_fail(e, "step launch");
     h->cur ^= 1;
@@ -692,6 +734,137 @@ int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms) {
   if (!h) return fail(LBM_EINVAL, "null handle");
   if (kernel_ms) *kernel_ms = h->step_kernel_ms;
   if (step_ms) *step_ms = h->step_ms;
+  return LBM_OK;
+}
+
+// Ticks per captured graph (even, >= 2), or 0 to launch every tick directly.
+int lbm_set_graph(lbm_handle *h, int32_t ticks_per_graph) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (ticks_per_graph < 0 || ticks_per_graph == 1 || (ticks_per_graph & 1))
+    return fail(LBM_EINVAL, "ticks per graph must be 0 or an even number");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  drop_graph(h);
+  h->graph_k = ticks_per_graph;
+  return LBM_OK;
+}
+
+// Autotuner: time the stream-collide kernel of this lattice over candidate
+// block shapes (reading the current populations, writing a scratch buffer,
+// so the simulation state is untouched) and keep the fastest.
+int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
+  if (reps < 1) reps = 1;
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  std::vector<int32_t> bxs = {16, 32, 64, 96, 128, 192, 256, 512};
+  if (h->n[0] <= 512) bxs.push_back(h->n[0]);
+  std::sort(bxs.begin(), bxs.end());
+  bxs.erase(std::unique(bxs.begin(), bxs.end()), bxs.end());
+  struct Cand {
+    int32_t bx, by, bz;
+  };
+  std::vector<Cand> cands;
+  for (int32_t bx : bxs)
+    for (int32_t by : {1, 2, 4, 8, 16})
+      for (int32_t bz : {1, 2, 4}) {
+        const int64_t t = (int64_t)bx * by * bz;
+        if (t < 64 || t > 512 || bx > h->n[0] || by > h->n[1] || bz > h->n[2]) continue;
+        const Tiling tl = make_tiling(h, bx, by, bz);
+        const double waste = (double)tl.ntiles * t / (double)h->N - 1.0;
+        if (waste > 0.25) continue;
+        cands.push_back({bx, by, bz});
+      }
+  if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits this lattice");
+  const size_t bytes = (size_t)h->N * h->Q * elem_size(h->cfg.storage);
+  void *scratch = nullptr;
+  HIP_TRY(hipMalloc(&scratch, bytes), "hipMalloc autotune scratch");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const Tiling saved = h->tiling;
+  double best_t = 1e300;
+  Cand best_c = cands[0];
+  hipError_t e = hipSuccess;
+  for (const Cand &c : cands) {
+    h->tiling = make_tiling(h, c.bx, c.by, c.bz);
+    StepArgs a;
+    a.fin = h->f[h->cur];
+    a.fout = scratch;
+    a.Ffield = h->F;
+    a.g = geom(h);
+    a.c = coll(h);
+    a.t = h->tiling;
+    Launch l{dim3(h->tiling.ntiles), dim3(c.bx, c.by, c.bz), h->stream};
+    for (int r = 0; r < 2 && e == hipSuccess; r++) {  // warm-up
+      switch (h->cfg.storage) {
+        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+      }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+      switch (h->cfg.storage) {
+        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+      }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    if (ms / reps < best_t) {
+      best_t = ms / reps;
+      best_c = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(scratch);
+  if (e != hipSuccess) {
+    h->tiling = saved;
+    return hip_fail(e, "autotune");
+  }
+  h->tiling = make_tiling(h, best_c.bx, best_c.by, best_c.bz);
+  drop_graph(h);
+  if (best) {
+    best[0] = best_c.bx;
+    best[1] = best_c.by;
+    best[2] = best_c.bz;
+  }
+  if (best_ms) *best_ms = best_t;
+  return LBM_OK;
+}
+
+// Device-to-device copy of `bytes` bytes (multiple of 16), `reps` times on a
+// private stream; *ms receives the average time per copy (HIP events).
+int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms) {
+  if (!ms || bytes < 16 || bytes % 16 || reps < 1) return fail(LBM_EINVAL, "bad copy size or repetitions");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMemset(a, 1, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy(a, b, bytes, s);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(a, b, bytes, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  if (e != hipSuccess) return hip_fail(e, "copy measurement");
+  *ms = t / reps;
   return LBM_OK;
 }
 

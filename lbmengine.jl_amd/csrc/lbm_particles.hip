@@ -1,161 +1,23 @@
-// lbm_particles.hip — particle side of the step: the fixed-order FP64
-// reduction of the momentum exchange and moveParticles!
-// (gpu/particles.jl:7-92, cpu/particles.jl:45-100), plus the link-bit
-// kernel used to list a particle's boundary links.
+// lbm_particles.hip — the link-bit kernel used to list a particle's
+// boundary links (lbm_get_links).
 #include "lbm_dispatch.hpp"
 
 namespace lbm {
 
-__device__ static inline bool any_nonzero(const double *v, int n) {
-  for (int k = 0; k < n; k++)
-    if (v[k] != 0.0) return true;
-  return false;
+// Plain device copy of n16 16-byte words: the measured HBM ceiling, and the
+// kernel of known traffic (n16*16 bytes read and written) that scales the
+// byte counters of the profiler.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                   int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
-// moveParticles!, gpu/particles.jl:7-92.  The half-kick branch
-// (model.tick == 0) is unreachable there because tick! increments the tick
-// first (gpu/fluids.jl:217), so only the full kick is restated.
-__device__ static void move_gpu(DevParticle &p, const double F[3], const double T[3], int dims,
-                                double dt, bool &moved, bool &find_nv) {
-#pragma clang fp contract(off)
-  moved = false;
-  find_nv = false;
-  if (p.couple_forces) {
-    double A[3];
-    for (int k = 0; k < dims; k++) A[k] = F[k] * p.inv_mass;
-    for (int k = 0; k < dims; k++) p.V[k] = p.V[k] + dt * A[k];
-    for (int k = 0; k < dims; k++) p.R[k] = p.R[k] + dt * p.V[k];
-    moved = any_nonzero(p.V, dims);
-  }
-  if (p.couple_torques) {
-    const int nw = dims == 2 ? 1 : 3;
-    double Al[3];
-    for (int k = 0; k < nw; k++) Al[k] = T[k] * p.inv_I;
-    for (int k = 0; k < nw; k++) p.W[k] = p.W[k] + dt * Al[k];
-    if (p.kind == LBM_PARTICLE_SQUIRMER) {
-      // Rodrigues' rotation of the swimming direction by |omega| dt
-      double *e = p.e;
-      const double *W = p.W;
-      double theta;
-      if (dims == 2) {
-        theta = fabs(W[0]);
-      } else {
-        double s = 0.0;
-        for (int k = 0; k < 3; k++) s = s + W[k] * W[k];
-        theta = sqrt(s);
-      }
-      if (!(theta == 0.0)) {
-        if (dims == 2) {
-          const double kk = W[0] / theta;
-          theta *= dt;
-          const double c = cos(theta), s = sin(theta);
-          const double cr[2] = {-kk * e[1], kk * e[0]};
-          double vr[2];
-          vr[0] = e[0] * c + cr[0] * s;
-          vr[1] = e[1] * c + cr[1] * s;
-          const double nv = sqrt(vr[0] * vr[0] + vr[1] * vr[1]);
-          e[0] = vr[0] / nv;
-          e[1] = vr[1] / nv;
-        } else {
-          const double kk[3] = {W[0] / theta, W[1] / theta, W[2] / theta};
-          theta *= dt;
-          const double c = cos(theta), s = sin(theta);
-          const double cr[3] = {kk[1] * e[2] - kk[2] * e[1], -kk[0] * e[2] + kk[2] * e[0],
-                                kk[0] * e[1] - kk[1] * e[0]};
-          double vr[3];
-          for (int k = 0; k < 3; k++) vr[k] = e[k] * c + cr[k] * s;
-          double kd = 0.0;
-          for (int k = 0; k < 3; k++) kd = kd + kk[k] * e[k];
-          for (int k = 0; k < 3; k++) vr[k] = vr[k] + kk[k] * kd * (1.0 - c);
-          double nv = 0.0;
-          for (int k = 0; k < 3; k++) nv = nv + vr[k] * vr[k];
-          nv = sqrt(nv);
-          for (int k = 0; k < 3; k++) e[k] = vr[k] / nv;
-        }
-      }
-    }
-    find_nv = any_nonzero(Al, nw);
-  }
-  find_nv = moved || find_nv;
-}
-
-// moveParticles!, cpu/particles.jl:45-100: v += dP/m, omega += dL/I,
-// Euler step only when v != 0, no rotation of the swimming direction.
-__device__ static void move_cpu(DevParticle &p, const double dP[3], const double dL[3], int dims,
-                                double dt, bool &moved, bool &find_nv) {
-#pragma clang fp contract(off)
-  const int nw = dims == 2 ? 1 : 3;
-  for (int k = 0; k < dims; k++) p.V[k] = p.V[k] + p.inv_mass * dP[k];
-  for (int k = 0; k < nw; k++) p.W[k] = p.W[k] + p.inv_I * dL[k];
-  moved = false;
-  if (any_nonzero(p.V, dims)) {
-    for (int k = 0; k < dims; k++) p.R[k] = p.R[k] + dt * p.V[k];
-    moved = true;
-  }
-  find_nv = any_nonzero(dL, nw) || moved;
-}
-
-__global__ void __launch_bounds__(256) integrate_kernel(IntegrateArgs a) {
-  __shared__ DD red[256];
-  __shared__ double tot[LBM_MAX_PARTICLES][6];
-  const int t = threadIdx.x;
-  for (int p = 0; p < a.np; p++) {
-    for (int k = 0; k < 6; k++) {
-      // fixed-order double-double reduction of the link-kernel partials
-      DD s{0.0, 0.0};
-      for (int b = t; b < a.nblocks; b += blockDim.x) {
-        const double *src = a.partial + (((int64_t)b * a.np + p) * 6 + k) * 2;
-        s = dd_merge(s, DD{src[0], src[1]});
-      }
-      red[t] = s;
-      __syncthreads();
-      for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
-        if (t < w) red[t] = dd_merge(red[t], red[t + w]);
-        __syncthreads();
-      }
-      if (t == 0) {
-#pragma clang fp contract(off)
-        // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
-        // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
-        const double v = red[0].hi + red[0].lo;
-        tot[p][k] = -a.coeff * (v * a.ls);
-      }
-      __syncthreads();
-    }
-  }
-  if (t != 0) return;
-  for (int p = 0; p < a.np; p++) {
-    DevParticle &P = a.parts[p];
-    const double F[3] = {tot[p][0], tot[p][1], a.dims == 3 ? tot[p][2] : 0.0};
-    const double T[3] = {tot[p][3], a.dims == 3 ? tot[p][4] : 0.0, a.dims == 3 ? tot[p][5] : 0.0};
-    bool moved, find_nv;
-    if (a.semantics == LBM_SEMANTICS_GPU)
-      move_gpu(P, F, T, a.dims, a.dt, moved, find_nv);
-    else
-      move_cpu(P, F, T, a.dims, a.dt, moved, find_nv);
-    for (int k = 0; k < 3; k++) {
-      P.force[k] = F[k];
-      P.torque[k] = T[k];
-    }
-    if (moved) {
-      for (int k = 0; k < 3; k++) P.Rg[k] = P.R[k];
-      int32_t err = 0;
-      link_box(P.Rg, P.radius, a.x0, a.dx, a.n, a.dims, P.lo, P.hi, &err);
-      if (err) P.err = 1;
-    }
-    if (find_nv) {
-      for (int k = 0; k < 3; k++) {
-        P.nR[k] = P.R[k];
-        P.nV[k] = P.V[k];
-        P.nW[k] = P.W[k];
-        P.ne[k] = P.e[k];
-      }
-    }
-  }
-}
-
-hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream) {
-  hipLaunchKernelGGL(integrate_kernel, dim3(1), dim3(256), 0, stream, a);
+hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream) {
+  const int64_t n16 = bytes / 16;
+  const int blocks = 256 * 32;  // 32 blocks of 256 threads per CU, grid-stride beyond
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4 *>(src),
+                     static_cast<uint4 *>(dst), n16);
   return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@
 #pragma once
 
 #include "lbm_common.hpp"
+#include "lbm_integrate.hpp"
 
 namespace lbm {
 
@@ -383,6 +384,7 @@ struct PartLaunch {
   int32_t semantics;
   double coeff;                        // dx^d / dt (gpu) or dx^d (cpu)
   double cutoff;                       // 2 dx
+  double shell2;                       // (1.5 dx)^2 margin of the link shell
 };
 
 struct LinkArgs {
@@ -393,11 +395,28 @@ struct LinkArgs {
   CollParams c;
   const DevParticle *parts;
   const double *coords[3];
-  double *partial;  // [block][np][6]
+  double *partial;       // [block][np][6][hi, lo]
+  uint32_t *ticket;      // blocks done this step; the last one integrates
+  IntegrateArgs ia;      // for that last block
   PartLaunch pl;
 };
 
-// (X + (-R))_1^2 + ... < r^2 at node (x, y, z) (gpu/kernels.jl:344-353)
+// (X + (-R))_1^2 + ... at node (x, y, z) (gpu/kernels.jl:344-353)
+template <int D>
+__device__ __forceinline__ double dist2_at(const double *const coords[3], const double Rg[3], int x,
+                                           int y, int z) {
+#pragma clang fp contract(off)
+  const double a = coords[0][x] + (-Rg[0]);
+  const double b = coords[1][y] + (-Rg[1]);
+  double s = a * a + b * b;
+  if constexpr (D == 3) {
+    const double c = coords[2][z] + (-Rg[2]);
+    s = s + c * c;
+  }
+  return s;
+}
+
+// getSphere: (X + (-R))_1^2 + ... < r^2
 template <int D>
 __device__ __forceinline__ bool solid_at(const double *const coords[3], const double Rg[3],
                                          double r2, int x, int y, int z) {
@@ -489,29 +508,6 @@ __device__ __forceinline__ void node_velocity(const double *const coords[3], con
   }
 }
 
-// Double-double accumulation (Knuth's TwoSum) of the momentum-exchange sums:
-// they are evaluated essentially exactly and rounded once, so the result does
-// not depend on the order in which links, waves and blocks are combined.
-struct DD {
-  double hi, lo;
-};
-
-__device__ __forceinline__ void dd_add(DD &a, double x) {
-#pragma clang fp contract(off)
-  const double s = a.hi + x;
-  const double bb = s - a.hi;
-  const double err = (a.hi - (s - bb)) + (x - bb);
-  a.hi = s;
-  a.lo = a.lo + err;
-}
-
-__device__ __forceinline__ DD dd_merge(DD a, DD b) {
-#pragma clang fp contract(off)
-  DD r{a.hi, a.lo + b.lo};
-  dd_add(r, b.hi);
-  return r;
-}
-
 __device__ __forceinline__ DD wave_sum(DD v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -522,12 +518,41 @@ __device__ __forceinline__ DD wave_sum(DD v) {
   return v;
 }
 
+// Link bits of particle P at the fluid node (x, y, z): bit q set when
+// x - c_q is solid, i.e. (x, q̄) is a boundary link of P (bounceBackPrep,
+// gpu/aux.jl:187-198).  A node farther than 1.5 dx from the sphere surface,
+// or inside it, has none; the sphere test is the reference's expression.
+template <int Q>
+__device__ __forceinline__ uint32_t link_bits(const LinkArgs &a, const DevParticle &P, int x, int y,
+                                              int z) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const BoxGeom &g = a.g;
+  if (!(x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] &&
+        (D == 2 || (z >= P.lo[2] && z <= P.hi[2]))))
+    return 0;
+  const double r2 = P.radius * P.radius;
+  const double d2 = dist2_at<D>(a.coords, P.Rg, x, y, z);
+  if (d2 < r2) return 0;  // solid node
+  const double ro = P.radius + sqrt(a.pl.shell2);
+  if (d2 >= ro * ro) return 0;  // no solid neighbour possible
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 1; q < Q; q++) {
+    const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
+              sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
+    if (solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz)) m |= 1u << q;
+  }
+  return m;
+}
+
 // Bounce-back, Ladd and momentum exchange at the particle links.  Runs after
 // step_kernel on the same buffers: the pushes step_kernel made into link
 // slots (from solid nodes) are overwritten here with f*_{q̄}(x) plus the
 // Ladd terms, as the reference overwrites `streamed[cIR]` (gpu/fluids.jl:
 // 157-207).  f*(x) is recomputed from the unchanged input buffer with the
-// same collision code, so it equals the value step_kernel pushed.
+// same collision code, so it equals the value step_kernel pushed.  The last
+// block to finish reduces the momentum exchange and moves the particles.
 template <int Q, int S, int COLL, int FM>
 __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
   using L = Lattice<Q>;
@@ -549,13 +574,6 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
   const int y = P.lo[1] + tpy * pl.ly + (lid / pl.lx) % pl.ly;
   const int z = (D == 3 ? P.lo[2] : 0) + tpz * pl.lz + lid / (pl.lx * pl.ly);
 
-  // per particle: sum of s c_q (force) and of (s (x - R)) x c_q (torque)
-  DD acc[LBM_MAX_PARTICLES][6];
-#pragma unroll
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
-#pragma unroll
-    for (int k = 0; k < 6; k++) acc[p][k] = DD{0.0, 0.0};
-
   bool active = x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2]) && x < g.nx &&
                 y < g.ny && z < g.nz && !is_wall_node(g, x, y, z);
   // a node inside several boxes is handled by the first one
@@ -565,125 +583,113 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
         (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2])))
       active = false;
   }
-
-  uint32_t links[LBM_MAX_PARTICLES];
   uint32_t any = 0;
-  if (active) {
-    for (int p = 0; p < pl.np; p++) {
-      const DevParticle &Pp = a.parts[p];
-      uint32_t m = 0;
-      const bool inbox = x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
-                         (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2]));
-      if (inbox) {
-        const double r2 = Pp.radius * Pp.radius;
-        if (!solid_at<D>(a.coords, Pp.Rg, r2, x, y, z)) {
-#pragma unroll
-          for (int q = 1; q < Q; q++) {
-            const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
-                      sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
-            if (solid_at<D>(a.coords, Pp.Rg, r2, sx, sy, sz)) m |= 1u << q;
-          }
-        }
-      }
-      links[p] = m;
-      any |= m;
-    }
-  }
+  if (active)
+    for (int p = 0; p < pl.np; p++) any |= link_bits<Q>(a, a.parts[p], x, y, z);
 
+  const int64_t N = g.N;
+  const int n = x + g.nx * (y + g.ny * z);
+  R fs[Q];
+  R rho = 0;
   if (any) {
-    const int64_t N = g.N;
-    const int n = x + g.nx * (y + g.ny * z);
     const T *__restrict__ fin = static_cast<const T *>(a.fin);
-    T *__restrict__ fout = static_cast<T *>(a.fout);
-    R fs[Q];
-    R rho;
-    {
-      R f[Q];
+    R f[Q];
 #pragma unroll
-      for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
-      R Fn[3] = {0, 0, 0};
-      if constexpr (FM == LBM_FORCE_UNIFORM) {
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
+    R Fn[3] = {0, 0, 0};
+    if constexpr (FM == LBM_FORCE_UNIFORM) {
 #pragma unroll
-        for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
-      } else if constexpr (FM == LBM_FORCE_FIELD) {
-        const R *Ff = static_cast<const R *>(a.Ffield);
+      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
+    } else if constexpr (FM == LBM_FORCE_FIELD) {
+      const R *Ff = static_cast<const R *>(a.Ffield);
 #pragma unroll
-        for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
-      }
-      if constexpr (S == kF64)
-        collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-      else
-        collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
     }
-    const double X[3] = {a.coords[0][x], a.coords[1][y], D == 3 ? a.coords[2][z] : 0.0};
+    if constexpr (S == kF64)
+      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    else
+      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+  } else {
 #pragma unroll
-    for (int q = 1; q < Q; q++) {
-      if (!((any >> q) & 1)) continue;
-      const int o = L::opp[q];
-      R val = fs[o];
-      const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
-      const double lc = 2.0 * L::w(q) * a.c.invC2s;
-      const double shift = S == kF64 ? 0.0 : 2.0 * L::w(q) * a.c.rho0;
-      const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
-                sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
-      for (int p = 0; p < pl.np; p++) {
-        if (!((links[p] >> q) & 1)) continue;
-        const DevParticle &Pp = a.parts[p];
+    for (int q = 0; q < Q; q++) fs[q] = 0;
+  }
+  // streamed value of every link slot, starting from the bounce-back value
+  R val[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) val[q] = fs[L::opp[q]];
+  const double X[3] = {active ? a.coords[0][x] : 0.0, active ? a.coords[1][y] : 0.0,
+                       (D == 3 && active) ? a.coords[2][z] : 0.0};
+
+  __shared__ DD red[8][6];
+  const int wave = lid >> 6, lane = lid & 63;
+  const int nwaves = (blockDim.x + 63) >> 6;
+  // particles in order: a link shared by two particles carries the Ladd
+  // terms of the earlier ones into the later one's momentum exchange
+  for (int p = 0; p < pl.np; p++) {
+    const DevParticle &Pp = a.parts[p];
+    DD acc[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
+    const uint32_t m = any ? link_bits<Q>(a, Pp, x, y, z) : 0u;
+    if (m) {
+      const bool coupled = Pp.couple_forces || Pp.couple_torques;
+#pragma unroll
+      for (int q = 1; q < Q; q++) {
+        if (!((m >> q) & 1)) continue;
+        const int o = L::opp[q];
+        const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
+        const double lc = 2.0 * L::w(q) * a.c.invC2s;
+        const double shift = S == kF64 ? 0.0 : 2.0 * L::w(q) * a.c.rho0;
+        const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
+                  sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
         double ub[3] = {0.0, 0.0, 0.0};
         node_velocity<D>(a.coords, Pp, pl.semantics, pl.cutoff, sx, sy, sz, ub);
-        double udc = 0.0;
         {
 #pragma clang fp contract(off)
+          double udc = 0.0;
 #pragma unroll
           for (int k = 0; k < D; k++) udc = udc + ub[k] * ci[k];
+          // Ladd: streamed += 2 w_q / c_s^2 rho(x) (u_b . c_q) (gpu/fluids.jl:188-189)
           if constexpr (S == kF64)
-            val = val + lc * static_cast<double>(rho) * udc;
+            val[q] = val[q] + lc * static_cast<double>(rho) * udc;
           else
-            val = val + static_cast<R>(lc * static_cast<double>(rho) * udc);
+            val[q] = val[q] + static_cast<R>(lc * static_cast<double>(rho) * udc);
         }
-        if (Pp.couple_forces || Pp.couple_torques) {
+        if (coupled) {
 #pragma clang fp contract(off)
           // s = f*_{q̄}(x) + f_q(x) (gpu/fluids.jl:193); the loads are
           // -coeff * sum(s) * c_q and -coeff * cross(sum(s (x - R)), c_q),
           // formed from these exact sums by the integrator
-          const double s = static_cast<double>(fs[o]) + static_cast<double>(val) + shift;
+          const double s = static_cast<double>(fs[o]) + static_cast<double>(val[q]) + shift;
           const int c0 = L::cx[q], c1 = L::cy[q], c2 = L::cz[q];
           double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-Pp.R[k]));
           if (Pp.couple_forces) {
-            if (c0) dd_add(acc[p][0], c0 > 0 ? s : -s);
-            if (c1) dd_add(acc[p][1], c1 > 0 ? s : -s);
-            if (D == 3 && c2) dd_add(acc[p][2], c2 > 0 ? s : -s);
+            if (c0) dd_add(acc[0], c0 > 0 ? s : -s);
+            if (c1) dd_add(acc[1], c1 > 0 ? s : -s);
+            if (D == 3 && c2) dd_add(acc[2], c2 > 0 ? s : -s);
           }
           if (Pp.couple_torques) {
             if constexpr (D == 2) {
-              if (c1) dd_add(acc[p][3], c1 > 0 ? pr[0] : -pr[0]);
-              if (c0) dd_add(acc[p][3], c0 > 0 ? -pr[1] : pr[1]);
+              if (c1) dd_add(acc[3], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) dd_add(acc[3], c0 > 0 ? -pr[1] : pr[1]);
             } else {
-              if (c2) dd_add(acc[p][3], c2 > 0 ? pr[1] : -pr[1]);
-              if (c1) dd_add(acc[p][3], c1 > 0 ? -pr[2] : pr[2]);
-              if (c2) dd_add(acc[p][4], c2 > 0 ? -pr[0] : pr[0]);
-              if (c0) dd_add(acc[p][4], c0 > 0 ? pr[2] : -pr[2]);
-              if (c1) dd_add(acc[p][5], c1 > 0 ? pr[0] : -pr[0]);
-              if (c0) dd_add(acc[p][5], c0 > 0 ? -pr[1] : pr[1]);
+              if (c2) dd_add(acc[3], c2 > 0 ? pr[1] : -pr[1]);
+              if (c1) dd_add(acc[3], c1 > 0 ? -pr[2] : pr[2]);
+              if (c2) dd_add(acc[4], c2 > 0 ? -pr[0] : pr[0]);
+              if (c0) dd_add(acc[4], c0 > 0 ? pr[2] : -pr[2]);
+              if (c1) dd_add(acc[5], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) dd_add(acc[5], c0 > 0 ? -pr[1] : pr[1]);
             }
           }
         }
       }
-      fout[q * N + n] = encode<S, Q>(val, q);
     }
-  }
-
-  // Block reduction of the momentum exchange, one double-double partial per
-  // particle and component: [block][np][6][hi, lo].
-  __shared__ DD red[8][6];
-  const int wave = lid >> 6, lane = lid & 63;
-  const int nwaves = (blockDim.x + 63) >> 6;
-  for (int p = 0; p < pl.np; p++) {
+    // block reduction, one double-double partial per particle and component
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      const DD v = wave_sum(acc[p][k]);
+      const DD v = wave_sum(acc[k]);
       if (lane == 0) red[wave][k] = v;
     }
     __syncthreads();
@@ -696,6 +702,25 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
     }
     __syncthreads();
   }
+  if (any) {
+    T *__restrict__ fout = static_cast<T *>(a.fout);
+#pragma unroll
+    for (int q = 1; q < Q; q++)
+      if ((any >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
+  }
+
+  // The last block to finish integrates the particles (moveParticles!).
+  __shared__ uint32_t last;
+  if (lid == 0) {
+    __threadfence();
+    const uint32_t done = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  integrate_block(a.ia);
+  if (lid == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // -------------------------------------------------------- aux kernels

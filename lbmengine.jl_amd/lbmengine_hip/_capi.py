@@ -85,6 +85,9 @@ SIGNATURES = {
     "lbm_set_timing": [_vp, C.c_int],
     "lbm_last_run_ms": [_vp, _dp],
     "lbm_last_run_times": [_vp, _dp, _dp],
+    "lbm_measure_copy": [C.c_int64, C.c_int32, _dp],
+    "lbm_autotune": [_vp, C.c_int32, C.POINTER(C.c_int32), _dp],
+    "lbm_set_graph": [_vp, C.c_int32],
 }
 _RESTYPES = {"lbm_last_error": C.c_char_p, "lbm_version": C.c_char_p}
 
@@ -133,6 +136,13 @@ def as_dptr(a: np.ndarray):
 
 def fortran_f64(a) -> np.ndarray:
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def measure_copy_ms(nbytes: int, reps: int = 10) -> float:
+    """Average ms of a plain device copy of nbytes (read nbytes, write nbytes)."""
+    ms = C.c_double(0.0)
+    check(load()["lbm_measure_copy"](int(nbytes) // 16 * 16, int(reps), C.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

@@ -543,6 +543,17 @@ class LBMmodel:
         _capi.check(self._fn["lbm_get_block"](self._h, *[C.byref(x) for x in b]))
         return tuple(x.value for x in b)
 
+    def autotune(self, reps: int = 20):
+        """Pick the fastest block shape of the stream-collide kernel."""
+        best = (C.c_int32 * 3)()
+        ms = C.c_double()
+        _capi.check(self._fn["lbm_autotune"](self._h, int(reps), best, C.byref(ms)))
+        return tuple(best), ms.value
+
+    def set_graph(self, ticks_per_graph: int) -> None:
+        """Ticks per captured hipGraph (even), or 0 for direct launches."""
+        _capi.check(self._fn["lbm_set_graph"](self._h, int(ticks_per_graph)))
+
     def sync(self) -> None:
         _capi.check(self._fn["lbm_sync"](self._h))
 

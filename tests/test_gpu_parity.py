@@ -200,3 +200,39 @@ def test_particle_near_edge_is_reported(gpu):
     m = gpu.modelInit(**KW3, relaxationTimeRatio=0.8, kineticParameter=3 / 16, xlims=(-8, 7))
     with pytest.raises(lbmengine_hip.LBMError):
         lbmengine_hip.addSquirmer(m, radius=3.0, position=[5.5, 0, 0])
+
+
+@pytest.mark.parametrize("ticks_per_graph", [2, 6])
+def test_graph_replays_match_direct_launches(gpu, oracle, ticks_per_graph):
+    """Captured hipGraphs replayed several times, with eager work and syncs
+    between the replays, give bit-identical states to direct launches."""
+    import lbmengine_hip
+    kw = dict(**KW3, relaxationTimeRatio=0.8, kineticParameter=3 / 16, xlims=(-10, 9), forceDensity=[1e-6, 0, 0])
+    sq = dict(radius=4.0, position=[0.3, -0.4, 0.2], swimmingSpeed=1e-3, beta=-2.0, swimmingDirection=[1, 2, 0],
+              coupleForces=True, coupleTorques=True)
+    a = gpu.modelInit(**kw)
+    b = gpu.modelInit(**kw)
+    lbmengine_hip.addSquirmer(a, **sq)
+    lbmengine_hip.addSquirmer(b, **sq)
+    a.set_graph(ticks_per_graph)
+    b.set_graph(0)
+    for steps in (7, ticks_per_graph * 3, 5, ticks_per_graph * 2 + 1):
+        a.run(steps)
+        b.run(steps)
+        fa, fb = a.distributions, b.distributions  # eager work between replays
+        assert np.array_equal(fa, fb)
+        assert np.array_equal(a.particles[0].position, b.particles[0].position)
+        assert np.array_equal(a.particles[0].velocity, b.particles[0].velocity)
+    assert a.tick == b.tick
+
+
+def test_autotune_keeps_state_and_results(gpu, oracle):
+    kw = dict(**KW3, relaxationTimeRatio=0.8, kineticParameter=3 / 16, xlims=(0, 47), ylims=(0, 31), zlims=(0, 23))
+    g, o = pair(gpu, oracle, seed=9, **kw)
+    f0 = g.distributions
+    shape, ms = g.autotune(3)
+    assert ms > 0 and np.prod(shape) >= 64 and tuple(g.get_block()) == tuple(shape)
+    assert np.array_equal(g.distributions, f0)
+    g.run(12)
+    o.run(12)
+    compare_fields(g, o)
