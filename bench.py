@@ -142,7 +142,7 @@ def main():
         "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
-                   "nodes": nodes, "block": list(m.get_block()), "autotuned_kernel_ms": round(tuned_ms, 5)},
+                   "nodes": nodes, "block": list(m.get_block()), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
         "graph_MLUPS": round(graph_mlups, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
