@@ -56,7 +56,6 @@ struct lbm_handle {
   std::vector<DevParticle> parts_h;  // parameters as added
   double *partial = nullptr;
   size_t partial_cap = 0;
-  uint32_t *ticket = nullptr;  // link-kernel blocks finished in the current step
   PartLaunch pl{};
   int link_blocks = 0;
   Tiling tiling{};
@@ -150,9 +149,16 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   la.parts = h->parts_d;
   for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
   la.partial = h->partial;
-  la.ticket = h->ticket;
   la.pl = h->pl;
-  IntegrateArgs &ia = la.ia;
+  // links of every particle box, then one block that reduces and integrates
+  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+    case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+    default: e = launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+  }
+  if (e != hipSuccess) return e;
+  IntegrateArgs ia;
   ia.parts = h->parts_d;
   ia.partial = h->partial;
   ia.np = (int32_t)h->parts_h.size();
@@ -167,13 +173,7 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
     ia.x0[k] = h->x0[k];
     ia.n[k] = h->n[k];
   }
-  // one launch: links of every particle box; its last block integrates
-  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
-  switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: return launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
-    case LBM_STORAGE_F32: return launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
-    default: return launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll);
-  }
+  return launch_integrate(ia, h->stream);
 }
 
 void drop_graph(lbm_handle *h) {
@@ -313,7 +313,6 @@ int lbm_destroy(lbm_handle *h) {
   if (h->F) hipFree(h->F);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
-  if (h->ticket) hipFree(h->ticket);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -388,8 +387,6 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   }
   if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
-  if (hipMalloc(&h->ticket, sizeof(uint32_t)) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc ticket"));
-  if (hipMemset(h->ticket, 0, sizeof(uint32_t)) != hipSuccess) return bail(fail(LBM_EDEVICE, "clear ticket"));
   h->tiling = default_tiling(h);
   if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
     return bail(fail(LBM_EDEVICE, "hipEventCreate"));

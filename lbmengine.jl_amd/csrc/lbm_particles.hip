@@ -4,13 +4,31 @@
 
 namespace lbm {
 
+// Fixed-order reduction of the link-kernel partials and moveParticles!, one
+// block; launched after link_kernel so the kernel boundary orders them.
+__global__ void __launch_bounds__(256) integrate_kernel(IntegrateArgs a) { integrate_block(a); }
+
+hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream) {
+  hipLaunchKernelGGL(integrate_kernel, dim3(1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 // Plain device copy of n16 16-byte words: the measured HBM ceiling, and the
 // kernel of known traffic (n16*16 bytes read and written) that scales the
 // byte counters of the profiler.
 __global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                    int64_t n16) {
+  // four independent 16-byte loads in flight per thread, then four stores
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream) {

@@ -396,8 +396,6 @@ struct LinkArgs {
   const DevParticle *parts;
   const double *coords[3];
   double *partial;       // [block][np][6][hi, lo]
-  uint32_t *ticket;      // blocks done this step; the last one integrates
-  IntegrateArgs ia;      // for that last block
   PartLaunch pl;
 };
 
@@ -551,8 +549,9 @@ __device__ __forceinline__ uint32_t link_bits(const LinkArgs &a, const DevPartic
 // slots (from solid nodes) are overwritten here with f*_{q̄}(x) plus the
 // Ladd terms, as the reference overwrites `streamed[cIR]` (gpu/fluids.jl:
 // 157-207).  f*(x) is recomputed from the unchanged input buffer with the
-// same collision code, so it equals the value step_kernel pushed.  The last
-// block to finish reduces the momentum exchange and moves the particles.
+// same collision code, so it equals the value step_kernel pushed.  Each block
+// leaves one double-double partial of the momentum exchange per particle for
+// integrate_kernel.
 template <int Q, int S, int COLL, int FM>
 __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
   using L = Lattice<Q>;
@@ -583,9 +582,20 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
         (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2])))
       active = false;
   }
-  uint32_t any = 0;
+  // only nodes in some particle's link shell (r <= |x - R| < r + 1.5 dx) can
+  // have links; only they need f*
+  bool any = false;
   if (active)
-    for (int p = 0; p < pl.np; p++) any |= link_bits<Q>(a, a.parts[p], x, y, z);
+    for (int p = 0; p < pl.np; p++) {
+      const DevParticle &Pp = a.parts[p];
+      if (!(x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
+            (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2]))))
+        continue;
+      const double d2 = dist2_at<D>(a.coords, Pp.Rg, x, y, z);
+      const double ro = Pp.radius + sqrt(pl.shell2);
+      if (d2 >= Pp.radius * Pp.radius && d2 < ro * ro) any = true;
+    }
+  uint32_t written = 0;
 
   const int64_t N = g.N;
   const int n = x + g.nx * (y + g.ny * z);
@@ -631,6 +641,7 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
 #pragma unroll
     for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
     const uint32_t m = any ? link_bits<Q>(a, Pp, x, y, z) : 0u;
+    written |= m;
     if (m) {
       const bool coupled = Pp.couple_forces || Pp.couple_torques;
 #pragma unroll
@@ -702,25 +713,13 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
     }
     __syncthreads();
   }
-  if (any) {
+  if (written) {
     T *__restrict__ fout = static_cast<T *>(a.fout);
 #pragma unroll
     for (int q = 1; q < Q; q++)
-      if ((any >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
+      if ((written >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
   }
 
-  // The last block to finish integrates the particles (moveParticles!).
-  __shared__ uint32_t last;
-  if (lid == 0) {
-    __threadfence();
-    const uint32_t done = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = done == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  integrate_block(a.ia);
-  if (lid == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // -------------------------------------------------------- aux kernels
