@@ -59,7 +59,9 @@ struct lbm_handle {
   PartLaunch pl{};
   int link_blocks = 0;
   Tiling tiling{};
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // step kernels
+  hipStream_t stream2 = nullptr;  // link + integrate kernels, concurrent with the step
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int64_t tick = 0;
   double time = 0.0;
   bool populated = false;
@@ -121,16 +123,39 @@ Tiling default_tiling(const lbm_handle *h) {
   return make_tiling(h, bx, by, 1);
 }
 
-hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
+// Particle records of ping-pong parity b (double-buffered: the integrator of
+// a tick writes the other parity while the step kernel may still read this one).
+DevParticle *parts_of(lbm_handle *h, int b) { return h->parts_d + b * LBM_MAX_PARTICLES; }
+
+StepArgs step_args(lbm_handle *h, int in, void *fout) {
   StepArgs a;
   a.fin = h->f[in];
-  a.fout = h->f[in ^ 1];
+  a.fout = fout;
   a.Ffield = h->F;
   a.g = geom(h);
   a.c = coll(h);
   a.t = h->tiling;
+  a.parts = parts_of(h, in);
+  for (int k = 0; k < 3; k++) a.coords[k] = h->coords_d[k];
+  a.np = (int32_t)h->parts_h.size();
+  a.shell = 1.5 * h->cfg.dx;
+  return a;
+}
+
+// One tick.  With particles, the link and integrate kernels run on stream2
+// concurrently with the step kernel: stream2 starts after everything queued
+// on the main stream (the previous tick), and the main stream's next tick
+// waits for stream2's integrator.
+hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
+  StepArgs a = step_args(h, in, h->f[in ^ 1]);
   Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
   hipError_t e;
+  const bool particles = !h->parts_h.empty();
+  if (particles) {
+    e = hipEventRecord(h->ev_fork, h->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+    if (e != hipSuccess) return e;
+  }
   if (e0) hipEventRecord(e0, h->stream);
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
@@ -139,19 +164,19 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   }
   if (e != hipSuccess) return e;
   if (e1) hipEventRecord(e1, h->stream);
-  if (h->parts_h.empty()) return hipSuccess;
+  if (!particles) return hipSuccess;
   LinkArgs la;
   la.fin = h->f[in];
   la.fout = h->f[in ^ 1];
   la.Ffield = h->F;
   la.g = a.g;
   la.c = a.c;
-  la.parts = h->parts_d;
+  la.parts = parts_of(h, in);
   for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
   la.partial = h->partial;
   la.pl = h->pl;
   // links of every particle box, then one block that reduces and integrates
-  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream};
+  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream2};
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
     case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
@@ -159,7 +184,8 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   }
   if (e != hipSuccess) return e;
   IntegrateArgs ia;
-  ia.parts = h->parts_d;
+  ia.parts_in = parts_of(h, in);
+  ia.parts_out = parts_of(h, in ^ 1);
   ia.partial = h->partial;
   ia.np = (int32_t)h->parts_h.size();
   ia.nblocks = h->link_blocks;
@@ -173,7 +199,10 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
     ia.x0[k] = h->x0[k];
     ia.n[k] = h->n[k];
   }
-  return launch_integrate(ia, h->stream);
+  e = launch_integrate(ia, h->stream2);
+  if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->stream2);
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
+  return e;
 }
 
 void drop_graph(lbm_handle *h) {
@@ -276,7 +305,7 @@ int refresh_particle_launch(lbm_handle *h) {
 int check_particles(lbm_handle *h) {
   if (h->parts_h.empty()) return LBM_OK;
   std::vector<DevParticle> tmp(h->parts_h.size());
-  HIP_TRY(hipMemcpy(tmp.data(), h->parts_d, tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
+  HIP_TRY(hipMemcpy(tmp.data(), parts_of(h, h->cur), tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
           "copy particles");
   for (size_t p = 0; p < tmp.size(); p++)
     if (tmp[p].err)
@@ -316,6 +345,9 @@ int lbm_destroy(lbm_handle *h) {
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return LBM_OK;
@@ -385,9 +417,14 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (hipMalloc(&h->F, fb) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc force field"));
     if (hipMemset(h->F, 0, fb) != hipSuccess) return bail(fail(LBM_EDEVICE, "clear force field"));
   }
-  if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES) != hipSuccess)
+  if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES * 2) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
   h->tiling = default_tiling(h);
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(LBM_EDEVICE, "hipStreamCreate"));
+  if (hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(LBM_EDEVICE, "hipEventCreate"));
   if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
     return bail(fail(LBM_EDEVICE, "hipEventCreate"));
   *out = h;
@@ -532,7 +569,7 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
     return fail(LBM_ERANGE, "particle must lie at least two nodes inside every face of the lattice");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   const int idx = (int)h->parts_h.size();
-  HIP_TRY(hipMemcpy(h->parts_d + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
+  HIP_TRY(hipMemcpy(parts_of(h, h->cur) + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
   h->parts_h.push_back(d);
   int rc = refresh_particle_launch(h);
   if (rc) {
@@ -549,7 +586,7 @@ int lbm_get_particle(lbm_handle *h, int32_t id, lbm_particle *out) {
   if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   std::memset(out, 0, sizeof *out);
   out->kind = d.kind;
   out->couple_forces = d.couple_forces;
@@ -573,7 +610,7 @@ int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3], double tor
   if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   for (int k = 0; k < 3; k++) {
     force[k] = d.force[k];
     torque[k] = d.torque[k];
@@ -587,10 +624,10 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_
   if (cap > 0 && (!node || !dir)) return fail(LBM_EINVAL, "null output buffer");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, h->parts_d + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   LinkMaskArgs a;
   a.g = geom(h);
-  a.part = h->parts_d + id;
+  a.part = parts_of(h, h->cur) + id;
   for (int k = 0; k < 3; k++) {
     a.coords[k] = h->coords_d[k];
     a.lo[k] = d.lo[k];
@@ -784,13 +821,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   hipError_t e = hipSuccess;
   for (const Cand &c : cands) {
     h->tiling = make_tiling(h, c.bx, c.by, c.bz);
-    StepArgs a;
-    a.fin = h->f[h->cur];
-    a.fout = scratch;
-    a.Ffield = h->F;
-    a.g = geom(h);
-    a.c = coll(h);
-    a.t = h->tiling;
+    StepArgs a = step_args(h, h->cur, scratch);
     Launch l{dim3(h->tiling.ntiles), dim3(c.bx, c.by, c.bz), h->stream};
     for (int r = 0; r < 2 && e == hipSuccess; r++) {  // warm-up
       switch (h->cfg.storage) {

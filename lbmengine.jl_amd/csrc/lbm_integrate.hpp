@@ -31,8 +31,23 @@ __device__ __forceinline__ DD dd_merge(DD a, DD b) {
   return r;
 }
 
+__device__ __forceinline__ DD wave_sum(DD v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    DD o{__shfl_xor(v.hi, off), __shfl_xor(v.lo, off)};
+    // both lanes of a pair must form the same sum: order the operands
+    v = (threadIdx.x & off) ? dd_merge(o, v) : dd_merge(v, o);
+  }
+  return v;
+}
+
+// Particle records are double-buffered by tick parity: the step and link
+// kernels of tick t read buffer t&1 while the integrator writes t+1's, so a
+// step kernel still running on another stream never sees a half-updated
+// particle.
 struct IntegrateArgs {
-  DevParticle *parts;
+  const DevParticle *parts_in;
+  DevParticle *parts_out;
   const double *partial;  // [block][np][6][hi, lo]
   int32_t np, nblocks;
   int32_t dims, semantics;
@@ -161,36 +176,30 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
 
 // Run by one whole block of 256 threads (the last block of link_kernel).
 __device__ inline void integrate_block(const IntegrateArgs &a) {
-  __shared__ DD red[256];
   __shared__ double tot[LBM_MAX_PARTICLES][6];
-  const int t = threadIdx.x;
-  for (int p = 0; p < a.np; p++) {
-    for (int k = 0; k < 6; k++) {
-      // fixed-order double-double reduction of the link-kernel partials
-      DD s{0.0, 0.0};
-      for (int b = t; b < a.nblocks; b += blockDim.x) {
-        const double *src = a.partial + (((int64_t)b * a.np + p) * 6 + k) * 2;
-        s = dd_merge(s, DD{src[0], src[1]});
-      }
-      red[t] = s;
-      __syncthreads();
-      for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
-        if (t < w) red[t] = dd_merge(red[t], red[t + w]);
-        __syncthreads();
-      }
-      if (t == 0) {
+  const int t = threadIdx.x, lane = t & 63;
+  // one wave per (particle, component): lanes fold strided partials, then an
+  // ordered butterfly; both orders are fixed, so the result is reproducible
+  for (int job = t >> 6; job < a.np * 6; job += blockDim.x >> 6) {
+    const int p = job / 6, k = job % 6;
+    DD s{0.0, 0.0};
+    for (int b = lane; b < a.nblocks; b += 64) {
+      const double *src = a.partial + (((int64_t)b * a.np + p) * 6 + k) * 2;
+      s = dd_merge(s, DD{src[0], src[1]});
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
 #pragma clang fp contract(off)
-        // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
-        // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
-        const double v = red[0].hi + red[0].lo;
-        tot[p][k] = -a.coeff * (v * a.ls);
-      }
-      __syncthreads();
+      // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
+      // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
+      const double v = s.hi + s.lo;
+      tot[p][k] = -a.coeff * (v * a.ls);
     }
   }
+  __syncthreads();
   if (t != 0) return;
   for (int p = 0; p < a.np; p++) {
-    DevParticle &P = a.parts[p];
+    DevParticle P = a.parts_in[p];
     const double F[3] = {tot[p][0], tot[p][1], a.dims == 3 ? tot[p][2] : 0.0};
     const double T[3] = {tot[p][3], a.dims == 3 ? tot[p][4] : 0.0, a.dims == 3 ? tot[p][5] : 0.0};
     bool moved, find_nv;
@@ -216,6 +225,7 @@ __device__ inline void integrate_block(const IntegrateArgs &a) {
         P.ne[k] = P.e[k];
       }
     }
+    a.parts_out[p] = P;
   }
 }
 

@@ -86,6 +86,11 @@ struct StepArgs {
   BoxGeom g;
   CollParams c;
   Tiling t;
+  // particles: the step kernel leaves their link slots to link_kernel
+  const DevParticle *parts;  // records of this tick's parity
+  const double *coords[3];
+  int32_t np;
+  double shell;              // 1.5 dx: links lie within r +- shell of a centre
 };
 
 // ------------------------------------------------------------ collision
@@ -281,6 +286,36 @@ __device__ __forceinline__ bool is_wall_node(const BoxGeom &g, int x, int y, int
          ((g.walled & 4) && (z == 0 || z == g.nz - 1));
 }
 
+// (X + (-R))_1^2 + ... at node (x, y, z) (gpu/kernels.jl:344-353)
+template <int D>
+__device__ __forceinline__ double dist2_at(const double *const coords[3], const double Rg[3], int x,
+                                           int y, int z) {
+#pragma clang fp contract(off)
+  const double a = coords[0][x] + (-Rg[0]);
+  const double b = coords[1][y] + (-Rg[1]);
+  double s = a * a + b * b;
+  if constexpr (D == 3) {
+    const double c = coords[2][z] + (-Rg[2]);
+    s = s + c * c;
+  }
+  return s;
+}
+
+// getSphere: (X + (-R))_1^2 + ... < r^2
+template <int D>
+__device__ __forceinline__ bool solid_at(const double *const coords[3], const double Rg[3],
+                                         double r2, int x, int y, int z) {
+#pragma clang fp contract(off)
+  const double a = coords[0][x] + (-Rg[0]);
+  const double b = coords[1][y] + (-Rg[1]);
+  double s = a * a + b * b;
+  if constexpr (D == 3) {
+    const double c = coords[2][z] + (-Rg[2]);
+    s = s + c * c;
+  }
+  return s < r2;
+}
+
 // -------------------------------------------------------------- step
 template <int Q, int S, int COLL, int FM>
 __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
@@ -308,6 +343,32 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
     for (int q = 0; q < Q; q++)
       fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
     return;
+  }
+
+  // Particle boundary links are written by link_kernel, which may run
+  // concurrently: a solid node does not push into a fluid neighbour of the
+  // same sphere (skip_push), and a fluid node does not bounce back into a
+  // slot that is a particle link (skip_own).  Only nodes within 1.5 dx of a
+  // sphere surface can be either.
+  uint32_t skip_push = 0, skip_own = 0;
+  for (int p = 0; p < a.np; p++) {
+    const DevParticle &P = a.parts[p];
+    if (x < P.lo[0] || x > P.hi[0] || y < P.lo[1] || y > P.hi[1] ||
+        (D == 3 && (z < P.lo[2] || z > P.hi[2])))
+      continue;
+    const double d2 = dist2_at<D>(a.coords, P.Rg, x, y, z);
+    const double r2 = P.radius * P.radius;
+    const double ro = P.radius + a.shell, ri = P.radius - a.shell;
+    if (d2 >= ro * ro || (ri > 0.0 && d2 < ri * ri)) continue;
+    const bool s0 = d2 < r2;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+      const int sx = wrapi(x + L::cx[q], g.nx), sy = wrapi(y + L::cy[q], g.ny),
+                sz = D == 3 ? wrapi(z + L::cz[q], g.nz) : 0;
+      const bool sn = solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz);
+      if (s0 && !sn) skip_push |= 1u << q;            // x + c_q is a link node of P
+      if (!s0 && sn) skip_own |= 1u << L::opp[q];     // (x, q̄) is a link slot of P
+    }
   }
 
   R fs[Q];
@@ -346,8 +407,9 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
   const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
   const bool near = xl | xh | yl | yh | zl | zh;
 
-  if (!near) {
-    // away from walls every population streams: no per-direction branches
+  if (!near && !skip_push) {
+    // away from walls and sphere surfaces every population streams: no
+    // per-direction branches
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int t = ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1];
@@ -368,6 +430,8 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
       // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
       own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
             (cz > 0 && zl) || (cz < 0 && zh);
+      push = push && !((skip_push >> q) & 1);
+      own = own && !((skip_own >> q) & 1);
     }
     if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
     if (own) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
@@ -398,36 +462,6 @@ struct LinkArgs {
   double *partial;       // [block][np][6][hi, lo]
   PartLaunch pl;
 };
-
-// (X + (-R))_1^2 + ... at node (x, y, z) (gpu/kernels.jl:344-353)
-template <int D>
-__device__ __forceinline__ double dist2_at(const double *const coords[3], const double Rg[3], int x,
-                                           int y, int z) {
-#pragma clang fp contract(off)
-  const double a = coords[0][x] + (-Rg[0]);
-  const double b = coords[1][y] + (-Rg[1]);
-  double s = a * a + b * b;
-  if constexpr (D == 3) {
-    const double c = coords[2][z] + (-Rg[2]);
-    s = s + c * c;
-  }
-  return s;
-}
-
-// getSphere: (X + (-R))_1^2 + ... < r^2
-template <int D>
-__device__ __forceinline__ bool solid_at(const double *const coords[3], const double Rg[3],
-                                         double r2, int x, int y, int z) {
-#pragma clang fp contract(off)
-  const double a = coords[0][x] + (-Rg[0]);
-  const double b = coords[1][y] + (-Rg[1]);
-  double s = a * a + b * b;
-  if constexpr (D == 3) {
-    const double c = coords[2][z] + (-Rg[2]);
-    s = s + c * c;
-  }
-  return s < r2;
-}
 
 // Node velocity of particle p at the solid node (x, y, z):
 // getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
@@ -504,16 +538,6 @@ __device__ __forceinline__ void node_velocity(const double *const coords[3], con
 #pragma unroll
     for (int k = 0; k < D; k++) out[k] = first * (edr * fr[k] - e[k]) + bulk[k];
   }
-}
-
-__device__ __forceinline__ DD wave_sum(DD v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    DD o{__shfl_xor(v.hi, off), __shfl_xor(v.lo, off)};
-    // both lanes of a pair must form the same sum: order the operands
-    v = (threadIdx.x & off) ? dd_merge(o, v) : dd_merge(v, o);
-  }
-  return v;
 }
 
 // Link bits of particle P at the fluid node (x, y, z): bit q set when
