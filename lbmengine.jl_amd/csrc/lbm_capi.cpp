@@ -56,6 +56,8 @@ struct lbm_handle {
   std::vector<DevParticle> parts_h;  // parameters as added
   double *partial = nullptr;
   size_t partial_cap = 0;
+  uint2 *masks = nullptr;    // link masks, 2 parities x mask_total entries
+  int64_t mask_total = 0;
   PartLaunch pl{};
   int link_blocks = 0;
   Tiling tiling{};
@@ -136,47 +138,45 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   a.c = coll(h);
   a.t = h->tiling;
   a.parts = parts_of(h, in);
-  for (int k = 0; k < 3; k++) a.coords[k] = h->coords_d[k];
+  a.masks = h->masks ? h->masks + in * h->mask_total : nullptr;
   a.np = (int32_t)h->parts_h.size();
-  a.shell = 1.5 * h->cfg.dx;
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++) {
+    a.mask_ext[p][0] = h->pl.ext[p][0];
+    a.mask_ext[p][1] = h->pl.ext[p][1];
+    a.mask_off[p] = h->pl.mask_off[p];
+  }
   return a;
 }
 
-// One tick.  With particles, the link and integrate kernels run on stream2
-// concurrently with the step kernel: stream2 starts after everything queued
-// on the main stream (the previous tick), and the main stream's next tick
-// waits for stream2's integrator.
-hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
-  StepArgs a = step_args(h, in, h->f[in ^ 1]);
-  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
-  hipError_t e;
-  const bool particles = !h->parts_h.empty();
-  if (particles) {
-    e = hipEventRecord(h->ev_fork, h->stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-    if (e != hipSuccess) return e;
-  }
-  if (e0) hipEventRecord(e0, h->stream);
-  switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-    case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-    default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-  }
-  if (e != hipSuccess) return e;
-  if (e1) hipEventRecord(e1, h->stream);
-  if (!particles) return hipSuccess;
+// Link masks of parity b from the particle records of parity b.
+hipError_t masks_for(lbm_handle *h, int b, hipStream_t stream) {
+  MaskArgs m;
+  m.g = geom(h);
+  m.parts = parts_of(h, b);
+  for (int k = 0; k < 3; k++) m.coords[k] = h->coords_d[k];
+  m.masks = h->masks + b * h->mask_total;
+  m.np = h->pl.np;
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
+    for (int k = 0; k < 3; k++) m.ext[p][k] = h->pl.ext[p][k];
+  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) m.off[p] = h->pl.mask_off[p];
+  return launch_masks(h->Q, m, stream);
+}
+
+// Particle side of a tick on stream2: links, integration, next masks.
+hipError_t particle_side(lbm_handle *h, int in) {
   LinkArgs la;
   la.fin = h->f[in];
   la.fout = h->f[in ^ 1];
   la.Ffield = h->F;
-  la.g = a.g;
-  la.c = a.c;
+  la.g = geom(h);
+  la.c = coll(h);
   la.parts = parts_of(h, in);
   for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
   la.partial = h->partial;
+  la.masks = h->masks + in * h->mask_total;
   la.pl = h->pl;
-  // links of every particle box, then one block that reduces and integrates
   Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream2};
+  hipError_t e;
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
     case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
@@ -200,8 +200,36 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
     ia.n[k] = h->n[k];
   }
   e = launch_integrate(ia, h->stream2);
-  if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->stream2);
-  if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
+  // the next tick's link masks from the records just written
+  if (e == hipSuccess) e = masks_for(h, in ^ 1, h->stream2);
+  return e;
+}
+
+// One tick.  With particles, the link, integrate and mask kernels run on
+// stream2 concurrently with the step kernel (they touch disjoint slots): they
+// start after everything queued on the main stream (the previous tick) and
+// are enqueued first, and the main stream's next tick waits for them.
+hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
+  StepArgs a = step_args(h, in, h->f[in ^ 1]);
+  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
+  hipError_t e = hipSuccess;
+  const bool particles = !h->parts_h.empty();
+  if (particles) {
+    e = hipEventRecord(h->ev_fork, h->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+    if (e == hipSuccess) e = particle_side(h, in);
+    if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->stream2);
+    if (e != hipSuccess) return e;
+  }
+  if (e0) hipEventRecord(e0, h->stream);
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+    case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+    default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+  }
+  if (e != hipSuccess) return e;
+  if (e1) hipEventRecord(e1, h->stream);
+  if (particles) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
   return e;
 }
 
@@ -292,12 +320,23 @@ int refresh_particle_launch(lbm_handle *h) {
   }
   pl.first_block[np] = blocks;
   h->link_blocks = blocks;
+  int64_t off = 0;
+  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) {
+    pl.mask_off[p] = off;
+    if (p < np) off += (int64_t)pl.ext[p][0] * pl.ext[p][1] * pl.ext[p][2];
+  }
   const size_t need = (size_t)std::max(blocks, 1) * std::max(np, 1) * 6 * 2;  // double-double
   if (need > h->partial_cap) {
     if (h->partial) hipFree(h->partial);
     h->partial = nullptr;
     HIP_TRY(hipMalloc(&h->partial, need * sizeof(double)), "hipMalloc partials");
     h->partial_cap = need;
+  }
+  if (off != h->mask_total) {
+    if (h->masks) hipFree(h->masks);
+    h->masks = nullptr;
+    HIP_TRY(hipMalloc(&h->masks, 2 * off * sizeof(uint2)), "hipMalloc link masks");
+    h->mask_total = off;
   }
   return LBM_OK;
 }
@@ -342,6 +381,7 @@ int lbm_destroy(lbm_handle *h) {
   if (h->F) hipFree(h->F);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
+  if (h->masks) hipFree(h->masks);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -577,6 +617,9 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
     return rc;
   }
   drop_graph(h);
+  // link masks of the current geometry (every particle: the layout changed)
+  HIP_TRY(masks_for(h, h->cur, h->stream), "link masks");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   if (id) *id = idx;
   return LBM_OK;
 }

@@ -29,5 +29,6 @@ hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream);
+hipError_t launch_masks(int Q, const MaskArgs &a, hipStream_t stream);
 
 }  // namespace lbm

@@ -39,6 +39,17 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
   return hipGetLastError();
 }
 
+hipError_t launch_masks(int Q, const MaskArgs &a, hipStream_t stream) {
+  const int64_t tot = a.off[a.np];
+  if (tot <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((tot + 255) / 256)), block(256);
+  if (Q == 9)
+    hipLaunchKernelGGL(mask_kernel<9>, grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL(mask_kernel<19>, grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l) {
   if (Q == 9)
     hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, 0, l.stream, a);

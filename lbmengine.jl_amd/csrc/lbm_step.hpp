@@ -88,9 +88,10 @@ struct StepArgs {
   Tiling t;
   // particles: the step kernel leaves their link slots to link_kernel
   const DevParticle *parts;  // records of this tick's parity
-  const double *coords[3];
+  const uint2 *masks;        // per link-box node: (pushes to skip, own link slots)
   int32_t np;
-  double shell;              // 1.5 dx: links lie within r +- shell of a centre
+  int32_t mask_ext[LBM_MAX_PARTICLES][2];  // x, y extents of each box's mask
+  int64_t mask_off[LBM_MAX_PARTICLES];     // first mask entry of each particle
 };
 
 // ------------------------------------------------------------ collision
@@ -348,27 +349,18 @@ __global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
   // Particle boundary links are written by link_kernel, which may run
   // concurrently: a solid node does not push into a fluid neighbour of the
   // same sphere (skip_push), and a fluid node does not bounce back into a
-  // slot that is a particle link (skip_own).  Only nodes within 1.5 dx of a
-  // sphere surface can be either.
+  // slot that is a particle link (skip_own).  mask_kernel evaluated both for
+  // the nodes of each particle's link box.
   uint32_t skip_push = 0, skip_own = 0;
   for (int p = 0; p < a.np; p++) {
     const DevParticle &P = a.parts[p];
     if (x < P.lo[0] || x > P.hi[0] || y < P.lo[1] || y > P.hi[1] ||
         (D == 3 && (z < P.lo[2] || z > P.hi[2])))
       continue;
-    const double d2 = dist2_at<D>(a.coords, P.Rg, x, y, z);
-    const double r2 = P.radius * P.radius;
-    const double ro = P.radius + a.shell, ri = P.radius - a.shell;
-    if (d2 >= ro * ro || (ri > 0.0 && d2 < ri * ri)) continue;
-    const bool s0 = d2 < r2;
-#pragma unroll
-    for (int q = 1; q < Q; q++) {
-      const int sx = wrapi(x + L::cx[q], g.nx), sy = wrapi(y + L::cy[q], g.ny),
-                sz = D == 3 ? wrapi(z + L::cz[q], g.nz) : 0;
-      const bool sn = solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz);
-      if (s0 && !sn) skip_push |= 1u << q;            // x + c_q is a link node of P
-      if (!s0 && sn) skip_own |= 1u << L::opp[q];     // (x, q̄) is a link slot of P
-    }
+    const uint2 m = a.masks[a.mask_off[p] + (x - P.lo[0]) +
+                            a.mask_ext[p][0] * ((y - P.lo[1]) + a.mask_ext[p][1] * (D == 3 ? z - P.lo[2] : 0))];
+    skip_push |= m.x;
+    skip_own |= m.y;
   }
 
   R fs[Q];
@@ -444,6 +436,7 @@ struct PartLaunch {
   int32_t ext[LBM_MAX_PARTICLES][3];   // link box extent per particle (nodes)
   int32_t tiles[LBM_MAX_PARTICLES][3]; // tiles per dimension per particle
   int32_t first_block[LBM_MAX_PARTICLES + 1];
+  int64_t mask_off[LBM_MAX_PARTICLES + 1]; // first mask entry per particle (ext-sized boxes)
   int32_t lx, ly, lz;                  // link-kernel block shape
   int32_t semantics;
   double coeff;                        // dx^d / dt (gpu) or dx^d (cpu)
@@ -460,8 +453,22 @@ struct LinkArgs {
   const DevParticle *parts;
   const double *coords[3];
   double *partial;       // [block][np][6][hi, lo]
+  const uint2 *masks;    // mask_kernel output of this tick (.y = own link slots)
   PartLaunch pl;
 };
+
+// Own link slots of particle p at node (x, y, z) inside its link box: bit q
+// set when x is fluid and x - c_q solid.
+template <int D>
+__device__ __forceinline__ uint32_t own_links(const uint2 *masks, const PartLaunch &pl,
+                                              const DevParticle &P, int p, int x, int y, int z) {
+  if (x < P.lo[0] || x > P.hi[0] || y < P.lo[1] || y > P.hi[1] ||
+      (D == 3 && (z < P.lo[2] || z > P.hi[2])))
+    return 0;
+  return masks[pl.mask_off[p] + (x - P.lo[0]) +
+               pl.ext[p][0] * ((y - P.lo[1]) + pl.ext[p][1] * (D == 3 ? z - P.lo[2] : 0))]
+      .y;
+}
 
 // Node velocity of particle p at the solid node (x, y, z):
 // getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
@@ -540,32 +547,51 @@ __device__ __forceinline__ void node_velocity(const double *const coords[3], con
   }
 }
 
-// Link bits of particle P at the fluid node (x, y, z): bit q set when
-// x - c_q is solid, i.e. (x, q̄) is a boundary link of P (bounceBackPrep,
-// gpu/aux.jl:187-198).  A node farther than 1.5 dx from the sphere surface,
-// or inside it, has none; the sphere test is the reference's expression.
+// Per node of each particle's link box, the two link masks of this tick's
+// geometry (bounceBackPrep, gpu/aux.jl:187-198, with the getSphere test of
+// gpu/kernels.jl:344-353):
+//   .x  pushes a solid node must leave to link_kernel: bit q when x is solid
+//       and x + c_q fluid;
+//   .y  the node's own link slots: bit q when x is fluid and x - c_q solid.
+// Boxes are stored with the fixed extent ext (the largest box a sphere of
+// that radius can need), starting at the box's lo corner.
+struct MaskArgs {
+  BoxGeom g;
+  const DevParticle *parts;
+  const double *coords[3];
+  uint2 *masks;
+  int32_t np;
+  int32_t ext[LBM_MAX_PARTICLES][3];
+  int64_t off[LBM_MAX_PARTICLES + 1];
+};
+
 template <int Q>
-__device__ __forceinline__ uint32_t link_bits(const LinkArgs &a, const DevParticle &P, int x, int y,
-                                              int z) {
+__global__ void __launch_bounds__(256) mask_kernel(MaskArgs a) {
   using L = Lattice<Q>;
   constexpr int D = L::D;
-  const BoxGeom &g = a.g;
-  if (!(x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] &&
-        (D == 2 || (z >= P.lo[2] && z <= P.hi[2]))))
-    return 0;
-  const double r2 = P.radius * P.radius;
-  const double d2 = dist2_at<D>(a.coords, P.Rg, x, y, z);
-  if (d2 < r2) return 0;  // solid node
-  const double ro = P.radius + sqrt(a.pl.shell2);
-  if (d2 >= ro * ro) return 0;  // no solid neighbour possible
-  uint32_t m = 0;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.off[a.np]) return;
+  int p = 0;
+  while (p + 1 < a.np && t >= a.off[p + 1]) p++;
+  const int64_t l = t - a.off[p];
+  const DevParticle &P = a.parts[p];
+  const int x = P.lo[0] + (int)(l % a.ext[p][0]);
+  const int y = P.lo[1] + (int)((l / a.ext[p][0]) % a.ext[p][1]);
+  const int z = (D == 3 ? P.lo[2] : 0) + (int)(l / ((int64_t)a.ext[p][0] * a.ext[p][1]));
+  uint2 m = make_uint2(0u, 0u);
+  if (x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2])) {
+    const double r2 = P.radius * P.radius;
+    const bool s0 = solid_at<D>(a.coords, P.Rg, r2, x, y, z);
 #pragma unroll
-  for (int q = 1; q < Q; q++) {
-    const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
-              sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
-    if (solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz)) m |= 1u << q;
+    for (int q = 1; q < Q; q++) {
+      const int sx = wrapi(x + L::cx[q], a.g.nx), sy = wrapi(y + L::cy[q], a.g.ny),
+                sz = D == 3 ? wrapi(z + L::cz[q], a.g.nz) : 0;
+      const bool sn = solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz);
+      if (s0 && !sn) m.x |= 1u << q;
+      if (!s0 && sn) m.y |= 1u << L::opp[q];
+    }
   }
-  return m;
+  a.masks[t] = m;
 }
 
 // Bounce-back, Ladd and momentum exchange at the particle links.  Runs after
@@ -606,19 +632,11 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
         (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2])))
       active = false;
   }
-  // only nodes in some particle's link shell (r <= |x - R| < r + 1.5 dx) can
-  // have links; only they need f*
+  // only nodes with a link slot of some particle need f*
   bool any = false;
   if (active)
-    for (int p = 0; p < pl.np; p++) {
-      const DevParticle &Pp = a.parts[p];
-      if (!(x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
-            (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2]))))
-        continue;
-      const double d2 = dist2_at<D>(a.coords, Pp.Rg, x, y, z);
-      const double ro = Pp.radius + sqrt(pl.shell2);
-      if (d2 >= Pp.radius * Pp.radius && d2 < ro * ro) any = true;
-    }
+    for (int p = 0; p < pl.np; p++)
+      if (own_links<D>(a.masks, pl, a.parts[p], p, x, y, z)) any = true;
   uint32_t written = 0;
 
   const int64_t N = g.N;
@@ -664,7 +682,7 @@ __global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
     DD acc[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
-    const uint32_t m = any ? link_bits<Q>(a, Pp, x, y, z) : 0u;
+    const uint32_t m = any ? own_links<D>(a.masks, pl, Pp, p, x, y, z) : 0u;
     written |= m;
     if (m) {
       const bool coupled = Pp.couple_forces || Pp.couple_torques;
