@@ -6,7 +6,10 @@ namespace lbm {
 
 // Fixed-order reduction of the link-kernel partials and moveParticles!, one
 // block; launched after link_kernel so the kernel boundary orders them.
-__global__ void __launch_bounds__(256) integrate_kernel(IntegrateArgs a) { integrate_block(a); }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+integrate_kernel(IntegrateArgs a) {
+  integrate_block(a);
+}
 
 hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream) {
   hipLaunchKernelGGL(integrate_kernel, dim3(1), dim3(256), 0, stream, a);

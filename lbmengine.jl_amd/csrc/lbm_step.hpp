@@ -566,7 +566,7 @@ struct MaskArgs {
 };
 
 template <int Q>
-__global__ void __launch_bounds__(256) mask_kernel(MaskArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) mask_kernel(MaskArgs a) {
   using L = Lattice<Q>;
   constexpr int D = L::D;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -603,7 +603,9 @@ __global__ void __launch_bounds__(256) mask_kernel(MaskArgs a) {
 // leaves one double-double partial of the momentum exchange per particle for
 // integrate_kernel.
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(256) link_kernel(LinkArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) link_kernel(LinkArgs a) {
+  // (8 waves per SIMD: at most 64 VGPRs, so its waves fit in the slots the
+  // concurrently running step kernel frees)
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
