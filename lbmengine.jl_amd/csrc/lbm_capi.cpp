@@ -54,16 +54,15 @@ struct lbm_handle {
   void *F = nullptr;  // force field (double or float)
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
-  double *partial = nullptr;
-  size_t partial_cap = 0;
-  uint2 *masks = nullptr;    // link masks, 2 parities x mask_total entries
-  int64_t mask_total = 0;
-  PartLaunch pl{};
-  int link_blocks = 0;
+  double *partial = nullptr;     // [box block][particle][6][hi, lo]
+  size_t partial_cap = 0;        // doubles
+  uint32_t *ticket = nullptr;    // box blocks finished in the running tick
+  // box_kernel grid (see StepParticles)
+  int32_t first_block[LBM_MAX_PARTICLES + 1] = {0};
+  int32_t box_tiles[LBM_MAX_PARTICLES][3] = {{0}};
+  dim3 box_block{1, 1, 1};
   Tiling tiling{};
-  hipStream_t stream = nullptr;   // step kernels
-  hipStream_t stream2 = nullptr;  // link + integrate kernels, concurrent with the step
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream = nullptr;
   int64_t tick = 0;
   double time = 0.0;
   bool populated = false;
@@ -129,6 +128,36 @@ Tiling default_tiling(const lbm_handle *h) {
 // a tick writes the other parity while the step kernel may still read this one).
 DevParticle *parts_of(lbm_handle *h, int b) { return h->parts_d + b * LBM_MAX_PARTICLES; }
 
+// Grid of box_kernel: 256-thread blocks tiling each particle's link box
+// from its lower corner, enough of them for the largest box the particle's
+// radius allows; partials for every block.
+int plan_boxes(lbm_handle *h) {
+  h->box_block = h->dims == 3 ? dim3(8, 8, 4) : dim3(16, 16, 1);
+  const int32_t b[3] = {(int32_t)h->box_block.x, (int32_t)h->box_block.y, (int32_t)h->box_block.z};
+  int32_t blocks = 0;
+  for (size_t p = 0; p < h->parts_h.size(); p++) {
+    h->first_block[p] = blocks;
+    int32_t nb = 1;
+    for (int k = 0; k < 3; k++) {
+      const int32_t e = k < h->dims ? std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]) : 1;
+      h->box_tiles[p][k] = (e + b[k] - 1) / b[k];
+      nb *= h->box_tiles[p][k];
+    }
+    blocks += nb;
+  }
+  for (size_t p = h->parts_h.size(); p <= LBM_MAX_PARTICLES; p++) h->first_block[p] = blocks;
+  const size_t need = (size_t)std::max(blocks, 1) * LBM_MAX_PARTICLES * 6 * 2;
+  if (need > h->partial_cap) {
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    if (h->partial) hipFree(h->partial);
+    h->partial = nullptr;
+    h->partial_cap = 0;
+    HIP_TRY(hipMalloc(&h->partial, need * sizeof(double)), "hipMalloc partials");
+    h->partial_cap = need;
+  }
+  return LBM_OK;
+}
+
 StepArgs step_args(lbm_handle *h, int in, void *fout) {
   StepArgs a;
   a.fin = h->f[in];
@@ -137,88 +166,52 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   a.g = geom(h);
   a.c = coll(h);
   a.t = h->tiling;
-  a.parts = parts_of(h, in);
-  a.masks = h->masks ? h->masks + in * h->mask_total : nullptr;
-  a.np = (int32_t)h->parts_h.size();
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++) {
-    a.mask_ext[p][0] = h->pl.ext[p][0];
-    a.mask_ext[p][1] = h->pl.ext[p][1];
-    a.mask_off[p] = h->pl.mask_off[p];
+  StepParticles &sp = a.sp;
+  sp.parts_in = parts_of(h, in);
+  sp.parts_out = parts_of(h, in ^ 1);
+  for (int k = 0; k < 3; k++) sp.coords[k] = h->coords_d[k];
+  sp.partial = h->partial;
+  sp.ticket = h->ticket;
+  sp.np = (int32_t)h->parts_h.size();
+  sp.semantics = h->cfg.semantics;
+  sp.dims = h->dims;
+  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) sp.first_block[p] = h->first_block[p];
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
+    for (int k = 0; k < 3; k++) sp.tiles[p][k] = h->box_tiles[p][k];
+  double coeff = 1.0;
+  for (int k = 0; k < h->dims; k++) coeff *= h->cfg.dx;
+  if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
+  sp.coeff = coeff;
+  sp.ls = h->cfg.dx / h->cfg.dt;
+  sp.cutoff = 2.0 * h->cfg.dx;
+  sp.shell = 1.5 * h->cfg.dx;
+  sp.dt = h->cfg.dt;
+  sp.dx = h->cfg.dx;
+  for (int k = 0; k < 3; k++) {
+    sp.x0[k] = h->x0[k];
+    sp.n[k] = h->n[k];
   }
   return a;
 }
 
-// Link masks of parity b from the particle records of parity b.
-hipError_t masks_for(lbm_handle *h, int b, hipStream_t stream) {
-  MaskArgs m;
-  m.g = geom(h);
-  m.parts = parts_of(h, b);
-  for (int k = 0; k < 3; k++) m.coords[k] = h->coords_d[k];
-  m.masks = h->masks + b * h->mask_total;
-  m.np = h->pl.np;
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
-    for (int k = 0; k < 3; k++) m.ext[p][k] = h->pl.ext[p][k];
-  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) m.off[p] = h->pl.mask_off[p];
-  return launch_masks(h->Q, m, stream);
-}
-
-// Particle side of a tick on stream2: links, integration, next masks.
-hipError_t particle_side(lbm_handle *h, int in) {
-  LinkArgs la;
-  la.fin = h->f[in];
-  la.fout = h->f[in ^ 1];
-  la.Ffield = h->F;
-  la.g = geom(h);
-  la.c = coll(h);
-  la.parts = parts_of(h, in);
-  for (int k = 0; k < 3; k++) la.coords[k] = h->coords_d[k];
-  la.partial = h->partial;
-  la.masks = h->masks + in * h->mask_total;
-  la.pl = h->pl;
-  Launch ll{dim3(h->link_blocks), dim3(h->pl.lx * h->pl.ly * h->pl.lz), h->stream2};
-  hipError_t e;
+hipError_t launch_box_kernel(lbm_handle *h, const StepArgs &a) {
+  Launch l{dim3(h->first_block[h->parts_h.size()]), h->box_block, h->stream};
   switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: e = launch_link<kF64>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
-    case LBM_STORAGE_F32: e = launch_link<kF32>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
-    default: e = launch_link<kF16>(h->Q, h->cfg.collision, h->cfg.force, la, ll); break;
+    case LBM_STORAGE_F64: return launch_box<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l);
+    case LBM_STORAGE_F32: return launch_box<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l);
+    default: return launch_box<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l);
   }
-  if (e != hipSuccess) return e;
-  IntegrateArgs ia;
-  ia.parts_in = parts_of(h, in);
-  ia.parts_out = parts_of(h, in ^ 1);
-  ia.partial = h->partial;
-  ia.np = (int32_t)h->parts_h.size();
-  ia.nblocks = h->link_blocks;
-  ia.dims = h->dims;
-  ia.semantics = h->cfg.semantics;
-  ia.dt = h->cfg.dt;
-  ia.dx = h->cfg.dx;
-  ia.coeff = h->pl.coeff;
-  ia.ls = h->cfg.dx / h->cfg.dt;
-  for (int k = 0; k < 3; k++) {
-    ia.x0[k] = h->x0[k];
-    ia.n[k] = h->n[k];
-  }
-  e = launch_integrate(ia, h->stream2);
-  // the next tick's link masks from the records just written
-  if (e == hipSuccess) e = masks_for(h, in ^ 1, h->stream2);
-  return e;
 }
 
-// One tick.  With particles, the link, integrate and mask kernels run on
-// stream2 concurrently with the step kernel (they touch disjoint slots): they
-// start after everything queued on the main stream (the previous tick) and
-// are enqueued first, and the main stream's next tick waits for them.
+// One tick: box_kernel over the particles' link boxes (it also moves the
+// particles into their records of the next parity), then the step kernel
+// over every other node.  The timing events bracket the step kernel.
 hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   StepArgs a = step_args(h, in, h->f[in ^ 1]);
   Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
   hipError_t e = hipSuccess;
-  const bool particles = !h->parts_h.empty();
-  if (particles) {
-    e = hipEventRecord(h->ev_fork, h->stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-    if (e == hipSuccess) e = particle_side(h, in);
-    if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->stream2);
+  if (!h->parts_h.empty()) {
+    e = launch_box_kernel(h, a);
     if (e != hipSuccess) return e;
   }
   if (e0) hipEventRecord(e0, h->stream);
@@ -229,7 +222,6 @@ hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   }
   if (e != hipSuccess) return e;
   if (e1) hipEventRecord(e1, h->stream);
-  if (particles) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
   return e;
 }
 
@@ -284,63 +276,6 @@ int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
   return LBM_OK;
 }
 
-int refresh_particle_launch(lbm_handle *h) {
-  const int np = (int)h->parts_h.size();
-  PartLaunch &pl = h->pl;
-  pl = PartLaunch{};
-  pl.np = np;
-  if (h->dims == 3) {
-    pl.lx = 8;
-    pl.ly = 8;
-    pl.lz = 4;
-  } else {
-    pl.lx = 16;
-    pl.ly = 16;
-    pl.lz = 1;
-  }
-  pl.semantics = h->cfg.semantics;
-  double coeff = 1.0;
-  for (int k = 0; k < h->dims; k++) coeff *= h->cfg.dx;
-  if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
-  pl.coeff = coeff;
-  pl.cutoff = 2.0 * h->cfg.dx;
-  pl.shell2 = (1.5 * h->cfg.dx) * (1.5 * h->cfg.dx);
-  const int32_t bl[3] = {pl.lx, pl.ly, pl.lz};
-  int blocks = 0;
-  for (int p = 0; p < np; p++) {
-    pl.first_block[p] = blocks;
-    int nb = 1;
-    for (int k = 0; k < 3; k++) {
-      const int32_t e = k < h->dims ? std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]) : 1;
-      pl.ext[p][k] = e;
-      pl.tiles[p][k] = (e + bl[k] - 1) / bl[k];
-      nb *= pl.tiles[p][k];
-    }
-    blocks += nb;
-  }
-  pl.first_block[np] = blocks;
-  h->link_blocks = blocks;
-  int64_t off = 0;
-  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) {
-    pl.mask_off[p] = off;
-    if (p < np) off += (int64_t)pl.ext[p][0] * pl.ext[p][1] * pl.ext[p][2];
-  }
-  const size_t need = (size_t)std::max(blocks, 1) * std::max(np, 1) * 6 * 2;  // double-double
-  if (need > h->partial_cap) {
-    if (h->partial) hipFree(h->partial);
-    h->partial = nullptr;
-    HIP_TRY(hipMalloc(&h->partial, need * sizeof(double)), "hipMalloc partials");
-    h->partial_cap = need;
-  }
-  if (off != h->mask_total) {
-    if (h->masks) hipFree(h->masks);
-    h->masks = nullptr;
-    HIP_TRY(hipMalloc(&h->masks, 2 * off * sizeof(uint2)), "hipMalloc link masks");
-    h->mask_total = off;
-  }
-  return LBM_OK;
-}
-
 int check_particles(lbm_handle *h) {
   if (h->parts_h.empty()) return LBM_OK;
   std::vector<DevParticle> tmp(h->parts_h.size());
@@ -381,13 +316,10 @@ int lbm_destroy(lbm_handle *h) {
   if (h->F) hipFree(h->F);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
-  if (h->masks) hipFree(h->masks);
+  if (h->ticket) hipFree(h->ticket);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
-  if (h->ev_fork) hipEventDestroy(h->ev_fork);
-  if (h->ev_join) hipEventDestroy(h->ev_join);
-  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return LBM_OK;
@@ -459,12 +391,11 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   }
   if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES * 2) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
+  if (hipMalloc(&h->ticket, sizeof(uint32_t) * LBM_MAX_PARTICLES) != hipSuccess)
+    return bail(fail(LBM_ENOMEM, "hipMalloc tickets"));
+  if (hipMemset(h->ticket, 0, sizeof(uint32_t) * LBM_MAX_PARTICLES) != hipSuccess)
+    return bail(fail(LBM_EDEVICE, "clear tickets"));
   h->tiling = default_tiling(h);
-  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(LBM_EDEVICE, "hipStreamCreate"));
-  if (hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
-    return bail(fail(LBM_EDEVICE, "hipEventCreate"));
   if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
     return bail(fail(LBM_EDEVICE, "hipEventCreate"));
   *out = h;
@@ -611,15 +542,12 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
   const int idx = (int)h->parts_h.size();
   HIP_TRY(hipMemcpy(parts_of(h, h->cur) + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
   h->parts_h.push_back(d);
-  int rc = refresh_particle_launch(h);
+  int rc = plan_boxes(h);
   if (rc) {
     h->parts_h.pop_back();
     return rc;
   }
   drop_graph(h);
-  // link masks of the current geometry (every particle: the layout changed)
-  HIP_TRY(masks_for(h, h->cur, h->stream), "link masks");
-  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   if (id) *id = idx;
   return LBM_OK;
 }
@@ -779,14 +707,17 @@ int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time) {
 
 int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz) {
   if (!h) return fail(LBM_EINVAL, "null handle");
+  Tiling t;
   if (bx == 0 && by == 0 && bz == 0) {
-    h->tiling = default_tiling(h);
+    t = default_tiling(h);
   } else {
-    const int64_t t = (int64_t)bx * by * bz;
-    if (bx < 1 || by < 1 || bz < 1 || t < 64 || t > 512)
+    const int64_t nt = (int64_t)bx * by * bz;
+    if (bx < 1 || by < 1 || bz < 1 || nt < 64 || nt > 512)
       return fail(LBM_EINVAL, "block shape must have 64..512 threads");
-    h->tiling = make_tiling(h, bx, by, bz);
+    t = make_tiling(h, bx, by, bz);
   }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  h->tiling = t;
   drop_graph(h);
   return LBM_OK;
 }

@@ -12,11 +12,10 @@ struct Launch {
   hipStream_t stream;
 };
 
-
 template <int S>
 hipError_t launch_step(int Q, int coll, int fm, const StepArgs &a, const Launch &l);
 template <int S>
-hipError_t launch_link(int Q, int coll, int fm, const LinkArgs &a, const Launch &l);
+hipError_t launch_box(int Q, int coll, int fm, const StepArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
@@ -28,7 +27,5 @@ hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
-hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream);
-hipError_t launch_masks(int Q, const MaskArgs &a, hipStream_t stream);
 
 }  // namespace lbm

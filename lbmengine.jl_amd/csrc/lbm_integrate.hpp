@@ -1,7 +1,7 @@
 // lbm_integrate.hpp — the particle side of a step: double-double
-// accumulation helpers for the momentum exchange, its fixed-order final
-// reduction, and moveParticles! (gpu/particles.jl:7-92,
-// cpu/particles.jl:45-100) with the link-box update.
+// accumulation of the momentum exchange, its fixed-order final reduction,
+// and moveParticles! (gpu/particles.jl:7-92, cpu/particles.jl:45-100) with
+// the link-box update.
 #pragma once
 
 #include "lbm_common.hpp"
@@ -32,28 +32,35 @@ __device__ __forceinline__ DD dd_merge(DD a, DD b) {
 }
 
 __device__ __forceinline__ DD wave_sum(DD v) {
+  const unsigned lane = __lane_id();
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     DD o{__shfl_xor(v.hi, off), __shfl_xor(v.lo, off)};
     // both lanes of a pair must form the same sum: order the operands
-    v = (threadIdx.x & off) ? dd_merge(o, v) : dd_merge(v, o);
+    v = (lane & off) ? dd_merge(o, v) : dd_merge(v, o);
   }
   return v;
 }
 
-// Particle records are double-buffered by tick parity: the step and link
-// kernels of tick t read buffer t&1 while the integrator writes t+1's, so a
-// step kernel still running on another stream never sees a half-updated
-// particle.
-struct IntegrateArgs {
+// What the kernels of a tick need for the particles.  Records are
+// double-buffered by tick parity: the kernels of tick t read parts_in while
+// the block that integrates writes the records of tick t+1 into parts_out.
+struct StepParticles {
   const DevParticle *parts_in;
   DevParticle *parts_out;
-  const double *partial;  // [block][np][6][hi, lo]
-  int32_t np, nblocks;
-  int32_t dims, semantics;
+  const double *coords[3];
+  double *partial;       // [box block][particle][6][hi, lo]
+  uint32_t *ticket;      // box blocks finished in the running tick
+  int32_t np, semantics, dims;
+  // box_kernel grid: particle p's blocks start at first_block[p] and tile
+  // its link box (from lo) with tiles[p][k] blocks along dimension k
+  int32_t first_block[LBM_MAX_PARTICLES + 1];
+  int32_t tiles[LBM_MAX_PARTICLES][3];
+  double coeff;          // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
+  double ls;             // lattice speed dx / dt
+  double cutoff;         // 2 dx (cpu-semantics node-velocity range)
+  double shell;          // 1.5 dx: link nodes lie within r +- shell of a centre
   double dt, dx;
-  double coeff;  // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
-  double ls;     // lattice speed dx / dt
   double x0[3];
   int32_t n[3];
 };
@@ -174,18 +181,23 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
   find_nv = any_nonzero(dL, nw) || moved;
 }
 
-// Run by one whole block of 256 threads (the last block of link_kernel).
-__device__ inline void integrate_block(const IntegrateArgs &a) {
-  __shared__ double tot[LBM_MAX_PARTICLES][6];
-  const int t = threadIdx.x, lane = t & 63;
-  // one wave per (particle, component): lanes fold strided partials, then an
-  // ordered butterfly; both orders are fixed, so the result is reproducible
-  for (int job = t >> 6; job < a.np * 6; job += blockDim.x >> 6) {
-    const int p = job / 6, k = job % 6;
+// Reduce particle p's box-block partials and move it (moveParticles!), run
+// by the whole box block that finished last.  Both reduction orders are
+// fixed, so the result is reproducible.
+__device__ inline void integrate_particle(const StepParticles &sp, int p) {
+  __shared__ double tot[6];
+  const int nblocks = gridDim.x;
+  __syncthreads();  // the previous particle's copy-out may still read rec
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  // block sizes are whole waves, so every wave reaches wave_sum with 64 lanes
+  const int wave = tid >> 6, nwaves = nthreads >> 6, lane = __lane_id();
+  for (int k = wave; k < 6; k += nwaves) {
     DD s{0.0, 0.0};
-    for (int b = lane; b < a.nblocks; b += 64) {
-      const double *src = a.partial + (((int64_t)b * a.np + p) * 6 + k) * 2;
-      s = dd_merge(s, DD{src[0], src[1]});
+    for (int b = lane; b < nblocks; b += 64) {
+      const double *src = sp.partial + (((int64_t)b * LBM_MAX_PARTICLES + p) * 6 + k) * 2;
+      s = dd_merge(s, DD{__hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)});
     }
     s = wave_sum(s);
     if (lane == 0) {
@@ -193,20 +205,29 @@ __device__ inline void integrate_block(const IntegrateArgs &a) {
       // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
       // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
       const double v = s.hi + s.lo;
-      tot[p][k] = -a.coeff * (v * a.ls);
+      tot[k] = -sp.coeff * (v * sp.ls);
     }
   }
+  // The record is updated in LDS by thread 0 (global memory round trips
+  // would put a chain of HBM latencies on the tick's critical path), then
+  // copied out by the whole block.
+  static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
+  constexpr int kWords = sizeof(DevParticle) / 8;
+  __shared__ uint64_t rec[kWords];
+  {
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(sp.parts_in + p);
+    for (int w = tid; w < kWords; w += nthreads) rec[w] = src[w];
+  }
   __syncthreads();
-  if (t != 0) return;
-  for (int p = 0; p < a.np; p++) {
-    DevParticle P = a.parts_in[p];
-    const double F[3] = {tot[p][0], tot[p][1], a.dims == 3 ? tot[p][2] : 0.0};
-    const double T[3] = {tot[p][3], a.dims == 3 ? tot[p][4] : 0.0, a.dims == 3 ? tot[p][5] : 0.0};
+  if (tid == 0) {
+    DevParticle &P = *reinterpret_cast<DevParticle *>(rec);
+    const double F[3] = {tot[0], tot[1], sp.dims == 3 ? tot[2] : 0.0};
+    const double T[3] = {tot[3], sp.dims == 3 ? tot[4] : 0.0, sp.dims == 3 ? tot[5] : 0.0};
     bool moved, find_nv;
-    if (a.semantics == LBM_SEMANTICS_GPU)
-      move_gpu(P, F, T, a.dims, a.dt, moved, find_nv);
+    if (sp.semantics == LBM_SEMANTICS_GPU)
+      move_gpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
     else
-      move_cpu(P, F, T, a.dims, a.dt, moved, find_nv);
+      move_cpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
     for (int k = 0; k < 3; k++) {
       P.force[k] = F[k];
       P.torque[k] = T[k];
@@ -214,7 +235,7 @@ __device__ inline void integrate_block(const IntegrateArgs &a) {
     if (moved) {
       for (int k = 0; k < 3; k++) P.Rg[k] = P.R[k];
       int32_t err = 0;
-      link_box(P.Rg, P.radius, a.x0, a.dx, a.n, a.dims, P.lo, P.hi, &err);
+      link_box(P.Rg, P.radius, sp.x0, sp.dx, sp.n, sp.dims, P.lo, P.hi, &err);
       if (err) P.err = 1;
     }
     if (find_nv) {
@@ -225,8 +246,10 @@ __device__ inline void integrate_block(const IntegrateArgs &a) {
         P.ne[k] = P.e[k];
       }
     }
-    a.parts_out[p] = P;
   }
+  __syncthreads();
+  uint64_t *dst = reinterpret_cast<uint64_t *>(sp.parts_out + p);
+  for (int w = tid; w < kWords; w += nthreads) dst[w] = rec[w];
 }
 
 }  // namespace lbm

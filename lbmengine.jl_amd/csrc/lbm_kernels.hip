@@ -1,4 +1,4 @@
-// lbm_kernels.hip — launchers of the step, link and auxiliary kernels for
+// lbm_kernels.hip — launchers of the step and auxiliary kernels for
 // one storage kind.  Compiled three times with -DLBM_STORE=0 (FP64),
 // 1 (FP32) and 2 (FP16).
 #include "lbm_dispatch.hpp"
@@ -38,8 +38,8 @@ hipError_t launch_step<S>(int Q, int coll, int fm, const StepArgs &a, const Laun
 }
 
 template <>
-hipError_t launch_link<S>(int Q, int coll, int fm, const LinkArgs &a, const Launch &l) {
-  LBM_DISPATCH_QCF(link_kernel, a);
+hipError_t launch_box<S>(int Q, int coll, int fm, const StepArgs &a, const Launch &l) {
+  LBM_DISPATCH_QCF(box_kernel, a);
   return hipGetLastError();
 }
 

@@ -1,20 +1,9 @@
 // lbm_particles.hip — the link-bit kernel used to list a particle's
-// boundary links (lbm_get_links).
+// boundary links (lbm_get_links), and the plain copy kernel that measures
+// the HBM ceiling.
 #include "lbm_dispatch.hpp"
 
 namespace lbm {
-
-// Fixed-order reduction of the link-kernel partials and moveParticles!, one
-// block; launched after link_kernel so the kernel boundary orders them.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-integrate_kernel(IntegrateArgs a) {
-  integrate_block(a);
-}
-
-hipError_t launch_integrate(const IntegrateArgs &a, hipStream_t stream) {
-  hipLaunchKernelGGL(integrate_kernel, dim3(1), dim3(256), 0, stream, a);
-  return hipGetLastError();
-}
 
 // Plain device copy of n16 16-byte words: the measured HBM ceiling, and the
 // kernel of known traffic (n16*16 bytes read and written) that scales the
@@ -39,17 +28,6 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
   const int blocks = 256 * 32;  // 32 blocks of 256 threads per CU, grid-stride beyond
   hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4 *>(src),
                      static_cast<uint4 *>(dst), n16);
-  return hipGetLastError();
-}
-
-hipError_t launch_masks(int Q, const MaskArgs &a, hipStream_t stream) {
-  const int64_t tot = a.off[a.np];
-  if (tot <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((tot + 255) / 256)), block(256);
-  if (Q == 9)
-    hipLaunchKernelGGL(mask_kernel<9>, grid, block, 0, stream, a);
-  else
-    hipLaunchKernelGGL(mask_kernel<19>, grid, block, 0, stream, a);
   return hipGetLastError();
 }
 

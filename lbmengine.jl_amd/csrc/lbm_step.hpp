@@ -1,16 +1,16 @@
 // lbm_step.hpp — the time step of LBMengine.jl as HIP kernels for gfx950.
 //
-// One step = tick! (gpu/fluids.jl:132-226):
-//   step_kernel   every node: load f_i(x) (one coalesced read per q),
-//                 hydro moments, BGK/TRT collision + Guo forcing, then push
-//                 f*_i(x) to x + c_i; static walls handled analytically
-//                 (zeroed wall layers, halfway bounce-back into own slots).
-//   link_kernel   nodes of each particle's link box only: halfway
-//                 bounce-back at particle links, the Ladd moving-wall term
-//                 with the squirmer slip, and the momentum exchange reduced
-//                 per block in FP64 (wavefront shuffles, then LDS).
-//   (integrate_kernel in lbm_particles.hip: fixed-order final reduction
-//                 and moveParticles!.)
+// One step = tick! (gpu/fluids.jl:132-226) is one launch of step_kernel:
+//   every node   load f_i(x) (one coalesced read per q), hydro moments,
+//                BGK/TRT collision + Guo forcing, then push f*_i(x) to
+//                x + c_i; static walls handled analytically (zeroed wall
+//                layers, halfway bounce-back into own slots).
+//   tiles that   additionally test the spheres: halfway bounce-back at
+//   touch a      particle links, the Ladd moving-wall term with the
+//   particle's   squirmer slip, and the momentum exchange reduced per block
+//   link box     in double-double; the block that finishes a particle's
+//                last box tile reduces the partials in a fixed order and
+//                moves the particle (moveParticles!) for the next tick.
 //
 // Populations are SoA in the reference's Julia layout, f[q*N + n] with
 // n = x + nx*(y + ny*z).  The stored state is the reference's
@@ -86,12 +86,7 @@ struct StepArgs {
   BoxGeom g;
   CollParams c;
   Tiling t;
-  // particles: the step kernel leaves their link slots to link_kernel
-  const DevParticle *parts;  // records of this tick's parity
-  const uint2 *masks;        // per link-box node: (pushes to skip, own link slots)
-  int32_t np;
-  int32_t mask_ext[LBM_MAX_PARTICLES][2];  // x, y extents of each box's mask
-  int64_t mask_off[LBM_MAX_PARTICLES];     // first mask entry of each particle
+  StepParticles sp;    // particles (sp.np == 0: none)
 };
 
 // ------------------------------------------------------------ collision
@@ -317,159 +312,7 @@ __device__ __forceinline__ bool solid_at(const double *const coords[3], const do
   return s < r2;
 }
 
-// -------------------------------------------------------------- step
-template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
-  using L = Lattice<Q>;
-  using T = typename Store<S>::T;
-  using R = typename Store<S>::R;
-  constexpr int D = L::D;
-  const BoxGeom &g = a.g;
-  const int tile = xcd_remap(blockIdx.x, a.t.ntiles);
-  const int tpx = tile % a.t.tx;
-  const int tpy = (tile / a.t.tx) % a.t.ty;
-  const int tpz = tile / (a.t.tx * a.t.ty);
-  const int x = tpx * a.t.bx + threadIdx.x;
-  const int y = tpy * a.t.by + threadIdx.y;
-  const int z = tpz * a.t.bz + threadIdx.z;
-  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  const int64_t N = g.N;
-  const int n = x + g.nx * (y + g.ny * z);
-  const T *__restrict__ fin = static_cast<const T *>(a.fin);
-  T *__restrict__ fout = static_cast<T *>(a.fout);
-
-  if (is_wall_node(g, x, y, z)) {
-    // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
-#pragma unroll
-    for (int q = 0; q < Q; q++)
-      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
-    return;
-  }
-
-  // Particle boundary links are written by link_kernel, which may run
-  // concurrently: a solid node does not push into a fluid neighbour of the
-  // same sphere (skip_push), and a fluid node does not bounce back into a
-  // slot that is a particle link (skip_own).  mask_kernel evaluated both for
-  // the nodes of each particle's link box.
-  uint32_t skip_push = 0, skip_own = 0;
-  for (int p = 0; p < a.np; p++) {
-    const DevParticle &P = a.parts[p];
-    if (x < P.lo[0] || x > P.hi[0] || y < P.lo[1] || y > P.hi[1] ||
-        (D == 3 && (z < P.lo[2] || z > P.hi[2])))
-      continue;
-    const uint2 m = a.masks[a.mask_off[p] + (x - P.lo[0]) +
-                            a.mask_ext[p][0] * ((y - P.lo[1]) + a.mask_ext[p][1] * (D == 3 ? z - P.lo[2] : 0))];
-    skip_push |= m.x;
-    skip_own |= m.y;
-  }
-
-  R fs[Q];
-  {
-    R f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
-    R Fn[3] = {0, 0, 0};
-    if constexpr (FM == LBM_FORCE_UNIFORM) {
-#pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
-    } else if constexpr (FM == LBM_FORCE_FIELD) {
-      const R *Ff = static_cast<const R *>(a.Ffield);
-#pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
-    }
-    R rho;
-    if constexpr (S == kF64)
-      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-    else
-      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-  }
-
-  // Periodic neighbour offsets.
-  const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
-  const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
-  const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
-  const int ox[3] = {xm, x, xp};
-  const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
-  const int nxy = g.nx * g.ny;
-  const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
-
-  // Flags of fluid nodes next to a wall layer.
-  const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-  const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-  const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
-  const bool near = xl | xh | yl | yh | zl | zh;
-
-  if (!near && !skip_push) {
-    // away from walls and sphere surfaces every population streams: no
-    // per-direction branches
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1];
-      fout[q * N + t] = encode<S, Q>(fs[q], q);
-    }
-    return;
-  }
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-    const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
-    bool push = true, own = false;
-    {
-      // x + c_q inside a wall layer: the wall keeps zero, no push.
-      push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
-               (cz < 0 && zl) || (cz > 0 && zh));
-      // x - c_q inside a wall layer: halfway bounce-back into own slot q
-      // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
-      own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
-            (cz > 0 && zl) || (cz < 0 && zh);
-      push = push && !((skip_push >> q) & 1);
-      own = own && !((skip_own >> q) & 1);
-    }
-    if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
-    if (own) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
-  }
-}
-
 // ------------------------------------------------------------ particles
-struct PartLaunch {
-  int32_t np;
-  int32_t ext[LBM_MAX_PARTICLES][3];   // link box extent per particle (nodes)
-  int32_t tiles[LBM_MAX_PARTICLES][3]; // tiles per dimension per particle
-  int32_t first_block[LBM_MAX_PARTICLES + 1];
-  int64_t mask_off[LBM_MAX_PARTICLES + 1]; // first mask entry per particle (ext-sized boxes)
-  int32_t lx, ly, lz;                  // link-kernel block shape
-  int32_t semantics;
-  double coeff;                        // dx^d / dt (gpu) or dx^d (cpu)
-  double cutoff;                       // 2 dx
-  double shell2;                       // (1.5 dx)^2 margin of the link shell
-};
-
-struct LinkArgs {
-  const void *fin;
-  void *fout;
-  const void *Ffield;
-  BoxGeom g;
-  CollParams c;
-  const DevParticle *parts;
-  const double *coords[3];
-  double *partial;       // [block][np][6][hi, lo]
-  const uint2 *masks;    // mask_kernel output of this tick (.y = own link slots)
-  PartLaunch pl;
-};
-
-// Own link slots of particle p at node (x, y, z) inside its link box: bit q
-// set when x is fluid and x - c_q solid.
-template <int D>
-__device__ __forceinline__ uint32_t own_links(const uint2 *masks, const PartLaunch &pl,
-                                              const DevParticle &P, int p, int x, int y, int z) {
-  if (x < P.lo[0] || x > P.hi[0] || y < P.lo[1] || y > P.hi[1] ||
-      (D == 3 && (z < P.lo[2] || z > P.hi[2])))
-    return 0;
-  return masks[pl.mask_off[p] + (x - P.lo[0]) +
-               pl.ext[p][0] * ((y - P.lo[1]) + pl.ext[p][1] * (D == 3 ? z - P.lo[2] : 0))]
-      .y;
-}
-
 // Node velocity of particle p at the solid node (x, y, z):
 // getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
 // (cpu/particles.jl:20-43), from the node-velocity snapshot.
@@ -547,106 +390,62 @@ __device__ __forceinline__ void node_velocity(const double *const coords[3], con
   }
 }
 
-// Per node of each particle's link box, the two link masks of this tick's
-// geometry (bounceBackPrep, gpu/aux.jl:187-198, with the getSphere test of
-// gpu/kernels.jl:344-353):
-//   .x  pushes a solid node must leave to link_kernel: bit q when x is solid
-//       and x + c_q fluid;
-//   .y  the node's own link slots: bit q when x is fluid and x - c_q solid.
-// Boxes are stored with the fixed extent ext (the largest box a sphere of
-// that radius can need), starting at the box's lo corner.
-struct MaskArgs {
-  BoxGeom g;
-  const DevParticle *parts;
-  const double *coords[3];
-  uint2 *masks;
-  int32_t np;
-  int32_t ext[LBM_MAX_PARTICLES][3];
-  int64_t off[LBM_MAX_PARTICLES + 1];
-};
-
-template <int Q>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) mask_kernel(MaskArgs a) {
-  using L = Lattice<Q>;
-  constexpr int D = L::D;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.off[a.np]) return;
-  int p = 0;
-  while (p + 1 < a.np && t >= a.off[p + 1]) p++;
-  const int64_t l = t - a.off[p];
-  const DevParticle &P = a.parts[p];
-  const int x = P.lo[0] + (int)(l % a.ext[p][0]);
-  const int y = P.lo[1] + (int)((l / a.ext[p][0]) % a.ext[p][1]);
-  const int z = (D == 3 ? P.lo[2] : 0) + (int)(l / ((int64_t)a.ext[p][0] * a.ext[p][1]));
-  uint2 m = make_uint2(0u, 0u);
-  if (x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2])) {
-    const double r2 = P.radius * P.radius;
-    const bool s0 = solid_at<D>(a.coords, P.Rg, r2, x, y, z);
-#pragma unroll
-    for (int q = 1; q < Q; q++) {
-      const int sx = wrapi(x + L::cx[q], a.g.nx), sy = wrapi(y + L::cy[q], a.g.ny),
-                sz = D == 3 ? wrapi(z + L::cz[q], a.g.nz) : 0;
-      const bool sn = solid_at<D>(a.coords, P.Rg, r2, sx, sy, sz);
-      if (s0 && !sn) m.x |= 1u << q;
-      if (!s0 && sn) m.y |= 1u << L::opp[q];
-    }
-  }
-  a.masks[t] = m;
+__device__ __forceinline__ bool in_box(const DevParticle &P, int x, int y, int z) {
+  return x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] && z >= P.lo[2] && z <= P.hi[2];
 }
 
-// Bounce-back, Ladd and momentum exchange at the particle links.  Runs after
-// step_kernel on the same buffers: the pushes step_kernel made into link
-// slots (from solid nodes) are overwritten here with f*_{q̄}(x) plus the
-// Ladd terms, as the reference overwrites `streamed[cIR]` (gpu/fluids.jl:
-// 157-207).  f*(x) is recomputed from the unchanged input buffer with the
-// same collision code, so it equals the value step_kernel pushed.  Each block
-// leaves one double-double partial of the momentum exchange per particle for
-// integrate_kernel.
+// -------------------------------------------------------------- boxes
+// The nodes of every particle's link box (box_kernel, launched before
+// step_kernel, which leaves these nodes alone).  Each node is collided and
+// streamed like anywhere else, except that the slots of particle links are
+// left to the link pass: a solid node does not push into a fluid neighbour
+// of the same sphere, and a fluid node does not bounce back off a wall into
+// a slot that is a particle link.  Each link slot then gets f*_{q̄}(x) plus
+// the Ladd terms of the particles it belongs to, in particle order
+// (gpu/fluids.jl:157-207), and the momentum exchange of each particle is
+// reduced over the block into one double-double partial per block.  The block
+// that finishes last reduces all partials in a fixed order and moves the
+// particles (moveParticles!) into the records of the next tick.
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) link_kernel(LinkArgs a) {
-  // (8 waves per SIMD: at most 64 VGPRs, so its waves fit in the slots the
-  // concurrently running step kernel frees)
+__global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   constexpr int D = L::D;
   const BoxGeom &g = a.g;
-  const PartLaunch &pl = a.pl;
-  // which particle box and which tile of it
+  const StepParticles &sp = a.sp;
+
+  // owner particle of this block and the block's tile of its box
   int pb = 0;
-  while (pb + 1 < pl.np && (int)blockIdx.x >= pl.first_block[pb + 1]) pb++;
-  const int tb = blockIdx.x - pl.first_block[pb];
-  const int tpx = tb % pl.tiles[pb][0];
-  const int tpy = (tb / pl.tiles[pb][0]) % pl.tiles[pb][1];
-  const int tpz = tb / (pl.tiles[pb][0] * pl.tiles[pb][1]);
-  const int lid = threadIdx.x;
-  const DevParticle &P = a.parts[pb];
-  const int x = P.lo[0] + tpx * pl.lx + lid % pl.lx;
-  const int y = P.lo[1] + tpy * pl.ly + (lid / pl.lx) % pl.ly;
-  const int z = (D == 3 ? P.lo[2] : 0) + tpz * pl.lz + lid / (pl.lx * pl.ly);
-
-  bool active = x <= P.hi[0] && y <= P.hi[1] && (D == 2 || z <= P.hi[2]) && x < g.nx &&
-                y < g.ny && z < g.nz && !is_wall_node(g, x, y, z);
-  // a node inside several boxes is handled by the first one
-  for (int p = 0; p < pb && active; p++) {
-    const DevParticle &Pp = a.parts[p];
-    if (x >= Pp.lo[0] && x <= Pp.hi[0] && y >= Pp.lo[1] && y <= Pp.hi[1] &&
-        (D == 2 || (z >= Pp.lo[2] && z <= Pp.hi[2])))
-      active = false;
-  }
-  // only nodes with a link slot of some particle need f*
-  bool any = false;
-  if (active)
-    for (int p = 0; p < pl.np; p++)
-      if (own_links<D>(a.masks, pl, a.parts[p], p, x, y, z)) any = true;
-  uint32_t written = 0;
-
+  while (pb + 1 < sp.np && (int)blockIdx.x >= sp.first_block[pb + 1]) pb++;
+  const int tb = blockIdx.x - sp.first_block[pb];
+  const int tpx = tb % sp.tiles[pb][0];
+  const int tpy = (tb / sp.tiles[pb][0]) % sp.tiles[pb][1];
+  const int tpz = tb / (sp.tiles[pb][0] * sp.tiles[pb][1]);
+  const DevParticle &B = sp.parts_in[pb];
+  const int x = B.lo[0] + tpx * blockDim.x + threadIdx.x;
+  const int y = B.lo[1] + tpy * blockDim.y + threadIdx.y;
+  const int z = B.lo[2] + tpz * blockDim.z + threadIdx.z;
+  // link boxes lie inside the lattice; a node in several boxes belongs to the first
+  bool mine = x <= B.hi[0] && y <= B.hi[1] && z <= B.hi[2];
+  for (int p = 0; p < pb; p++)
+    if (in_box(sp.parts_in[p], x, y, z)) mine = false;
+  const bool wall = mine && is_wall_node(g, x, y, z);
+  const bool node = mine && !wall;  // streamed and collided (particle interiors too)
   const int64_t N = g.N;
-  const int n = x + g.nx * (y + g.ny * z);
+  const int n = mine ? x + g.nx * (y + g.ny * z) : 0;
+  const T *__restrict__ fin = static_cast<const T *>(a.fin);
+  T *__restrict__ fout = static_cast<T *>(a.fout);
+
+  if (wall) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
+  }
+
   R fs[Q];
   R rho = 0;
-  if (any) {
-    const T *__restrict__ fin = static_cast<const T *>(a.fin);
+  if (node) {
     R f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
@@ -667,27 +466,86 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 #pragma unroll
     for (int q = 0; q < Q; q++) fs[q] = 0;
   }
-  // streamed value of every link slot, starting from the bounce-back value
+
+  // Geometry of this tick (bounceBackPrep with the getSphere test): only nodes
+  // within 1.5 dx of a sphere surface can be a link's solid or fluid end.
+  uint32_t skip_push = 0, own_any = 0;
+  uint32_t own[LBM_MAX_PARTICLES];
+#pragma unroll
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++) own[p] = 0;
+  if (node) {
+    for (int p = 0; p < sp.np; p++) {
+      const DevParticle &P = sp.parts_in[p];
+      if (!in_box(P, x, y, z)) continue;
+      const double d2 = dist2_at<D>(sp.coords, P.Rg, x, y, z);
+      const double r2 = P.radius * P.radius;
+      const double ro = P.radius + sp.shell, ri = P.radius - sp.shell;
+      if (d2 >= ro * ro || (ri > 0.0 && d2 < ri * ri)) continue;
+      const bool s0 = d2 < r2;
+      uint32_t op = 0;
+#pragma unroll
+      for (int q = 1; q < Q; q++) {
+        const int sx = wrapi(x + L::cx[q], g.nx), sy = wrapi(y + L::cy[q], g.ny),
+                  sz = D == 3 ? wrapi(z + L::cz[q], g.nz) : 0;
+        const bool sn = solid_at<D>(sp.coords, P.Rg, r2, sx, sy, sz);
+        if (s0 && !sn) skip_push |= 1u << q;  // x + c_q is a link node of P
+        if (!s0 && sn) op |= 1u << L::opp[q];  // (x, q̄) is a link slot of P
+      }
+#pragma unroll
+      for (int i = 0; i < LBM_MAX_PARTICLES; i++)
+        if (i == p) own[i] = op;
+      own_any |= op;
+    }
+
+    // streaming, wall bounce-back; particle link slots are left alone
+    const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+    const int ox[3] = {xm, x, xp};
+    const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
+    const int nxy = g.nx * g.ny;
+    const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
+    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
+      const bool push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
+                          (cz < 0 && zl) || (cz > 0 && zh)) &&
+                        !((skip_push >> q) & 1);
+      const bool bb = ((cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
+                       (cz > 0 && zl) || (cz < 0 && zh)) &&
+                      !((own_any >> q) & 1);
+      if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
+      if (bb) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
+    }
+  }
+
+  // Link slots: bounce-back, then the Ladd term of every particle the link
+  // belongs to, in particle order; the momentum exchange of each particle.
   R val[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) val[q] = fs[L::opp[q]];
-  const double X[3] = {active ? a.coords[0][x] : 0.0, active ? a.coords[1][y] : 0.0,
-                       (D == 3 && active) ? a.coords[2][z] : 0.0};
-
-  __shared__ DD red[8][6];
-  const int wave = lid >> 6, lane = lid & 63;
-  const int nwaves = (blockDim.x + 63) >> 6;
-  // particles in order: a link shared by two particles carries the Ladd
-  // terms of the earlier ones into the later one's momentum exchange
-  for (int p = 0; p < pl.np; p++) {
-    const DevParticle &Pp = a.parts[p];
+  const double X[3] = {node ? sp.coords[0][x] : 0.0, node ? sp.coords[1][y] : 0.0,
+                       (D == 3 && node) ? sp.coords[2][z] : 0.0};
+  __shared__ DD red[4][6];
+  __shared__ uint32_t last;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int wave = tid >> 6, lane = __lane_id();
+  const int nwaves = (blockDim.x * blockDim.y * blockDim.z) >> 6;  // 256 threads: 4 waves
+  for (int p = 0; p < sp.np; p++) {
+    const DevParticle &P = sp.parts_in[p];
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < LBM_MAX_PARTICLES; i++)
+      if (i == p) m = own[i];
     DD acc[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
-    const uint32_t m = any ? own_links<D>(a.masks, pl, Pp, p, x, y, z) : 0u;
-    written |= m;
     if (m) {
-      const bool coupled = Pp.couple_forces || Pp.couple_torques;
+      const bool coupled = P.couple_forces || P.couple_torques;
 #pragma unroll
       for (int q = 1; q < Q; q++) {
         if (!((m >> q) & 1)) continue;
@@ -698,7 +556,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
         const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
                   sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
         double ub[3] = {0.0, 0.0, 0.0};
-        node_velocity<D>(a.coords, Pp, pl.semantics, pl.cutoff, sx, sy, sz, ub);
+        node_velocity<D>(sp.coords, P, sp.semantics, sp.cutoff, sx, sy, sz, ub);
         {
 #pragma clang fp contract(off)
           double udc = 0.0;
@@ -719,13 +577,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
           const int c0 = L::cx[q], c1 = L::cy[q], c2 = L::cz[q];
           double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-          for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-Pp.R[k]));
-          if (Pp.couple_forces) {
+          for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-P.R[k]));
+          if (P.couple_forces) {
             if (c0) dd_add(acc[0], c0 > 0 ? s : -s);
             if (c1) dd_add(acc[1], c1 > 0 ? s : -s);
             if (D == 3 && c2) dd_add(acc[2], c2 > 0 ? s : -s);
           }
-          if (Pp.couple_torques) {
+          if (P.couple_torques) {
             if constexpr (D == 2) {
               if (c1) dd_add(acc[3], c1 > 0 ? pr[0] : -pr[0]);
               if (c0) dd_add(acc[3], c0 > 0 ? -pr[1] : pr[1]);
@@ -741,29 +599,137 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
         }
       }
     }
-    // block reduction, one double-double partial per particle and component
+    // this block's partial of particle p
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       const DD v = wave_sum(acc[k]);
       if (lane == 0) red[wave][k] = v;
     }
     __syncthreads();
-    if (lid < 6) {
-      DD s = red[0][lid];
-      for (int w = 1; w < nwaves; w++) s = dd_merge(s, red[w][lid]);
-      double *dst = a.partial + (((int64_t)blockIdx.x * pl.np + p) * 6 + lid) * 2;
-      dst[0] = s.hi;
-      dst[1] = s.lo;
+    if (tid < 6) {
+      DD s = red[0][tid];
+      for (int w = 1; w < nwaves; w++) s = dd_merge(s, red[w][tid]);
+      double *dst = sp.partial + (((int64_t)blockIdx.x * LBM_MAX_PARTICLES + p) * 6 + tid) * 2;
+      __hip_atomic_store(dst, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(dst + 1, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
   }
-  if (written) {
-    T *__restrict__ fout = static_cast<T *>(a.fout);
+  if (own_any) {
 #pragma unroll
     for (int q = 1; q < Q; q++)
-      if ((written >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
+      if ((own_any >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
   }
 
+  // Hand-over without cache flushes: the partials went out as device-coherent
+  // (write-through) stores from wave 0, which waits until they have completed
+  // before it takes a ticket.  The block that takes the last ticket reads
+  // every partial back with device-coherent loads and moves the particles.
+  if (wave == 0) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (tid == 0) {
+      const uint32_t done = __hip_atomic_fetch_add(sp.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = done == (uint32_t)gridDim.x - 1;
+    }
+  }
+  __syncthreads();
+  if (last) {
+    for (int p = 0; p < sp.np; p++) integrate_particle(sp, p);
+    if (tid == 0) __hip_atomic_store(sp.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// -------------------------------------------------------------- step
+// One tick of the lattice outside the particles' link boxes.
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
+  using L = Lattice<Q>;
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  constexpr int D = L::D;
+  const BoxGeom &g = a.g;
+  const int tile = xcd_remap(blockIdx.x, a.t.ntiles);
+  const int tpx = tile % a.t.tx;
+  const int tpy = (tile / a.t.tx) % a.t.ty;
+  const int tpz = tile / (a.t.tx * a.t.ty);
+  const int x = tpx * a.t.bx + threadIdx.x;
+  const int y = tpy * a.t.by + threadIdx.y;
+  const int z = tpz * a.t.bz + threadIdx.z;
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  for (int p = 0; p < a.sp.np; p++)
+    if (in_box(a.sp.parts_in[p], x, y, z)) return;  // box_kernel's node
+  const int64_t N = g.N;
+  const int n = x + g.nx * (y + g.ny * z);
+  const T *__restrict__ fin = static_cast<const T *>(a.fin);
+  T *__restrict__ fout = static_cast<T *>(a.fout);
+
+  if (is_wall_node(g, x, y, z)) {
+    // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
+    return;
+  }
+
+  R fs[Q];
+  {
+    R f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
+    R Fn[3] = {0, 0, 0};
+    if constexpr (FM == LBM_FORCE_UNIFORM) {
+#pragma unroll
+      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
+    } else if constexpr (FM == LBM_FORCE_FIELD) {
+      const R *Ff = static_cast<const R *>(a.Ffield);
+#pragma unroll
+      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
+    }
+    R rho;
+    if constexpr (S == kF64)
+      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    else
+      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+  }
+
+  // Periodic neighbour offsets.
+  const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+  const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+  const int ox[3] = {xm, x, xp};
+  const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
+  const int nxy = g.nx * g.ny;
+  const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
+
+  // Flags of fluid nodes next to a wall layer.
+  const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+  const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+  const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+  const bool near = xl | xh | yl | yh | zl | zh;
+
+  if (!near) {
+    // away from walls every population streams: no per-direction branches
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1];
+      fout[q * N + t] = encode<S, Q>(fs[q], q);
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+    const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
+    // x + c_q inside a wall layer: the wall keeps zero, no push.
+    const bool push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
+                        (cz < 0 && zl) || (cz > 0 && zh));
+    // x - c_q inside a wall layer: halfway bounce-back into own slot q
+    // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
+    const bool own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
+                     (cz > 0 && zl) || (cz < 0 && zh);
+    if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
+    if (own) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
+  }
 }
 
 // -------------------------------------------------------- aux kernels

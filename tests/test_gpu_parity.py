@@ -236,3 +236,53 @@ def test_autotune_keeps_state_and_results(gpu, oracle):
     g.run(12)
     o.run(12)
     compare_fields(g, o)
+
+
+@pytest.mark.parametrize("block", [(64, 1, 1), (16, 4, 1), (8, 4, 2), (32, 2, 4)])
+def test_particle_tiles_any_block_shape(gpu, oracle, block):
+    """Two squirmers whose link boxes overlap (a one-node gap, so some link
+    slots belong to both): every block shape splits their boxes into a
+    different set of tiles, and the results match the oracle."""
+    kw = dict(**KW3, relaxationTimeRatio=0.8, kineticParameter=3 / 16, xlims=(-13, 12), ylims=(-9, 10),
+              zlims=(-8, 9))
+    g, o = pair(gpu, oracle, seed=21, **kw)
+    g.set_block(*block)
+    add_both(g, o, "squirmer", radius=4.0, position=[-4.6, 0.3, 0.1], swimmingSpeed=2e-3, beta=1.0,
+             swimmingDirection=[1, 0.2, 0], coupleForces=True, coupleTorques=True)
+    add_both(g, o, "bead", radius=4.0, position=[4.6, -0.2, 0.3], coupleForces=True, coupleTorques=True)
+    g.run(16)
+    o.run(16)
+    compare_fields(g, o)
+    compare_particles(g, o)
+
+
+def test_d2q9_particle_tiles_block_shape(gpu, oracle):
+    kw = dict(**KW2, relaxationTimeRatio=0.8, collisionModel="trt", xlims=(-30, 29), ylims=(-20, 21))
+    g, o = pair(gpu, oracle, seed=4, **kw)
+    g.set_block(16, 8, 1)
+    add_both(g, o, "squirmer", radius=6.5, position=[0.4, 0.7], swimmingSpeed=3e-3, beta=-1.0,
+             swimmingDirection=[0.3, 1.0], coupleForces=True, coupleTorques=True)
+    g.run(30)
+    o.run(30)
+    compare_fields(g, o)
+    compare_particles(g, o)
+
+
+def test_autotune_with_particle_keeps_state(gpu, oracle):
+    kw = dict(**KW3, relaxationTimeRatio=0.8, kineticParameter=3 / 16, xlims=(-16, 15), ylims=(-12, 11),
+              zlims=(-10, 9))
+    g, o = pair(gpu, oracle, **kw)
+    sq = dict(radius=5.0, position=[0.3, 0.2, -0.1], swimmingSpeed=2e-3, beta=-1.0, swimmingDirection=[0, 0, 1],
+              coupleForces=True, coupleTorques=True)
+    add_both(g, o, "squirmer", **sq)
+    g.run(3)
+    o.run(3)
+    f0 = g.distributions
+    p0 = g.particles[0].position
+    g.autotune(2)
+    assert np.array_equal(g.distributions, f0)
+    assert np.array_equal(g.particles[0].position, p0)
+    g.run(10)
+    o.run(10)
+    compare_fields(g, o)
+    compare_particles(g, o)
