@@ -57,10 +57,8 @@ struct lbm_handle {
   double *partial = nullptr;     // [box block][particle][6][hi, lo]
   size_t partial_cap = 0;        // doubles
   uint32_t *ticket = nullptr;    // box blocks finished in the running tick
-  // box_kernel grid (see StepParticles)
+  // box blocks leading the step kernel's grid (see StepParticles)
   int32_t first_block[LBM_MAX_PARTICLES + 1] = {0};
-  int32_t box_tiles[LBM_MAX_PARTICLES][3] = {{0}};
-  dim3 box_block{1, 1, 1};
   Tiling tiling{};
   hipStream_t stream = nullptr;
   int64_t tick = 0;
@@ -118,32 +116,26 @@ Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
 Tiling default_tiling(const lbm_handle *h) {
   // A block spans whole x rows where rows are short, so the x-shifted pushes
   // of a row land in cache lines the same block writes.
-  int bx = h->n[0] <= 256 ? h->n[0] : 256;
-  int by = std::max(1, 512 / bx);
-  by = std::min(by, h->n[1]);
-  return make_tiling(h, bx, by, 1);
+  // Blocks are whole waves (the particle reductions rely on it).
+  const int bx = h->n[0] <= 256 ? h->n[0] : 256;
+  for (int by = std::min(512 / bx, h->n[1]); by >= 1; by--)
+    if ((bx * by) % 64 == 0) return make_tiling(h, bx, by, 1);
+  return make_tiling(h, 64, 1, 1);
 }
 
 // Particle records of ping-pong parity b (double-buffered: the integrator of
 // a tick writes the other parity while the step kernel may still read this one).
 DevParticle *parts_of(lbm_handle *h, int b) { return h->parts_d + b * LBM_MAX_PARTICLES; }
 
-// Grid of box_kernel: 256-thread blocks tiling each particle's link box
-// from its lower corner, enough of them for the largest box the particle's
-// radius allows; partials for every block.
-int plan_boxes(lbm_handle *h) {
-  h->box_block = h->dims == 3 ? dim3(8, 8, 4) : dim3(16, 16, 1);
-  const int32_t b[3] = {(int32_t)h->box_block.x, (int32_t)h->box_block.y, (int32_t)h->box_block.z};
+// Box blocks for blocks of `threads` threads: enough per particle to cover
+// the largest link box its radius allows; partials for every box block.
+int plan_boxes(lbm_handle *h, int64_t threads) {
   int32_t blocks = 0;
   for (size_t p = 0; p < h->parts_h.size(); p++) {
     h->first_block[p] = blocks;
-    int32_t nb = 1;
-    for (int k = 0; k < 3; k++) {
-      const int32_t e = k < h->dims ? std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]) : 1;
-      h->box_tiles[p][k] = (e + b[k] - 1) / b[k];
-      nb *= h->box_tiles[p][k];
-    }
-    blocks += nb;
+    int64_t vol = 1;
+    for (int k = 0; k < h->dims; k++) vol *= std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]);
+    blocks += (int32_t)((vol + threads - 1) / threads);
   }
   for (size_t p = h->parts_h.size(); p <= LBM_MAX_PARTICLES; p++) h->first_block[p] = blocks;
   const size_t need = (size_t)std::max(blocks, 1) * LBM_MAX_PARTICLES * 6 * 2;
@@ -157,6 +149,8 @@ int plan_boxes(lbm_handle *h) {
   }
   return LBM_OK;
 }
+
+int64_t tile_threads(const Tiling &t) { return (int64_t)t.bx * t.by * t.bz; }
 
 StepArgs step_args(lbm_handle *h, int in, void *fout) {
   StepArgs a;
@@ -176,8 +170,8 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   sp.semantics = h->cfg.semantics;
   sp.dims = h->dims;
   for (int p = 0; p <= LBM_MAX_PARTICLES; p++) sp.first_block[p] = h->first_block[p];
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++)
-    for (int k = 0; k < 3; k++) sp.tiles[p][k] = h->box_tiles[p][k];
+  sp.nbox = h->first_block[LBM_MAX_PARTICLES];
+  sp.nbox_pad = (sp.nbox + 7) / 8 * 8;
   double coeff = 1.0;
   for (int k = 0; k < h->dims; k++) coeff *= h->cfg.dx;
   if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
@@ -194,26 +188,25 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   return a;
 }
 
-hipError_t launch_box_kernel(lbm_handle *h, const StepArgs &a) {
-  Launch l{dim3(h->first_block[h->parts_h.size()]), h->box_block, h->stream};
-  switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: return launch_box<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l);
-    case LBM_STORAGE_F32: return launch_box<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l);
-    default: return launch_box<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l);
+// Launch shape of the step kernel for arguments a: box blocks first, then
+// the tiles; the box blocks' link lists take dynamic LDS.
+Launch step_launch(lbm_handle *h, const StepArgs &a) {
+  Launch l{dim3((unsigned)(a.sp.nbox_pad + h->tiling.ntiles)), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz),
+           h->stream};
+  if (a.sp.nbox) {
+    const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+    const size_t t = (size_t)tile_threads(h->tiling);
+    l.shmem = 5 * t * rsize + 2 * t * 4;  // 2t slots (f*, value, key) + t rho
   }
+  return l;
 }
 
-// One tick: box_kernel over the particles' link boxes (it also moves the
-// particles into their records of the next parity), then the step kernel
-// over every other node.  The timing events bracket the step kernel.
+// One tick: one launch of the step kernel, whose leading blocks also move the
+// particles into their records of the next parity.
 hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
   StepArgs a = step_args(h, in, h->f[in ^ 1]);
-  Launch l{dim3(h->tiling.ntiles), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
+  Launch l = step_launch(h, a);
   hipError_t e = hipSuccess;
-  if (!h->parts_h.empty()) {
-    e = launch_box_kernel(h, a);
-    if (e != hipSuccess) return e;
-  }
   if (e0) hipEventRecord(e0, h->stream);
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
@@ -542,7 +535,7 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
   const int idx = (int)h->parts_h.size();
   HIP_TRY(hipMemcpy(parts_of(h, h->cur) + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
   h->parts_h.push_back(d);
-  int rc = plan_boxes(h);
+  int rc = plan_boxes(h, tile_threads(h->tiling));
   if (rc) {
     h->parts_h.pop_back();
     return rc;
@@ -712,11 +705,13 @@ int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz) {
     t = default_tiling(h);
   } else {
     const int64_t nt = (int64_t)bx * by * bz;
-    if (bx < 1 || by < 1 || bz < 1 || nt < 64 || nt > 512)
-      return fail(LBM_EINVAL, "block shape must have 64..512 threads");
+    if (bx < 1 || by < 1 || bz < 1 || nt < 64 || nt > 512 || nt % 64)
+      return fail(LBM_EINVAL, "block shape must have 64..512 threads, a multiple of 64");
     t = make_tiling(h, bx, by, bz);
   }
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  int rc = plan_boxes(h, tile_threads(t));
+  if (rc) return rc;
   h->tiling = t;
   drop_graph(h);
   return LBM_OK;
@@ -756,9 +751,9 @@ int lbm_set_graph(lbm_handle *h, int32_t ticks_per_graph) {
   return LBM_OK;
 }
 
-// Autotuner: time the stream-collide kernel of this lattice over candidate
-// block shapes (reading the current populations, writing a scratch buffer,
-// so the simulation state is untouched) and keep the fastest.
+// Autotuner: time the step kernel of this lattice over candidate block
+// shapes (reading the current populations, writing a scratch buffer, so the
+// simulation state is untouched) and keep the fastest.
 int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) {
   if (!h) return fail(LBM_EINVAL, "null handle");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
@@ -776,7 +771,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     for (int32_t by : {1, 2, 4, 8, 16})
       for (int32_t bz : {1, 2, 4}) {
         const int64_t t = (int64_t)bx * by * bz;
-        if (t < 64 || t > 512 || bx > h->n[0] || by > h->n[1] || bz > h->n[2]) continue;
+        if (t < 64 || t > 512 || t % 64 || bx > h->n[0] || by > h->n[1] || bz > h->n[2]) continue;
         const Tiling tl = make_tiling(h, bx, by, bz);
         const double waste = (double)tl.ntiles * t / (double)h->N - 1.0;
         if (waste > 0.25) continue;
@@ -794,9 +789,21 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   Cand best_c = cands[0];
   hipError_t e = hipSuccess;
   for (const Cand &c : cands) {
-    h->tiling = make_tiling(h, c.bx, c.by, c.bz);
+    // the trial steps also move the particles, into the records of the other
+    // parity, which the next tick rewrites
+    const Tiling t = make_tiling(h, c.bx, c.by, c.bz);
+    const int rc = plan_boxes(h, tile_threads(t));
+    if (rc) {
+      h->tiling = saved;
+      plan_boxes(h, tile_threads(saved));
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      hipFree(scratch);
+      return rc;
+    }
+    h->tiling = t;
     StepArgs a = step_args(h, h->cur, scratch);
-    Launch l{dim3(h->tiling.ntiles), dim3(c.bx, c.by, c.bz), h->stream};
+    Launch l = step_launch(h, a);
     for (int r = 0; r < 2 && e == hipSuccess; r++) {  // warm-up
       switch (h->cfg.storage) {
         case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
@@ -827,9 +834,11 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   hipFree(scratch);
   if (e != hipSuccess) {
     h->tiling = saved;
+    plan_boxes(h, tile_threads(saved));
     return hip_fail(e, "autotune");
   }
   h->tiling = make_tiling(h, best_c.bx, best_c.by, best_c.bz);
+  if (const int rc = plan_boxes(h, tile_threads(h->tiling))) return rc;
   drop_graph(h);
   if (best) {
     best[0] = best_c.bx;

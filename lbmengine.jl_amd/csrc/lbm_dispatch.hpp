@@ -10,12 +10,11 @@ namespace lbm {
 struct Launch {
   dim3 grid, block;
   hipStream_t stream;
+  size_t shmem = 0;  // dynamic LDS bytes
 };
 
 template <int S>
 hipError_t launch_step(int Q, int coll, int fm, const StepArgs &a, const Launch &l);
-template <int S>
-hipError_t launch_box(int Q, int coll, int fm, const StepArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>

@@ -52,10 +52,11 @@ struct StepParticles {
   double *partial;       // [box block][particle][6][hi, lo]
   uint32_t *ticket;      // box blocks finished in the running tick
   int32_t np, semantics, dims;
-  // box_kernel grid: particle p's blocks start at first_block[p] and tile
-  // its link box (from lo) with tiles[p][k] blocks along dimension k
+  // box blocks: particle p's are [first_block[p], first_block[p + 1]);
+  // nbox = first_block[np] of them lead the grid, padded to nbox_pad (a
+  // multiple of the 8 XCDs)
   int32_t first_block[LBM_MAX_PARTICLES + 1];
-  int32_t tiles[LBM_MAX_PARTICLES][3];
+  int32_t nbox, nbox_pad;
   double coeff;          // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
   double ls;             // lattice speed dx / dt
   double cutoff;         // 2 dx (cpu-semantics node-velocity range)
@@ -186,7 +187,7 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
 // fixed, so the result is reproducible.
 __device__ inline void integrate_particle(const StepParticles &sp, int p) {
   __shared__ double tot[6];
-  const int nblocks = gridDim.x;
+  const int nblocks = sp.nbox;
   __syncthreads();  // the previous particle's copy-out may still read rec
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;

@@ -15,18 +15,18 @@ constexpr int S = LBM_STORE;
   do {                                                                                   \
     const int key = (Q == 19 ? 100 : 0) + coll * 10 + fm;                                \
     switch (key) {                                                                       \
-      case 0: hipLaunchKernelGGL((KERNEL<9, S, 0, 0>), l.grid, l.block, 0, l.stream, ARGS); break;    \
-      case 1: hipLaunchKernelGGL((KERNEL<9, S, 0, 1>), l.grid, l.block, 0, l.stream, ARGS); break;    \
-      case 2: hipLaunchKernelGGL((KERNEL<9, S, 0, 2>), l.grid, l.block, 0, l.stream, ARGS); break;    \
-      case 10: hipLaunchKernelGGL((KERNEL<9, S, 1, 0>), l.grid, l.block, 0, l.stream, ARGS); break;   \
-      case 11: hipLaunchKernelGGL((KERNEL<9, S, 1, 1>), l.grid, l.block, 0, l.stream, ARGS); break;   \
-      case 12: hipLaunchKernelGGL((KERNEL<9, S, 1, 2>), l.grid, l.block, 0, l.stream, ARGS); break;   \
-      case 100: hipLaunchKernelGGL((KERNEL<19, S, 0, 0>), l.grid, l.block, 0, l.stream, ARGS); break; \
-      case 101: hipLaunchKernelGGL((KERNEL<19, S, 0, 1>), l.grid, l.block, 0, l.stream, ARGS); break; \
-      case 102: hipLaunchKernelGGL((KERNEL<19, S, 0, 2>), l.grid, l.block, 0, l.stream, ARGS); break; \
-      case 110: hipLaunchKernelGGL((KERNEL<19, S, 1, 0>), l.grid, l.block, 0, l.stream, ARGS); break; \
-      case 111: hipLaunchKernelGGL((KERNEL<19, S, 1, 1>), l.grid, l.block, 0, l.stream, ARGS); break; \
-      case 112: hipLaunchKernelGGL((KERNEL<19, S, 1, 2>), l.grid, l.block, 0, l.stream, ARGS); break; \
+      case 0: hipLaunchKernelGGL((KERNEL<9, S, 0, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
+      case 1: hipLaunchKernelGGL((KERNEL<9, S, 0, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
+      case 2: hipLaunchKernelGGL((KERNEL<9, S, 0, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
+      case 10: hipLaunchKernelGGL((KERNEL<9, S, 1, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+      case 11: hipLaunchKernelGGL((KERNEL<9, S, 1, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+      case 12: hipLaunchKernelGGL((KERNEL<9, S, 1, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+      case 100: hipLaunchKernelGGL((KERNEL<19, S, 0, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 101: hipLaunchKernelGGL((KERNEL<19, S, 0, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 102: hipLaunchKernelGGL((KERNEL<19, S, 0, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 110: hipLaunchKernelGGL((KERNEL<19, S, 1, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 111: hipLaunchKernelGGL((KERNEL<19, S, 1, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 112: hipLaunchKernelGGL((KERNEL<19, S, 1, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
       default: return hipErrorInvalidValue;                                              \
     }                                                                                    \
   } while (0)
@@ -37,18 +37,12 @@ hipError_t launch_step<S>(int Q, int coll, int fm, const StepArgs &a, const Laun
   return hipGetLastError();
 }
 
-template <>
-hipError_t launch_box<S>(int Q, int coll, int fm, const StepArgs &a, const Launch &l) {
-  LBM_DISPATCH_QCF(box_kernel, a);
-  return hipGetLastError();
-}
-
 #define LBM_DISPATCH_Q(KERNEL, ARGS)                                                \
   do {                                                                              \
     if (Q == 9)                                                                     \
-      hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, 0, l.stream, ARGS);       \
+      hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, l.shmem, l.stream, ARGS);       \
     else                                                                            \
-      hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, 0, l.stream, ARGS);      \
+      hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, l.shmem, l.stream, ARGS);      \
   } while (0)
 
 template <>

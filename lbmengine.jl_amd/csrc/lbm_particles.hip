@@ -33,9 +33,9 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l) {
   if (Q == 9)
-    hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, 0, l.stream, a);
+    hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, l.shmem, l.stream, a);
   else
-    hipLaunchKernelGGL(link_mask_kernel<19>, l.grid, l.block, 0, l.stream, a);
+    hipLaunchKernelGGL(link_mask_kernel<19>, l.grid, l.block, l.shmem, l.stream, a);
   return hipGetLastError();
 }
 

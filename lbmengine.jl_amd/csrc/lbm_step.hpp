@@ -61,6 +61,16 @@ __device__ __forceinline__ typename Store<S>::T encode(typename Store<S>::R v, i
     return v;
 }
 
+// encode() for a moving population (weight class 1 or 2) known at run time.
+template <int S, int Q>
+__device__ __forceinline__ typename Store<S>::T encode_class(typename Store<S>::R v, uint32_t cls) {
+  if constexpr (S == kF16)
+    return static_cast<_Float16>(
+        v * static_cast<float>(kF16Scale * (cls == 1 ? Lattice<Q>::invw(1) : Lattice<Q>::invw(Q - 1))));
+  else
+    return v;
+}
+
 // Population f_i in FP64 from a stored element.
 template <int S, int Q>
 __device__ __forceinline__ double to_f64(typename Store<S>::T v, int i, double rho0) {
@@ -395,39 +405,41 @@ __device__ __forceinline__ bool in_box(const DevParticle &P, int x, int y, int z
 }
 
 // -------------------------------------------------------------- boxes
-// The nodes of every particle's link box (box_kernel, launched before
-// step_kernel, which leaves these nodes alone).  Each node is collided and
+// The nodes of every particle's link box are handled by the first blocks of
+// the step kernel's grid (dispatched first, spread over all XCDs), while the
+// other blocks stream the rest of the lattice.  Each box node is collided and
 // streamed like anywhere else, except that the slots of particle links are
-// left to the link pass: a solid node does not push into a fluid neighbour
-// of the same sphere, and a fluid node does not bounce back off a wall into
-// a slot that is a particle link.  Each link slot then gets f*_{q̄}(x) plus
-// the Ladd terms of the particles it belongs to, in particle order
+// left to the link pass: a solid node does not push into a fluid neighbour of
+// the same sphere, and a fluid node does not bounce back off a wall into a
+// slot that is a particle link.  Each link slot then gets f*_{q̄}(x) plus the
+// Ladd terms of the particles it belongs to, in particle order
 // (gpu/fluids.jl:157-207), and the momentum exchange of each particle is
-// reduced over the block into one double-double partial per block.  The block
-// that finishes last reduces all partials in a fixed order and moves the
-// particles (moveParticles!) into the records of the next tick.
+// reduced over the block into one double-double partial per block.  The box
+// block that finishes last reduces all partials in a fixed order and moves
+// the particles (moveParticles!) into the records of the next tick.
+//
+// Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's link
+// box in x-fastest order (j = bid - first_block[pb], T threads per block).
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
+__device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   constexpr int D = L::D;
   const BoxGeom &g = a.g;
   const StepParticles &sp = a.sp;
+  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 
-  // owner particle of this block and the block's tile of its box
   int pb = 0;
-  while (pb + 1 < sp.np && (int)blockIdx.x >= sp.first_block[pb + 1]) pb++;
-  const int tb = blockIdx.x - sp.first_block[pb];
-  const int tpx = tb % sp.tiles[pb][0];
-  const int tpy = (tb / sp.tiles[pb][0]) % sp.tiles[pb][1];
-  const int tpz = tb / (sp.tiles[pb][0] * sp.tiles[pb][1]);
+  while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
   const DevParticle &B = sp.parts_in[pb];
-  const int x = B.lo[0] + tpx * blockDim.x + threadIdx.x;
-  const int y = B.lo[1] + tpy * blockDim.y + threadIdx.y;
-  const int z = B.lo[2] + tpz * blockDim.z + threadIdx.z;
+  const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1, ez = B.hi[2] - B.lo[2] + 1;
+  const int base = (bid - sp.first_block[pb]) * nthreads;
+  const int li = base + tid;
+  const int x = B.lo[0] + li % ex, y = B.lo[1] + (li / ex) % ey, z = B.lo[2] + li / (ex * ey);
   // link boxes lie inside the lattice; a node in several boxes belongs to the first
-  bool mine = x <= B.hi[0] && y <= B.hi[1] && z <= B.hi[2];
+  bool mine = li < ex * ey * ez;
   for (int p = 0; p < pb; p++)
     if (in_box(sp.parts_in[p], x, y, z)) mine = false;
   const bool wall = mine && is_wall_node(g, x, y, z);
@@ -523,40 +535,83 @@ __global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
     }
   }
 
-  // Link slots: bounce-back, then the Ladd term of every particle the link
-  // belongs to, in particle order; the momentum exchange of each particle.
-  R val[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) val[q] = fs[L::opp[q]];
-  const double X[3] = {node ? sp.coords[0][x] : 0.0, node ? sp.coords[1][y] : 0.0,
-                       (D == 3 && node) ? sp.coords[2][z] : 0.0};
-  __shared__ DD red[4][6];
+  // Link slots, one lane per slot: the block's slots are listed in LDS in a
+  // fixed order (node, then q), each starting from the bounce-back value
+  // f*_{q̄}(x); the particles then add their Ladd terms in particle order
+  // and take their momentum exchange from the running value.  The list holds
+  // 2 T slots at a time; more are taken in further rounds.
+  const int cap = 2 * nthreads;
+  extern __shared__ double lds_dyn[];
+  R *fs_l = reinterpret_cast<R *>(lds_dyn);          // f*_{q̄}(x) of the slot
+  R *v_l = fs_l + cap;                               // streamed value of the slot so far
+  R *rho_l = v_l + cap;                              // rho of each lane's node
+  uint32_t *pk_l = reinterpret_cast<uint32_t *>(rho_l + nthreads);
+  // pk: lane | q << 9 | (c + 1) << 14, 16, 18 | weight class << 20 | owners << 22
+  __shared__ uint32_t wave_cnt[8];
+  __shared__ DD red[8][6];
+  __shared__ DD blk[LBM_MAX_PARTICLES][6];
   __shared__ uint32_t last;
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int wave = tid >> 6, lane = __lane_id();
-  const int nwaves = (blockDim.x * blockDim.y * blockDim.z) >> 6;  // 256 threads: 4 waves
-  for (int p = 0; p < sp.np; p++) {
-    const DevParticle &P = sp.parts_in[p];
-    uint32_t m = 0;
+  const int nwaves = (nthreads + 63) >> 6;
+
+  const uint32_t cnt = __popc(own_any);
+  uint32_t incl = cnt;  // inclusive scan over the wave
 #pragma unroll
-    for (int i = 0; i < LBM_MAX_PARTICLES; i++)
-      if (i == p) m = own[i];
-    DD acc[6];
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63 || tid == nthreads - 1) wave_cnt[wave] = incl;
+  rho_l[tid] = rho;
+  if (tid < 6 * LBM_MAX_PARTICLES) blk[tid / 6][tid % 6] = DD{0.0, 0.0};
+  __syncthreads();
+  uint32_t off = incl - cnt, total = 0;
+  for (int w = 0; w < nwaves; w++) {
+    if (w < wave) off += wave_cnt[w];
+    total += wave_cnt[w];
+  }
+  const int lx0 = B.lo[0], ly0 = B.lo[1], lz0 = B.lo[2];
+  for (uint32_t c0 = 0; c0 < total; c0 += cap) {
+    const int m = (int)min((uint32_t)cap, total - c0);
+    uint32_t idx = off;
 #pragma unroll
-    for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
-    if (m) {
+    for (int q = 1; q < Q; q++) {
+      if ((own_any >> q) & 1) {
+        if (idx >= c0 && idx < c0 + cap) {
+          uint32_t owners = 0;
+#pragma unroll
+          for (int i = 0; i < LBM_MAX_PARTICLES; i++) owners |= ((own[i] >> q) & 1u) << i;
+          fs_l[idx - c0] = fs[L::opp[q]];
+          v_l[idx - c0] = fs[L::opp[q]];
+          pk_l[idx - c0] = (uint32_t)tid | (uint32_t)q << 9 | (uint32_t)(L::cx[q] + 1) << 14 |
+                           (uint32_t)(L::cy[q] + 1) << 16 | (uint32_t)(L::cz[q] + 1) << 18 |
+                           (uint32_t)L::wclass[q] << 20 | owners << 22;
+        }
+        idx++;
+      }
+    }
+    __syncthreads();
+    for (int p = 0; p < sp.np; p++) {
+      const DevParticle &P = sp.parts_in[p];
       const bool coupled = P.couple_forces || P.couple_torques;
+      DD acc[6];
 #pragma unroll
-      for (int q = 1; q < Q; q++) {
-        if (!((m >> q) & 1)) continue;
-        const int o = L::opp[q];
-        const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
-        const double lc = 2.0 * L::w(q) * a.c.invC2s;
-        const double shift = S == kF64 ? 0.0 : 2.0 * L::w(q) * a.c.rho0;
-        const int sx = wrapi(x - L::cx[q], g.nx), sy = wrapi(y - L::cy[q], g.ny),
-                  sz = D == 3 ? wrapi(z - L::cz[q], g.nz) : 0;
+      for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
+      for (int e = tid; e < m; e += nthreads) {
+        const uint32_t pk = pk_l[e];
+        if (!((pk >> (22 + p)) & 1)) continue;
+        const int t = pk & 511, lt = base + t;
+        const int lx = lx0 + lt % ex, ly = ly0 + (lt / ex) % ey, lz = lz0 + lt / (ex * ey);
+        const int c0 = (int)((pk >> 14) & 3) - 1, c1 = (int)((pk >> 16) & 3) - 1,
+                  c2 = (int)((pk >> 18) & 3) - 1;
+        const double wq = ((pk >> 20) & 3) == 1 ? L::w(1) : L::w(Q - 1);
+        const double ci[3] = {c0 * a.c.ls, c1 * a.c.ls, c2 * a.c.ls};
+        const double lc = 2.0 * wq * a.c.invC2s;
+        const double shift = S == kF64 ? 0.0 : 2.0 * wq * a.c.rho0;
+        const int sx = wrapi(lx - c0, g.nx), sy = wrapi(ly - c1, g.ny), sz = D == 3 ? wrapi(lz - c2, g.nz) : 0;
         double ub[3] = {0.0, 0.0, 0.0};
         node_velocity<D>(sp.coords, P, sp.semantics, sp.cutoff, sx, sy, sz, ub);
+        R v = v_l[e];
         {
 #pragma clang fp contract(off)
           double udc = 0.0;
@@ -564,17 +619,18 @@ __global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
           for (int k = 0; k < D; k++) udc = udc + ub[k] * ci[k];
           // Ladd: streamed += 2 w_q / c_s^2 rho(x) (u_b . c_q) (gpu/fluids.jl:188-189)
           if constexpr (S == kF64)
-            val[q] = val[q] + lc * static_cast<double>(rho) * udc;
+            v = v + lc * static_cast<double>(rho_l[t]) * udc;
           else
-            val[q] = val[q] + static_cast<R>(lc * static_cast<double>(rho) * udc);
+            v = v + static_cast<R>(lc * static_cast<double>(rho_l[t]) * udc);
         }
+        v_l[e] = v;
         if (coupled) {
 #pragma clang fp contract(off)
           // s = f*_{q̄}(x) + f_q(x) (gpu/fluids.jl:193); the loads are
           // -coeff * sum(s) * c_q and -coeff * cross(sum(s (x - R)), c_q),
           // formed from these exact sums by the integrator
-          const double s = static_cast<double>(fs[o]) + static_cast<double>(val[q]) + shift;
-          const int c0 = L::cx[q], c1 = L::cy[q], c2 = L::cz[q];
+          const double s = static_cast<double>(fs_l[e]) + static_cast<double>(v) + shift;
+          const double X[3] = {sp.coords[0][lx], sp.coords[1][ly], D == 3 ? sp.coords[2][lz] : 0.0};
           double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-P.R[k]));
@@ -598,38 +654,46 @@ __global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
           }
         }
       }
-    }
-    // this block's partial of particle p
+      // this round's part of the block's partial of particle p
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const DD v = wave_sum(acc[k]);
-      if (lane == 0) red[wave][k] = v;
+      for (int k = 0; k < 6; k++) {
+        const DD v = wave_sum(acc[k]);
+        if (lane == 0) red[wave][k] = v;
+      }
+      __syncthreads();
+      if (tid < 6) {
+        DD s = red[0][tid];
+        for (int w = 1; w < nwaves; w++) s = dd_merge(s, red[w][tid]);
+        blk[p][tid] = dd_merge(blk[p][tid], s);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (tid < 6) {
-      DD s = red[0][tid];
-      for (int w = 1; w < nwaves; w++) s = dd_merge(s, red[w][tid]);
-      double *dst = sp.partial + (((int64_t)blockIdx.x * LBM_MAX_PARTICLES + p) * 6 + tid) * 2;
-      __hip_atomic_store(dst, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(dst + 1, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the link slots' streamed values
+    for (int e = tid; e < m; e += nthreads) {
+      const uint32_t pk = pk_l[e];
+      const int t = pk & 511, q = (pk >> 9) & 31, lt = base + t;
+      const int lx = lx0 + lt % ex, ly = ly0 + (lt / ex) % ey, lz = lz0 + lt / (ex * ey);
+      fout[q * N + (lx + g.nx * (ly + g.ny * lz))] = encode_class<S, Q>(v_l[e], (pk >> 20) & 3);
     }
-    __syncthreads();
+    __syncthreads();  // the next round reuses the list
   }
-  if (own_any) {
-#pragma unroll
-    for (int q = 1; q < Q; q++)
-      if ((own_any >> q) & 1) fout[q * N + n] = encode<S, Q>(val[q], q);
+  if (tid < 6 * sp.np) {
+    const DD s = blk[tid / 6][tid % 6];
+    double *dst = sp.partial + ((int64_t)bid * LBM_MAX_PARTICLES * 6 + tid) * 2;
+    __hip_atomic_store(dst, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(dst + 1, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   // Hand-over without cache flushes: the partials went out as device-coherent
-  // (write-through) stores from wave 0, which waits until they have completed
-  // before it takes a ticket.  The block that takes the last ticket reads
-  // every partial back with device-coherent loads and moves the particles.
+  // (write-through) stores from wave 0 (tid < 6 np <= 48), which waits until
+  // they have completed before it takes a ticket.  The block that takes the
+  // last ticket reads every partial back with device-coherent loads and moves
+  // the particles.
   if (wave == 0) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (tid == 0) {
       const uint32_t done = __hip_atomic_fetch_add(sp.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = done == (uint32_t)gridDim.x - 1;
+      last = done == (uint32_t)sp.nbox - 1;
     }
   }
   __syncthreads();
@@ -640,15 +704,24 @@ __global__ void __launch_bounds__(256) box_kernel(StepArgs a) {
 }
 
 // -------------------------------------------------------------- step
-// One tick of the lattice outside the particles' link boxes.
+// One tick of the lattice: blocks [0, nbox_pad) work on the particles' link
+// boxes, the others each stream one tile outside them.  Registers are capped
+// at the plain path's needs (FP32/FP16: 64, eight waves per SIMD; FP64: 128,
+// four), so the wider box path spills rather than lowering the occupancy of
+// every tile.
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(512) step_kernel(StepArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
+step_kernel(StepArgs a) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   constexpr int D = L::D;
   const BoxGeom &g = a.g;
-  const int tile = xcd_remap(blockIdx.x, a.t.ntiles);
+  if ((int)blockIdx.x < a.sp.nbox_pad) {
+    if ((int)blockIdx.x < a.sp.nbox) box_work<Q, S, COLL, FM>(a, blockIdx.x);
+    return;
+  }
+  const int tile = xcd_remap(blockIdx.x - a.sp.nbox_pad, a.t.ntiles);
   const int tpx = tile % a.t.tx;
   const int tpy = (tile / a.t.tx) % a.t.ty;
   const int tpz = tile / (a.t.tx * a.t.ty);
