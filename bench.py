@@ -6,9 +6,10 @@ position and orientation uniform from the seed), Ladd moving-wall bounce-back
 with momentum-exchange force/torque, populations stored as FP16 deviations
 with FP32 collision.  One step = one tick! of the reference.
 
-Prints ONE JSON line: MLUPS (all nodes x steps / s), the roofline of the
-stream-collide kernel (algorithmic bytes 2*Q*sizeof(storage) per node over its
-HIP-event-timed average duration), the CPU baseline (the oracle, a C
+Prints ONE JSON line: MLUPS (all nodes x steps / wall second on the default
+run path, hipGraph replays of the one-launch tick), the roofline of the step
+kernel (algorithmic bytes 2*Q*sizeof(storage) per node over its average
+duration from HIP events on its stream), the CPU baseline (the oracle, a C
 restatement of the reference step, on this host's cores), FP64/FP32 variants
 of the same step, and the FP16-vs-FP64 errors after --accuracy-steps steps.
 """
@@ -51,17 +52,65 @@ def build_model(lh, storage, args):
     return m
 
 
-def timed(lh, m, steps, warmup):
+def run_wall(m, steps, warmup):
+    """Wall time of `steps` ticks on the default path (hipGraph replays of
+    the step kernel), after `warmup` ticks, synchronised on both sides."""
     m.run(warmup)
     m.sync()
-    m.set_timing(True)
     t0 = time.perf_counter()
     m.run(steps)
     m.sync()
-    t1 = time.perf_counter()
+    return time.perf_counter() - t0
+
+
+def kernel_times(m, steps):
+    """Average device time of the step kernel and of a whole tick, from HIP
+    events recorded on the kernel's stream around every step kernel."""
+    m.set_timing(True)
+    m.run(steps)
+    m.sync()
     m.set_timing(False)
-    kms, sms = m.last_run_times()
-    return t1 - t0, kms, sms
+    return m.last_run_times()
+
+
+def _pmc_means(run_dir):
+    """Mean counter value per kernel family in a rocprofv3 --pmc output folder."""
+    import csv
+    from collections import defaultdict
+    files = sorted(Path(run_dir).glob("**/*counter_collection.csv"))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {run_dir}")
+    acc = defaultdict(list)
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            name = r["Kernel_Name"]
+            fam = "step_kernel" if "step_kernel" in name else ("copy_kernel" if "copy_kernel" in name else None)
+            if fam:
+                acc[(fam, r["Counter_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+
+def traffic_summary(fetch_dir, write_dir, storage, n):
+    """HBM bytes per step-kernel launch from two rocprofv3 --pmc runs of this
+    script (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by
+    what the same counter reads for the copy kernel, whose traffic is known
+    (it reads and writes half the step's algorithmic bytes)."""
+    f, fc = _pmc_means(fetch_dir)
+    w, wc = _pmc_means(write_dir)
+    bytes_per_launch = 2 * 19 * BYTES[storage] * n ** 3
+    copy_bytes = bytes_per_launch // 2 // 16 * 16
+    kf = copy_bytes / (f[("copy_kernel", "FETCH_SIZE")] * 1024)
+    kw = copy_bytes / (w[("copy_kernel", "WRITE_SIZE")] * 1024)
+    rd = f[("step_kernel", "FETCH_SIZE")] * 1024 * kf
+    wr = w[("step_kernel", "WRITE_SIZE")] * 1024 * kw
+    return {"kernel": "step_kernel", "storage": storage, "n": n,
+            "hbm_bytes_per_launch": round(rd + wr), "read_bytes": round(rd), "write_bytes": round(wr),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "traffic_over_algorithmic": round((rd + wr) / bytes_per_launch, 4),
+            "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
+            "raw_kib": {"step_fetch": f[("step_kernel", "FETCH_SIZE")], "step_write": w[("step_kernel", "WRITE_SIZE")],
+                        "copy_fetch": f[("copy_kernel", "FETCH_SIZE")], "copy_write": w[("copy_kernel", "WRITE_SIZE")]},
+            "dispatches": {"step_fetch": fc[("step_kernel", "FETCH_SIZE")], "step_write": wc[("step_kernel", "WRITE_SIZE")]}}
 
 
 def cpu_baseline(args):
@@ -102,7 +151,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--block", default="", help="bx,by,bz of the step kernel instead of autotuning")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs of this script into the traffic file and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        out = traffic_summary(*args.traffic_from, args.storage, args.n)
+        (ROOT / "profiles").mkdir(exist_ok=True)
+        dst = ROOT / "profiles" / f"traffic_step_{args.storage}_{args.n}.json"
+        dst.write_text(json.dumps(out, indent=1) + "\n")
+        print(json.dumps(out))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -118,14 +176,10 @@ def main():
         tuned_ms = float("nan")
     else:
         block, tuned_ms = m.autotune(20)
-    wall, kms, sms = timed(lh, m, args.steps, args.warmup)
+    wall = run_wall(m, args.steps, args.warmup)
     mlups = nodes * args.steps / wall / 1e6
-    # the same run length again from captured hipGraphs (no per-kernel events)
-    m.sync()
-    t0 = time.perf_counter()
-    m.run(args.steps)
-    m.sync()
-    graph_mlups = nodes * args.steps / (time.perf_counter() - t0) / 1e6
+    # the roofline's kernel time: HIP events around every step kernel
+    kms, sms = kernel_times(m, min(args.steps, 1000))
     bytes_per_launch = 2 * Q * BYTES[args.storage] * nodes
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     traffic = None
@@ -143,7 +197,7 @@ def main():
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
                    "nodes": nodes, "block": list(m.get_block()), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
-        "graph_MLUPS": round(graph_mlups, 1),
+        "event_timed_MLUPS": round(nodes / (sms * 1e-3) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
@@ -160,8 +214,10 @@ def main():
                 continue
             mv = build_model(lh, st, args)
             mv.autotune(10)
-            w, k, s = timed(lh, mv, max(args.steps // 4, 100), args.warmup // 4)
-            var[st] = {"MLUPS": round(nodes * max(args.steps // 4, 100) / w / 1e6, 1), "kernel_ms": round(k, 5),
+            ks = max(args.steps // 4, 100)
+            w = run_wall(mv, ks, args.warmup // 4)
+            k, _ = kernel_times(mv, min(ks, 500))
+            var[st] = {"MLUPS": round(nodes * ks / w / 1e6, 1), "kernel_ms": round(k, 5),
                        "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del mv
         var["speedup_f16_over_f64"] = round(var["f16"]["MLUPS"] / var["f64"]["MLUPS"], 2)
