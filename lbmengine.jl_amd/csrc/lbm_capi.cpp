@@ -374,6 +374,9 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
       return bail(fail(LBM_EDEVICE, "copy coordinates"));
   }
   h->cfg.coords[0] = h->cfg.coords[1] = h->cfg.coords[2] = nullptr;
+  if ((uint64_t)N * elem_size(cfg->storage) >= (1ull << 31))
+    return bail(fail(LBM_EINVAL, "lattice too large: one population plane (nodes x element size) must stay "
+                                 "below 2 GiB"));
   const size_t bytes = (size_t)N * cfg->q * elem_size(cfg->storage);
   for (int b = 0; b < 2; b++)
     if (hipMalloc(&h->f[b], bytes) != hipSuccess) return bail(fail(LBM_ENOMEM, "hipMalloc populations"));

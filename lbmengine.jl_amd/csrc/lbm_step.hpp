@@ -44,6 +44,41 @@ struct Store<kF16> {
   using R = float;
 };
 
+// Population planes are addressed through buffer resources: one descriptor
+// per plane in scalar registers and one 32-bit byte offset per lane, so the
+// 2 Q accesses of a node cost no 64-bit address arithmetic (the step is
+// close to VALU-bound otherwise).  Planes are below 2^31 bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, uint32_t bytes) {
+  // 0x00020000: the gfx9 raw-buffer word 3 (32-bit data format, no swizzle)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+
+// Load of a population that the step reads exactly once (cache policy
+// kStreamPolicy; 2 would mark it non-temporal).
+constexpr int kStreamPolicy = 0;
+template <typename T>
+__device__ __forceinline__ T plane_load_stream(const T *plane, uint32_t bytes, uint32_t boff) {
+  const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, bytes);
+  if constexpr (sizeof(T) == 2)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, kStreamPolicy));
+  else if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, kStreamPolicy));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, kStreamPolicy));
+}
+
+template <typename T>
+__device__ __forceinline__ void plane_store(T *plane, uint32_t bytes, uint32_t boff, T v) {
+  const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, bytes);
+  if constexpr (sizeof(T) == 2)
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, boff, 0, 0);
+  else if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, boff, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(unsigned int __attribute__((ext_vector_type(2))), v), r, boff, 0, 0);
+}
+
 // Stored element -> value in the compute type (deviation for FP32/FP16).
 template <int S, int Q>
 __device__ __forceinline__ typename Store<S>::R decode(typename Store<S>::T v, int i) {
@@ -729,18 +764,27 @@ step_kernel(StepArgs a) {
   const int y = tpy * a.t.by + threadIdx.y;
   const int z = tpz * a.t.bz + threadIdx.z;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  for (int p = 0; p < a.sp.np; p++)
-    if (in_box(a.sp.parts_in[p], x, y, z)) return;  // box_kernel's node
+  // nodes of a link box belong to the box blocks
+  for (int p = 0; p < a.sp.np; p++) {
+    const DevParticle &P = a.sp.parts_in[p];
+    const bool tile_in = tpx * a.t.bx <= P.hi[0] && (tpx + 1) * a.t.bx > P.lo[0] &&
+                         tpy * a.t.by <= P.hi[1] && (tpy + 1) * a.t.by > P.lo[1] &&
+                         tpz * a.t.bz <= P.hi[2] && (tpz + 1) * a.t.bz > P.lo[2];
+    if (tile_in && in_box(P, x, y, z)) return;
+  }
   const int64_t N = g.N;
-  const int n = x + g.nx * (y + g.ny * z);
-  const T *__restrict__ fin = static_cast<const T *>(a.fin);
-  T *__restrict__ fout = static_cast<T *>(a.fout);
+  constexpr uint32_t es = sizeof(T);
+  const uint32_t pb = (uint32_t)(N * es);  // bytes per plane (< 2^31, checked at creation)
+  const uint32_t n = x + g.nx * (y + g.ny * z);
+  const T *fin = static_cast<const T *>(a.fin);
+  T *fout = static_cast<T *>(a.fout);
 
   if (is_wall_node(g, x, y, z)) {
     // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
 #pragma unroll
     for (int q = 0; q < Q; q++)
-      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
+      plane_store<T>(fout + q * N, pb, n * es,
+                     S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q));
     return;
   }
 
@@ -748,7 +792,7 @@ step_kernel(StepArgs a) {
   {
     R f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(plane_load_stream<T>(fin + q * N, pb, n * es), q);
     R Fn[3] = {0, 0, 0};
     if constexpr (FM == LBM_FORCE_UNIFORM) {
 #pragma unroll
@@ -765,14 +809,22 @@ step_kernel(StepArgs a) {
       collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
   }
 
-  // Periodic neighbour offsets.
-  const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
-  const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
-  const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
-  const int ox[3] = {xm, x, xp};
-  const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
-  const int nxy = g.nx * g.ny;
-  const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
+  // Periodic neighbour offsets, in bytes.
+  const uint32_t xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
+  const uint32_t ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+  const uint32_t zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+  const uint32_t rx = es, ry = es * g.nx, rz = es * g.nx * g.ny;
+  const uint32_t ox[3] = {xm * rx, x * rx, xp * rx};
+  const uint32_t oy[3] = {ym * ry, y * ry, yp * ry};
+  const uint32_t oz[3] = {zm * rz, z * rz, zp * rz};
+
+  if (!g.walled) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      plane_store<T>(fout + q * N, pb, ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1],
+                     encode<S, Q>(fs[q], q));
+    return;
+  }
 
   // Flags of fluid nodes next to a wall layer.
   const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
@@ -783,16 +835,14 @@ step_kernel(StepArgs a) {
   if (!near) {
     // away from walls every population streams: no per-direction branches
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1];
-      fout[q * N + t] = encode<S, Q>(fs[q], q);
-    }
+    for (int q = 0; q < Q; q++)
+      plane_store<T>(fout + q * N, pb, ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1],
+                     encode<S, Q>(fs[q], q));
     return;
   }
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-    const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
     // x + c_q inside a wall layer: the wall keeps zero, no push.
     const bool push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
                         (cz < 0 && zl) || (cz > 0 && zh));
@@ -800,8 +850,8 @@ step_kernel(StepArgs a) {
     // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
     const bool own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
                      (cz > 0 && zl) || (cz < 0 && zh);
-    if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
-    if (own) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
+    if (push) plane_store<T>(fout + q * N, pb, ox[cx + 1] + oy[cy + 1] + oz[cz + 1], encode<S, Q>(fs[q], q));
+    if (own) plane_store<T>(fout + q * N, pb, n * es, encode<S, Q>(fs[L::opp[q]], q));
   }
 }
 
