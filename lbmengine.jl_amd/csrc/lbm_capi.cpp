@@ -50,7 +50,11 @@ struct lbm_handle {
   std::vector<double> coords_h[3];
   double *coords_d[3] = {nullptr, nullptr, nullptr};
   void *f[2] = {nullptr, nullptr};
-  int cur = 0;
+  int cur = 0;   // buffer holding the populations (the reference's distributions)
+  int pcur = 0;  // particle records of the current tick
+  void *rho_side = nullptr;  // rho of the last collision at link-box nodes
+  void *linkval = nullptr;   // link-slot values of the box blocks
+  size_t linkval_cap = 0;    // bytes
   void *F = nullptr;  // force field (double or float)
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
@@ -71,7 +75,8 @@ struct lbm_handle {
   int64_t timed_steps = 0;
   double step_kernel_ms = 0.0, step_ms = 0.0;
   // captured blocks of graph_k ticks, one per starting ping-pong buffer
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  // graphs of graph_k fused kernels, per starting population and particle parity
+  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int64_t graph_k = 32;
 };
 
@@ -110,6 +115,7 @@ Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
   t.ty = (h->n[1] + by - 1) / by;
   t.tz = (h->n[2] + bz - 1) / bz;
   t.ntiles = t.tx * t.ty * t.tz;
+  t.remap = 1;
   return t;
 }
 
@@ -138,6 +144,17 @@ int plan_boxes(lbm_handle *h, int64_t threads) {
     blocks += (int32_t)((vol + threads - 1) / threads);
   }
   for (size_t p = h->parts_h.size(); p <= LBM_MAX_PARTICLES; p++) h->first_block[p] = blocks;
+  const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+  if (!h->parts_h.empty() && !h->rho_side) HIP_TRY(hipMalloc(&h->rho_side, (size_t)h->N * rsize), "hipMalloc rho");
+  const size_t lv = (size_t)std::max(blocks, 1) * threads * (h->Q - 1) * rsize;
+  if (lv > h->linkval_cap) {
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    if (h->linkval) hipFree(h->linkval);
+    h->linkval = nullptr;
+    h->linkval_cap = 0;
+    HIP_TRY(hipMalloc(&h->linkval, lv), "hipMalloc link values");
+    h->linkval_cap = lv;
+  }
   const size_t need = (size_t)std::max(blocks, 1) * LBM_MAX_PARTICLES * 6 * 2;
   if (need > h->partial_cap) {
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
@@ -152,19 +169,31 @@ int plan_boxes(lbm_handle *h, int64_t threads) {
 
 int64_t tile_threads(const Tiling &t) { return (int64_t)t.bx * t.by * t.bz; }
 
-StepArgs step_args(lbm_handle *h, int in, void *fout) {
+// Doubles per axis of a box block's coordinate stage: the largest link-box
+// extent plus one node on each side.
+int32_t coord_stride(const lbm_handle *h) {
+  int32_t most = 1;
+  for (const DevParticle &p : h->parts_h)
+    for (int k = 0; k < h->dims; k++)
+      most = std::max(most, std::min(link_box_extent(p.radius, h->cfg.dx), h->n[k]) + 2);
+  return most;
+}
+
+StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   StepArgs a;
-  a.fin = h->f[in];
+  a.fin = fin;
   a.fout = fout;
   a.Ffield = h->F;
+  a.rho_side = h->rho_side;
   a.g = geom(h);
   a.c = coll(h);
   a.t = h->tiling;
   StepParticles &sp = a.sp;
-  sp.parts_in = parts_of(h, in);
-  sp.parts_out = parts_of(h, in ^ 1);
+  sp.parts_in = parts_of(h, pin);
+  sp.parts_out = parts_of(h, pin ^ 1);
   for (int k = 0; k < 3; k++) sp.coords[k] = h->coords_d[k];
   sp.partial = h->partial;
+  sp.linkval = h->linkval;
   sp.ticket = h->ticket;
   sp.np = (int32_t)h->parts_h.size();
   sp.semantics = h->cfg.semantics;
@@ -172,6 +201,7 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   for (int p = 0; p <= LBM_MAX_PARTICLES; p++) sp.first_block[p] = h->first_block[p];
   sp.nbox = h->first_block[LBM_MAX_PARTICLES];
   sp.nbox_pad = (sp.nbox + 7) / 8 * 8;
+  sp.coord_stride = coord_stride(h);
   double coeff = 1.0;
   for (int k = 0; k < h->dims; k++) coeff *= h->cfg.dx;
   if (h->cfg.semantics == LBM_SEMANTICS_GPU) coeff = coeff / h->cfg.dt;
@@ -188,51 +218,59 @@ StepArgs step_args(lbm_handle *h, int in, void *fout) {
   return a;
 }
 
-// Launch shape of the step kernel for arguments a: box blocks first, then
-// the tiles; the box blocks' link lists take dynamic LDS.
-Launch step_launch(lbm_handle *h, const StepArgs &a) {
-  Launch l{dim3((unsigned)(a.sp.nbox_pad + h->tiling.ntiles)), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz),
-           h->stream};
-  if (a.sp.nbox) {
+// Launch shape of a step kernel of the given mode: in the gathering modes the
+// box blocks come first, then the tiles; the box blocks' link lists take
+// dynamic LDS.
+Launch step_launch(lbm_handle *h, const StepArgs &a, int mode) {
+  const int nbox = mode == kCollide ? 0 : a.sp.nbox_pad;
+  Launch l{dim3((unsigned)(nbox + h->tiling.ntiles)), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
+  if (nbox) {
     const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
     const size_t t = (size_t)tile_threads(h->tiling);
-    l.shmem = 5 * t * rsize + 2 * t * 4;  // 2t slots (f*, value, key) + t rho
+    // coordinate stage, then 2t link slots (f*, value, key) and t rho
+    l.shmem = 3 * (size_t)a.sp.coord_stride * 8 + 5 * t * rsize + 2 * t * 4;
   }
   return l;
 }
 
-// One tick: one launch of the step kernel, whose leading blocks also move the
-// particles into their records of the next parity.
-hipError_t step_once(lbm_handle *h, int in, hipEvent_t e0, hipEvent_t e1) {
-  StepArgs a = step_args(h, in, h->f[in ^ 1]);
-  Launch l = step_launch(h, a);
-  hipError_t e = hipSuccess;
-  if (e0) hipEventRecord(e0, h->stream);
+hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int pin) {
+  StepArgs a = step_args(h, fin, fout, pin);
+  Launch l = step_launch(h, a, mode);
   switch (h->cfg.storage) {
-    case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-    case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-    default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
+    case LBM_STORAGE_F64: return launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
+    case LBM_STORAGE_F32: return launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
+    default: return launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
   }
+}
+
+// One fused kernel (gather tick k, collide tick k + 1) from buffer `in`
+// with particle records `pin`; the timing events bracket it.
+hipError_t fused_once(lbm_handle *h, int in, int pin, hipEvent_t e0, hipEvent_t e1) {
+  if (e0) hipEventRecord(e0, h->stream);
+  hipError_t e = launch_mode(h, kFused, h->f[in], h->f[in ^ 1], pin);
   if (e != hipSuccess) return e;
   if (e1) hipEventRecord(e1, h->stream);
   return e;
 }
 
 void drop_graph(lbm_handle *h) {
-  for (int b = 0; b < 2; b++) {
-    if (h->graph_exec[b]) hipGraphExecDestroy(h->graph_exec[b]);
-    h->graph_exec[b] = nullptr;
-  }
+  for (int b = 0; b < 2; b++)
+    for (int c = 0; c < 2; c++) {
+      if (h->graph_exec[b][c]) hipGraphExecDestroy(h->graph_exec[b][c]);
+      h->graph_exec[b][c] = nullptr;
+    }
 }
 
-// Capture k ticks starting from buffer `start` into an executable graph.  The
-// kernels take every pointer by value and read the particle state from
-// device memory, so the graph replays correctly as long as the handle's
-// buffers, particles and block shape stay as they are (drop_graph otherwise).
-int capture_graph(lbm_handle *h, int start, int64_t k, hipGraphExec_t *out) {
+// Capture k fused kernels starting from buffer `start` and particle records
+// `pstart` into an executable graph.  The kernels take every pointer by value
+// and read the particle state from device memory, so the graph replays
+// correctly as long as the handle's buffers, particles and block shape stay
+// as they are (drop_graph otherwise).
+int capture_graph(lbm_handle *h, int start, int pstart, int64_t k, hipGraphExec_t *out) {
   HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   hipError_t e = hipSuccess;
-  for (int64_t i = 0; i < k && e == hipSuccess; i++) e = step_once(h, start ^ (int)(i & 1), nullptr, nullptr);
+  for (int64_t i = 0; i < k && e == hipSuccess; i++)
+    e = fused_once(h, start ^ (int)(i & 1), pstart ^ (int)(i & 1), nullptr, nullptr);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(h->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -272,7 +310,7 @@ int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
 int check_particles(lbm_handle *h) {
   if (h->parts_h.empty()) return LBM_OK;
   std::vector<DevParticle> tmp(h->parts_h.size());
-  HIP_TRY(hipMemcpy(tmp.data(), parts_of(h, h->cur), tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
+  HIP_TRY(hipMemcpy(tmp.data(), parts_of(h, h->pcur), tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
           "copy particles");
   for (size_t p = 0; p < tmp.size(); p++)
     if (tmp[p].err)
@@ -310,6 +348,8 @@ int lbm_destroy(lbm_handle *h) {
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
   if (h->ticket) hipFree(h->ticket);
+  if (h->rho_side) hipFree(h->rho_side);
+  if (h->linkval) hipFree(h->linkval);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -536,7 +576,7 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
     return fail(LBM_ERANGE, "particle must lie at least two nodes inside every face of the lattice");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   const int idx = (int)h->parts_h.size();
-  HIP_TRY(hipMemcpy(parts_of(h, h->cur) + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
+  HIP_TRY(hipMemcpy(parts_of(h, h->pcur) + idx, &d, sizeof d, hipMemcpyHostToDevice), "copy particle");
   h->parts_h.push_back(d);
   int rc = plan_boxes(h, tile_threads(h->tiling));
   if (rc) {
@@ -553,7 +593,7 @@ int lbm_get_particle(lbm_handle *h, int32_t id, lbm_particle *out) {
   if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->pcur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   std::memset(out, 0, sizeof *out);
   out->kind = d.kind;
   out->couple_forces = d.couple_forces;
@@ -577,7 +617,7 @@ int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3], double tor
   if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->pcur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   for (int k = 0; k < 3; k++) {
     force[k] = d.force[k];
     torque[k] = d.torque[k];
@@ -591,10 +631,10 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_
   if (cap > 0 && (!node || !dir)) return fail(LBM_EINVAL, "null output buffer");
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   DevParticle d;
-  HIP_TRY(hipMemcpy(&d, parts_of(h, h->cur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->pcur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
   LinkMaskArgs a;
   a.g = geom(h);
-  a.part = parts_of(h, h->cur) + id;
+  a.part = parts_of(h, h->pcur) + id;
   for (int k = 0; k < 3; k++) {
     a.coords[k] = h->coords_d[k];
     a.lo[k] = d.lo[k];
@@ -631,58 +671,69 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_
   return LBM_OK;
 }
 
+// A run of n ticks is one collide-only kernel, n - 1 fused kernels (replayed
+// from captured graphs of graph_k where possible) and one gather-only kernel,
+// so the populations between runs are the reference's distributions.
 int lbm_run(lbm_handle *h, int64_t nsteps) {
   if (!h) return fail(LBM_EINVAL, "null handle");
   if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
   if (nsteps == 0) return LBM_OK;
-  if (h->timing) {
-    if ((int64_t)h->ev.size() < 2 * nsteps) {
+  const int64_t nfused = nsteps - 1;
+  if (h->timing && nfused > 0) {
+    if ((int64_t)h->ev.size() < 2 * nfused) {
       const size_t old = h->ev.size();
-      h->ev.resize(2 * nsteps);
+      h->ev.resize(2 * nfused);
       for (size_t i = old; i < h->ev.size(); i++)
         HIP_TRY(hipEventCreate(&h->ev[i]), "hipEventCreate");
     }
-    h->timed_steps = nsteps;
   }
   HIP_TRY(hipEventRecord(h->run_start, h->stream), "event record");
+  int a = h->cur;
+  hipError_t e = launch_mode(h, kCollide, h->f[a], h->f[a ^ 1], h->pcur);
+  if (e != hipSuccess) return hip_fail(e, "step launch");
+  a ^= 1;
   int64_t s = 0;
   if (!h->timing && h->graph_k > 0) {
-    // replay captured blocks of graph_k ticks (graph_k is even, so a block
-    // leaves the ping-pong parity where it found it)
-    while (nsteps - s >= h->graph_k) {
-      hipGraphExec_t &g = h->graph_exec[h->cur];
+    // replay captured blocks of graph_k fused kernels (graph_k is even, so a
+    // block leaves both parities where it found them)
+    while (nfused - s >= h->graph_k) {
+      hipGraphExec_t &g = h->graph_exec[a][h->pcur];
       if (!g) {
-        int rc = capture_graph(h, h->cur, h->graph_k, &g);
+        int rc = capture_graph(h, a, h->pcur, h->graph_k, &g);
         if (rc) return rc;
       }
       HIP_TRY(hipGraphLaunch(g, h->stream), "graph launch");
-      for (int64_t k = 0; k < h->graph_k; k++) h->time += h->cfg.dt;
-      h->tick += h->graph_k;
       s += h->graph_k;
     }
   }
-  for (int64_t i = 0; s < nsteps; s++, i++) {
+  for (int64_t i = 0; s < nfused; s++, i++) {
     hipEvent_t e0 = h->timing ? h->ev[2 * i] : nullptr;
     hipEvent_t e1 = h->timing ? h->ev[2 * i + 1] : nullptr;
-    hipError_t e = step_once(h, h->cur, e0, e1);
+    e = fused_once(h, a, h->pcur, e0, e1);
     if (e != hipSuccess) return hip_fail(e, "step launch");
-    h->cur ^= 1;
-    h->tick += 1;
-    h->time += h->cfg.dt;
+    a ^= 1;
+    h->pcur ^= 1;
   }
+  e = launch_mode(h, kGather, h->f[a], h->f[a ^ 1], h->pcur);
+  if (e != hipSuccess) return hip_fail(e, "step launch");
+  a ^= 1;
+  h->pcur ^= 1;
+  h->cur = a;
+  h->tick += nsteps;
+  for (int64_t k = 0; k < nsteps; k++) h->time += h->cfg.dt;
   HIP_TRY(hipEventRecord(h->run_stop, h->stream), "event record");
   if (h->timing) {
     HIP_TRY(hipEventSynchronize(h->run_stop), "sync");
     double sum = 0.0;
-    for (int64_t s = 0; s < nsteps; s++) {
+    for (int64_t k = 0; k < nfused; k++) {
       float ms = 0.0f;
-      HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * s], h->ev[2 * s + 1]), "event time");
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]), "event time");
       sum += ms;
     }
     float tot = 0.0f;
     HIP_TRY(hipEventElapsedTime(&tot, h->run_start, h->run_stop), "event time");
-    h->step_kernel_ms = sum / (double)nsteps;
+    h->step_kernel_ms = nfused > 0 ? sum / (double)nfused : 0.0;
     h->step_ms = tot / (double)nsteps;
   }
   return LBM_OK;
@@ -754,9 +805,11 @@ int lbm_set_graph(lbm_handle *h, int32_t ticks_per_graph) {
   return LBM_OK;
 }
 
-// Autotuner: time the step kernel of this lattice over candidate block
-// shapes (reading the current populations, writing a scratch buffer, so the
-// simulation state is untouched) and keep the fastest.
+// Autotuner: time the fused step kernel of this lattice over candidate block
+// shapes and tile orders (reading the current populations, writing a scratch
+// buffer, so the simulation state is untouched; the trial kernels also
+// rewrite the particle records of the other parity and the box nodes' rho,
+// which every run rewrites before reading) and keep the fastest.
 int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) {
   if (!h) return fail(LBM_EINVAL, "null handle");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
@@ -767,7 +820,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   std::sort(bxs.begin(), bxs.end());
   bxs.erase(std::unique(bxs.begin(), bxs.end()), bxs.end());
   struct Cand {
-    int32_t bx, by, bz;
+    int32_t bx, by, bz, remap;
   };
   std::vector<Cand> cands;
   for (int32_t bx : bxs)
@@ -778,7 +831,8 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
         const Tiling tl = make_tiling(h, bx, by, bz);
         const double waste = (double)tl.ntiles * t / (double)h->N - 1.0;
         if (waste > 0.25) continue;
-        cands.push_back({bx, by, bz});
+        cands.push_back({bx, by, bz, 1});
+        cands.push_back({bx, by, bz, 0});
       }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits this lattice");
   const size_t bytes = (size_t)h->N * h->Q * elem_size(h->cfg.storage);
@@ -794,7 +848,8 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   for (const Cand &c : cands) {
     // the trial steps also move the particles, into the records of the other
     // parity, which the next tick rewrites
-    const Tiling t = make_tiling(h, c.bx, c.by, c.bz);
+    Tiling t = make_tiling(h, c.bx, c.by, c.bz);
+    t.remap = c.remap;
     const int rc = plan_boxes(h, tile_threads(t));
     if (rc) {
       h->tiling = saved;
@@ -805,23 +860,11 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
       return rc;
     }
     h->tiling = t;
-    StepArgs a = step_args(h, h->cur, scratch);
-    Launch l = step_launch(h, a);
-    for (int r = 0; r < 2 && e == hipSuccess; r++) {  // warm-up
-      switch (h->cfg.storage) {
-        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-      }
-    }
+    // fused kernels reading the current populations into the scratch buffer
+    for (int r = 0; r < 2 && e == hipSuccess; r++)  // warm-up
+      e = launch_mode(h, kFused, h->f[h->cur], scratch, h->pcur);
     if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps && e == hipSuccess; r++) {
-      switch (h->cfg.storage) {
-        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, a, l); break;
-      }
-    }
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_mode(h, kFused, h->f[h->cur], scratch, h->pcur);
     if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
@@ -841,6 +884,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     return hip_fail(e, "autotune");
   }
   h->tiling = make_tiling(h, best_c.bx, best_c.by, best_c.bz);
+  h->tiling.remap = best_c.remap;
   if (const int rc = plan_boxes(h, tile_threads(h->tiling))) return rc;
   drop_graph(h);
   if (best) {

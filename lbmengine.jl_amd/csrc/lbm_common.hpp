@@ -88,6 +88,7 @@ struct Tiling {
   int32_t bx, by, bz;   // block shape
   int32_t tx, ty, tz;   // tiles per dimension
   int32_t ntiles;
+  int32_t remap;        // 1: xcd_remap the tile order
 };
 
 __host__ __device__ inline int32_t xcd_remap(int32_t b, int32_t nb) {

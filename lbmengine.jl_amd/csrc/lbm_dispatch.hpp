@@ -14,7 +14,7 @@ struct Launch {
 };
 
 template <int S>
-hipError_t launch_step(int Q, int coll, int fm, const StepArgs &a, const Launch &l);
+hipError_t launch_step(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>

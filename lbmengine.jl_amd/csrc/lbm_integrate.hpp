@@ -31,16 +31,33 @@ __device__ __forceinline__ DD dd_merge(DD a, DD b) {
   return r;
 }
 
+// One DPP move of a double (two 32-bit halves); lanes without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __builtin_bit_cast(double, (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32));
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ DD dd_dpp_merge(DD v) {
+  return dd_merge(v, DD{dpp_f64<CTRL, ROW_MASK>(v.hi), dpp_f64<CTRL, ROW_MASK>(v.lo)});
+}
+
+// Sum over the 64 lanes of a full wave, in a fixed order, valid in lane 63:
+// row_shr 1, 2, 4, 8 within each row of 16, then row_bcast 15 and 31 across
+// rows (all DPP, no LDS round trips).
 __device__ __forceinline__ DD wave_sum(DD v) {
-  const unsigned lane = __lane_id();
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    DD o{__shfl_xor(v.hi, off), __shfl_xor(v.lo, off)};
-    // both lanes of a pair must form the same sum: order the operands
-    v = (lane & off) ? dd_merge(o, v) : dd_merge(v, o);
-  }
+  v = dd_dpp_merge<0x111, 0xf>(v);  // row_shr:1
+  v = dd_dpp_merge<0x112, 0xf>(v);  // row_shr:2
+  v = dd_dpp_merge<0x114, 0xf>(v);  // row_shr:4
+  v = dd_dpp_merge<0x118, 0xf>(v);  // row_shr:8
+  v = dd_dpp_merge<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dd_dpp_merge<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
   return v;
 }
+constexpr int kWaveSumLane = 63;
 
 // What the kernels of a tick need for the particles.  Records are
 // double-buffered by tick parity: the kernels of tick t read parts_in while
@@ -50,6 +67,7 @@ struct StepParticles {
   DevParticle *parts_out;
   const double *coords[3];
   double *partial;       // [box block][particle][6][hi, lo]
+  void *linkval;         // [box block][T (Q - 1)]: link-slot values (compute type)
   uint32_t *ticket;      // box blocks finished in the running tick
   int32_t np, semantics, dims;
   // box blocks: particle p's are [first_block[p], first_block[p + 1]);
@@ -57,6 +75,7 @@ struct StepParticles {
   // multiple of the 8 XCDs)
   int32_t first_block[LBM_MAX_PARTICLES + 1];
   int32_t nbox, nbox_pad;
+  int32_t coord_stride;  // doubles per axis of a box block's coordinate stage
   double coeff;          // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
   double ls;             // lattice speed dx / dt
   double cutoff;         // 2 dx (cpu-semantics node-velocity range)
@@ -182,10 +201,10 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
   find_nv = any_nonzero(dL, nw) || moved;
 }
 
-// Reduce particle p's box-block partials and move it (moveParticles!), run
-// by the whole box block that finished last.  Both reduction orders are
-// fixed, so the result is reproducible.
-__device__ inline void integrate_particle(const StepParticles &sp, int p) {
+// Reduce particle p's box-block partials and move it (moveParticles!) from
+// its record cur of this tick, run by the whole box block that finished
+// last.  Both reduction orders are fixed, so the result is reproducible.
+__device__ inline void integrate_particle(const StepParticles &sp, int p, const DevParticle &cur) {
   __shared__ double tot[6];
   const int nblocks = sp.nbox;
   __syncthreads();  // the previous particle's copy-out may still read rec
@@ -201,7 +220,7 @@ __device__ inline void integrate_particle(const StepParticles &sp, int p) {
                          __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)});
     }
     s = wave_sum(s);
-    if (lane == 0) {
+    if (lane == kWaveSumLane) {
 #pragma clang fp contract(off)
       // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
       // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
@@ -216,7 +235,7 @@ __device__ inline void integrate_particle(const StepParticles &sp, int p) {
   constexpr int kWords = sizeof(DevParticle) / 8;
   __shared__ uint64_t rec[kWords];
   {
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(sp.parts_in + p);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&cur);
     for (int w = tid; w < kWords; w += nthreads) rec[w] = src[w];
   }
   __syncthreads();

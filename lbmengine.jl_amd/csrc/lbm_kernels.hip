@@ -11,30 +11,36 @@ namespace lbm {
 
 constexpr int S = LBM_STORE;
 
-#define LBM_DISPATCH_QCF(KERNEL, ARGS)                                                   \
-  do {                                                                                   \
-    const int key = (Q == 19 ? 100 : 0) + coll * 10 + fm;                                \
-    switch (key) {                                                                       \
-      case 0: hipLaunchKernelGGL((KERNEL<9, S, 0, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
-      case 1: hipLaunchKernelGGL((KERNEL<9, S, 0, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
-      case 2: hipLaunchKernelGGL((KERNEL<9, S, 0, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break;    \
-      case 10: hipLaunchKernelGGL((KERNEL<9, S, 1, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
-      case 11: hipLaunchKernelGGL((KERNEL<9, S, 1, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
-      case 12: hipLaunchKernelGGL((KERNEL<9, S, 1, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
-      case 100: hipLaunchKernelGGL((KERNEL<19, S, 0, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 101: hipLaunchKernelGGL((KERNEL<19, S, 0, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 102: hipLaunchKernelGGL((KERNEL<19, S, 0, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 110: hipLaunchKernelGGL((KERNEL<19, S, 1, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 111: hipLaunchKernelGGL((KERNEL<19, S, 1, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 112: hipLaunchKernelGGL((KERNEL<19, S, 1, 2>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      default: return hipErrorInvalidValue;                                              \
-    }                                                                                    \
-  } while (0)
+// (internal linkage: each storage's object file has its own)
+template <int Q, int COLL, int FM>
+static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) {
+  switch (mode) {
+    case kCollide: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kCollide>), l.grid, l.block, l.shmem, l.stream, a); break;
+    case kFused: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kFused>), l.grid, l.block, l.shmem, l.stream, a); break;
+    case kGather: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kGather>), l.grid, l.block, l.shmem, l.stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 template <>
-hipError_t launch_step<S>(int Q, int coll, int fm, const StepArgs &a, const Launch &l) {
-  LBM_DISPATCH_QCF(step_kernel, a);
-  return hipGetLastError();
+hipError_t launch_step<S>(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l) {
+  const int key = (Q == 19 ? 100 : 0) + coll * 10 + fm;
+  switch (key) {
+    case 0: return launch_step_qcf<9, 0, 0>(mode, a, l);
+    case 1: return launch_step_qcf<9, 0, 1>(mode, a, l);
+    case 2: return launch_step_qcf<9, 0, 2>(mode, a, l);
+    case 10: return launch_step_qcf<9, 1, 0>(mode, a, l);
+    case 11: return launch_step_qcf<9, 1, 1>(mode, a, l);
+    case 12: return launch_step_qcf<9, 1, 2>(mode, a, l);
+    case 100: return launch_step_qcf<19, 0, 0>(mode, a, l);
+    case 101: return launch_step_qcf<19, 0, 1>(mode, a, l);
+    case 102: return launch_step_qcf<19, 0, 2>(mode, a, l);
+    case 110: return launch_step_qcf<19, 1, 0>(mode, a, l);
+    case 111: return launch_step_qcf<19, 1, 1>(mode, a, l);
+    case 112: return launch_step_qcf<19, 1, 2>(mode, a, l);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 #define LBM_DISPATCH_Q(KERNEL, ARGS)                                                \

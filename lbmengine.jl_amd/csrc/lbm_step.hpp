@@ -128,6 +128,7 @@ struct StepArgs {
   const void *fin;
   void *fout;
   const void *Ffield;  // (N, d): double for FP64 storage, float otherwise
+  void *rho_side;      // (N): rho of the last collision at link-box nodes (compute type)
   BoxGeom g;
   CollParams c;
   Tiling t;
@@ -358,22 +359,20 @@ __device__ __forceinline__ bool solid_at(const double *const coords[3], const do
 }
 
 // ------------------------------------------------------------ particles
-// Node velocity of particle p at the solid node (x, y, z):
+// Node velocity of particle p at the solid node with coordinates xc:
 // getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
 // (cpu/particles.jl:20-43), from the node-velocity snapshot.
 template <int D>
-__device__ __forceinline__ void node_velocity(const double *const coords[3], const DevParticle &p,
-                                              int semantics, double cutoff, int x, int y, int z,
-                                              double (&out)[3]) {
+__device__ __forceinline__ void node_velocity(const double (&xc)[3], const DevParticle &p, int semantics,
+                                              double cutoff, double (&out)[3]) {
 #pragma clang fp contract(off)
-  const int idx[3] = {x, y, z};
   double r[3] = {0.0, 0.0, 0.0};
   if (semantics == LBM_SEMANTICS_GPU) {
 #pragma unroll
-    for (int k = 0; k < D; k++) r[k] = coords[k][idx[k]] + (-p.nR[k]);
+    for (int k = 0; k < D; k++) r[k] = xc[k] + (-p.nR[k]);
   } else {
 #pragma unroll
-    for (int k = 0; k < D; k++) r[k] = coords[k][idx[k]] - p.nR[k];
+    for (int k = 0; k < D; k++) r[k] = xc[k] - p.nR[k];
   }
   double nn = 0.0;
 #pragma unroll
@@ -439,23 +438,84 @@ __device__ __forceinline__ bool in_box(const DevParticle &P, int x, int y, int z
   return x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] && z >= P.lo[2] && z <= P.hi[2];
 }
 
+// Index of population q of node (x, row) in the row-blocked layout, row =
+// y + ny z: the Q planes of one x-row are contiguous.
+__device__ __forceinline__ int64_t pop_index(const BoxGeom &g, int Q, int q, int x, int row) {
+  return ((int64_t)row * Q + q) * g.nx + x;
+}
+
+// Row offsets (in elements) of the rows a node gathers from, for every
+// (cy, cz) in {-1, 0, 1}^2: source row of (y - cy, z - cz).
+struct GatherRows {
+  int64_t r[3][3];  // [cy + 1][cz + 1]
+};
+
+__device__ __forceinline__ GatherRows gather_rows(const BoxGeom &g, int Q, int y, int z) {
+  GatherRows o;
+  const int ys[3] = {y == g.ny - 1 ? 0 : y + 1, y, y == 0 ? g.ny - 1 : y - 1};  // y - cy, cy = -1, 0, 1
+  const int zs[3] = {z == g.nz - 1 ? 0 : z + 1, z, z == 0 ? g.nz - 1 : z - 1};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) o.r[a][b] = (int64_t)(ys[a] + g.ny * zs[b]) * Q * g.nx;
+  return o;
+}
+
+// Kernel modes of one run of ticks (S: the reference's distributions, P:
+// post-collision populations): kCollide P_0 = C(S_0); kFused S_k = G(P_{k-1})
+// then P_k = C(S_k); kGather S_n = G(P_{n-1}).  G is streaming with halfway
+// bounce-back, the Ladd term and the momentum exchange (tick n-1's, which
+// also moves the particles), C the collision.
+enum StepMode { kCollide = 0, kFused = 1, kGather = 2 };
+
+// Collision of a node's populations f (S), into fs (P), with rho.
+template <int Q, int S, int COLL, int FM>
+__device__ __forceinline__ void collide_node(const StepArgs &a, int64_t n,
+                                             const typename Store<S>::R (&f)[Q],
+                                             typename Store<S>::R (&fs)[Q], typename Store<S>::R &rho) {
+  using L = Lattice<Q>;
+  using R = typename Store<S>::R;
+  constexpr int D = L::D;
+  R Fn[3] = {0, 0, 0};
+  if constexpr (FM == LBM_FORCE_UNIFORM) {
+#pragma unroll
+    for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
+  } else if constexpr (FM == LBM_FORCE_FIELD) {
+    const R *Ff = static_cast<const R *>(a.Ffield);
+#pragma unroll
+    for (int k = 0; k < D; k++) Fn[k] = Ff[k * a.g.N + n];
+  }
+  if constexpr (S == kF64)
+    collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+  else
+    collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+}
+
+// The stored value of a population that is zero (wall layers).
+template <int Q, int S>
+__device__ __forceinline__ typename Store<S>::T zero_pop(const CollParams &c, int q) {
+  using R = typename Store<S>::R;
+  if constexpr (S == kF64)
+    return 0.0;
+  else
+    return encode<S, Q>(static_cast<R>(-Lattice<Q>::w(q) * c.rho0), q);
+}
+
 // -------------------------------------------------------------- boxes
 // The nodes of every particle's link box are handled by the first blocks of
 // the step kernel's grid (dispatched first, spread over all XCDs), while the
-// other blocks stream the rest of the lattice.  Each box node is collided and
-// streamed like anywhere else, except that the slots of particle links are
-// left to the link pass: a solid node does not push into a fluid neighbour of
-// the same sphere, and a fluid node does not bounce back off a wall into a
-// slot that is a particle link.  Each link slot then gets f*_{q̄}(x) plus the
-// Ladd terms of the particles it belongs to, in particle order
-// (gpu/fluids.jl:157-207), and the momentum exchange of each particle is
+// other blocks do the rest of the lattice.  A box node gathers like any
+// other, except at particle links: a fluid node x whose neighbour x - c_q is
+// solid takes f*_{q̄}(x) (halfway bounce-back) plus the Ladd terms of the
+// particles the link belongs to, in particle order (gpu/fluids.jl:157-207);
+// each particle's momentum exchange is taken from the running value and
 // reduced over the block into one double-double partial per block.  The box
 // block that finishes last reduces all partials in a fixed order and moves
 // the particles (moveParticles!) into the records of the next tick.
 //
 // Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's link
 // box in x-fastest order (j = bid - first_block[pb], T threads per block).
-template <int Q, int S, int COLL, int FM>
+template <int Q, int S, int COLL, int FM, int MODE>
 __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
@@ -466,9 +526,20 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 
+  // the particle records of this tick, in LDS (read throughout)
+  __shared__ DevParticle prec[LBM_MAX_PARTICLES];
+  {
+    static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
+    constexpr int kWords = sizeof(DevParticle) / 8;
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(sp.parts_in);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(prec);
+    for (int w = tid; w < sp.np * kWords; w += nthreads) dst[w] = src[w];
+  }
+  __syncthreads();
+
   int pb = 0;
   while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
-  const DevParticle &B = sp.parts_in[pb];
+  const DevParticle &B = prec[pb];
   const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1, ez = B.hi[2] - B.lo[2] + 1;
   const int base = (bid - sp.first_block[pb]) * nthreads;
   const int li = base + tid;
@@ -476,97 +547,90 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // link boxes lie inside the lattice; a node in several boxes belongs to the first
   bool mine = li < ex * ey * ez;
   for (int p = 0; p < pb; p++)
-    if (in_box(sp.parts_in[p], x, y, z)) mine = false;
+    if (in_box(prec[p], x, y, z)) mine = false;
+
+  // Coordinates of pb's box and one node around it, in LDS: axis k from
+  // index lo_k - 1 (periodic) at cst[k][0].
+  extern __shared__ double lds_dyn[];
+  const int cstride = sp.coord_stride;
+  double *cst = lds_dyn;
+  {
+    const int len[3] = {ex + 2, ey + 2, D == 3 ? ez + 2 : 1};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    for (int k = 0; k < D; k++)
+      for (int i = tid; i < len[k]; i += nthreads) {
+        int gi = B.lo[k] - 1 + i;
+        gi = gi < 0 ? gi + nn[k] : (gi >= nn[k] ? gi - nn[k] : gi);
+        cst[k * cstride + i] = sp.coords[k][gi];
+      }
+  }
+  __syncthreads();
+  // coordinate k of the node at unwrapped index i (lo_k - 1 <= i <= hi_k + 1)
+  auto crd = [&](int k, int i) { return cst[k * cstride + (i - B.lo[k] + 1)]; };
   const bool wall = mine && is_wall_node(g, x, y, z);
-  const bool node = mine && !wall;  // streamed and collided (particle interiors too)
-  const int64_t N = g.N;
-  const int n = mine ? x + g.nx * (y + g.ny * z) : 0;
+  const bool node = mine && !wall;
+  const int row = y + g.ny * z;
   const T *__restrict__ fin = static_cast<const T *>(a.fin);
   T *__restrict__ fout = static_cast<T *>(a.fout);
+  R *rho_side = static_cast<R *>(a.rho_side);
 
   if (wall) {
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-      fout[q * N + n] = S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q);
+    for (int q = 0; q < Q; q++) fout[pop_index(g, Q, q, x, row)] = zero_pop<Q, S>(a.c, q);
   }
 
-  R fs[Q];
-  R rho = 0;
-  if (node) {
-    R f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[q * N + n], q);
-    R Fn[3] = {0, 0, 0};
-    if constexpr (FM == LBM_FORCE_UNIFORM) {
-#pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
-    } else if constexpr (FM == LBM_FORCE_FIELD) {
-      const R *Ff = static_cast<const R *>(a.Ffield);
-#pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
-    }
-    if constexpr (S == kF64)
-      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-    else
-      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-  } else {
-#pragma unroll
-    for (int q = 0; q < Q; q++) fs[q] = 0;
-  }
+  const int64_t own_idx = (int64_t)row * Q * g.nx + x;  // population 0 of this node
 
   // Geometry of this tick (bounceBackPrep with the getSphere test): only nodes
-  // within 1.5 dx of a sphere surface can be a link's solid or fluid end.
-  uint32_t skip_push = 0, own_any = 0;
+  // within 1.5 dx of a sphere surface can be a link's fluid end.
+  uint32_t own_any = 0;
   uint32_t own[LBM_MAX_PARTICLES];
 #pragma unroll
   for (int p = 0; p < LBM_MAX_PARTICLES; p++) own[p] = 0;
   if (node) {
+    // coordinates of the node and its neighbours, [1 - c] for offset -c
+    double cxv[3], cyv[3], czv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      cxv[i] = crd(0, x - 1 + i);
+      cyv[i] = crd(1, y - 1 + i);
+      if (D == 3) czv[i] = crd(2, z - 1 + i);
+    }
     for (int p = 0; p < sp.np; p++) {
-      const DevParticle &P = sp.parts_in[p];
+      const DevParticle &P = prec[p];
       if (!in_box(P, x, y, z)) continue;
-      const double d2 = dist2_at<D>(sp.coords, P.Rg, x, y, z);
+      // (X + (-R))_1^2 + ... in the getSphere order (gpu/kernels.jl:344-353),
+      // from the per-axis squares
+      double sa[3], sb[3], sc[3];
+      {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const double u = cxv[i] + (-P.Rg[0]), v = cyv[i] + (-P.Rg[1]), w = czv[i] + (-P.Rg[2]);
+          sa[i] = u * u;
+          sb[i] = v * v;
+          sc[i] = w * w;
+        }
+      }
+      auto d2at = [&](int i, int j, int k) {
+#pragma clang fp contract(off)
+        const double s2 = sa[i] + sb[j];
+        return D == 3 ? s2 + sc[k] : s2;
+      };
+      const double d2 = d2at(1, 1, 1);
       const double r2 = P.radius * P.radius;
-      const double ro = P.radius + sp.shell, ri = P.radius - sp.shell;
-      if (d2 >= ro * ro || (ri > 0.0 && d2 < ri * ri)) continue;
-      const bool s0 = d2 < r2;
+      const double ro = P.radius + sp.shell;
+      if (d2 < r2 || d2 >= ro * ro) continue;  // solid, or too far from the surface
       uint32_t op = 0;
 #pragma unroll
       for (int q = 1; q < Q; q++) {
-        const int sx = wrapi(x + L::cx[q], g.nx), sy = wrapi(y + L::cy[q], g.ny),
-                  sz = D == 3 ? wrapi(z + L::cz[q], g.nz) : 0;
-        const bool sn = solid_at<D>(sp.coords, P.Rg, r2, sx, sy, sz);
-        if (s0 && !sn) skip_push |= 1u << q;  // x + c_q is a link node of P
-        if (!s0 && sn) op |= 1u << L::opp[q];  // (x, q̄) is a link slot of P
+        // (x, q) is a link slot of P when x - c_q is solid
+        if (d2at(1 - L::cx[q], 1 - L::cy[q], 1 - L::cz[q]) < r2) op |= 1u << q;
       }
 #pragma unroll
       for (int i = 0; i < LBM_MAX_PARTICLES; i++)
         if (i == p) own[i] = op;
       own_any |= op;
-    }
-
-    // streaming, wall bounce-back; particle link slots are left alone
-    const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
-    const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
-    const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
-    const int ox[3] = {xm, x, xp};
-    const int oy[3] = {ym * g.nx, y * g.nx, yp * g.nx};
-    const int nxy = g.nx * g.ny;
-    const int oz[3] = {zm * nxy, z * nxy, zp * nxy};
-    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-      const int t = ox[cx + 1] + oy[cy + 1] + oz[cz + 1];
-      const bool push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
-                          (cz < 0 && zl) || (cz > 0 && zh)) &&
-                        !((skip_push >> q) & 1);
-      const bool bb = ((cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
-                       (cz > 0 && zl) || (cz < 0 && zh)) &&
-                      !((own_any >> q) & 1);
-      if (push) fout[q * N + t] = encode<S, Q>(fs[q], q);
-      if (bb) fout[q * N + n] = encode<S, Q>(fs[L::opp[q]], q);
     }
   }
 
@@ -576,10 +640,11 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // and take their momentum exchange from the running value.  The list holds
   // 2 T slots at a time; more are taken in further rounds.
   const int cap = 2 * nthreads;
-  extern __shared__ double lds_dyn[];
-  R *fs_l = reinterpret_cast<R *>(lds_dyn);          // f*_{q̄}(x) of the slot
+  R *fs_l = reinterpret_cast<R *>(cst + 3 * cstride);  // f*_{q̄}(x) of the slot
   R *v_l = fs_l + cap;                               // streamed value of the slot so far
-  R *rho_l = v_l + cap;                              // rho of each lane's node
+  R *rho_l = v_l + cap;                              // rho(x) of each lane's node
+  // final values of the block's link slots, in list order
+  R *linkval = static_cast<R *>(sp.linkval) + (int64_t)bid * nthreads * (Q - 1);
   uint32_t *pk_l = reinterpret_cast<uint32_t *>(rho_l + nthreads);
   // pk: lane | q << 9 | (c + 1) << 14, 16, 18 | weight class << 20 | owners << 22
   __shared__ uint32_t wave_cnt[8];
@@ -597,7 +662,9 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     if (lane >= d) incl += o;
   }
   if (lane == 63 || tid == nthreads - 1) wave_cnt[wave] = incl;
-  rho_l[tid] = rho;
+  // rho of this node's populations of the previous tick (massDensity), kept
+  // by the collision that produced them
+  rho_l[tid] = own_any ? rho_side[(int64_t)x + g.nx * (int64_t)row] : R(0);
   if (tid < 6 * LBM_MAX_PARTICLES) blk[tid / 6][tid % 6] = DD{0.0, 0.0};
   __syncthreads();
   uint32_t off = incl - cnt, total = 0;
@@ -606,6 +673,11 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     total += wave_cnt[w];
   }
   const int lx0 = B.lo[0], ly0 = B.lo[1], lz0 = B.lo[2];
+  // f*_{q̄}(x) of this node's link slots, all loads in flight at once
+  R bbv[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+    bbv[q] = ((own_any >> q) & 1) ? decode<S, Q>(fin[own_idx + (int64_t)L::opp[q] * g.nx], L::opp[q]) : R(0);
   for (uint32_t c0 = 0; c0 < total; c0 += cap) {
     const int m = (int)min((uint32_t)cap, total - c0);
     uint32_t idx = off;
@@ -616,8 +688,8 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
           uint32_t owners = 0;
 #pragma unroll
           for (int i = 0; i < LBM_MAX_PARTICLES; i++) owners |= ((own[i] >> q) & 1u) << i;
-          fs_l[idx - c0] = fs[L::opp[q]];
-          v_l[idx - c0] = fs[L::opp[q]];
+          fs_l[idx - c0] = bbv[q];
+          v_l[idx - c0] = bbv[q];
           pk_l[idx - c0] = (uint32_t)tid | (uint32_t)q << 9 | (uint32_t)(L::cx[q] + 1) << 14 |
                            (uint32_t)(L::cy[q] + 1) << 16 | (uint32_t)(L::cz[q] + 1) << 18 |
                            (uint32_t)L::wclass[q] << 20 | owners << 22;
@@ -627,14 +699,23 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     }
     __syncthreads();
     for (int p = 0; p < sp.np; p++) {
-      const DevParticle &P = sp.parts_in[p];
+      const DevParticle &P = prec[p];
       const bool coupled = P.couple_forces || P.couple_torques;
-      DD acc[6];
+      // Each pass of the block over its slots is reduced over every wave at
+      // once (lanes without a slot of p add zero), so no accumulator has to
+      // live across the pass.
+      if (lane == kWaveSumLane) {
 #pragma unroll
-      for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
-      for (int e = tid; e < m; e += nthreads) {
-        const uint32_t pk = pk_l[e];
-        if (!((pk >> (22 + p)) & 1)) continue;
+        for (int k = 0; k < 6; k++) red[wave][k] = DD{0.0, 0.0};
+      }
+      const int passes = (m + nthreads - 1) / nthreads;
+      for (int it = 0; it < passes; it++) {
+        const int e = it * nthreads + tid;
+        DD acc[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
+        const uint32_t pk = e < m ? pk_l[e] : 0u;
+        if ((pk >> (22 + p)) & 1) {
         const int t = pk & 511, lt = base + t;
         const int lx = lx0 + lt % ex, ly = ly0 + (lt / ex) % ey, lz = lz0 + lt / (ex * ey);
         const int c0 = (int)((pk >> 14) & 3) - 1, c1 = (int)((pk >> 16) & 3) - 1,
@@ -643,9 +724,9 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         const double ci[3] = {c0 * a.c.ls, c1 * a.c.ls, c2 * a.c.ls};
         const double lc = 2.0 * wq * a.c.invC2s;
         const double shift = S == kF64 ? 0.0 : 2.0 * wq * a.c.rho0;
-        const int sx = wrapi(lx - c0, g.nx), sy = wrapi(ly - c1, g.ny), sz = D == 3 ? wrapi(lz - c2, g.nz) : 0;
+        const double xs[3] = {crd(0, lx - c0), crd(1, ly - c1), D == 3 ? crd(2, lz - c2) : 0.0};
         double ub[3] = {0.0, 0.0, 0.0};
-        node_velocity<D>(sp.coords, P, sp.semantics, sp.cutoff, sx, sy, sz, ub);
+        node_velocity<D>(xs, P, sp.semantics, sp.cutoff, ub);
         R v = v_l[e];
         {
 #pragma clang fp contract(off)
@@ -665,7 +746,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
           // -coeff * sum(s) * c_q and -coeff * cross(sum(s (x - R)), c_q),
           // formed from these exact sums by the integrator
           const double s = static_cast<double>(fs_l[e]) + static_cast<double>(v) + shift;
-          const double X[3] = {sp.coords[0][lx], sp.coords[1][ly], D == 3 ? sp.coords[2][lz] : 0.0};
+          const double X[3] = {crd(0, lx), crd(1, ly), D == 3 ? crd(2, lz) : 0.0};
           double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-P.R[k]));
@@ -688,13 +769,14 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
             }
           }
         }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          const DD v = wave_sum(acc[k]);
+          if (lane == kWaveSumLane) red[wave][k] = dd_merge(red[wave][k], v);
+        }
       }
       // this round's part of the block's partial of particle p
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        const DD v = wave_sum(acc[k]);
-        if (lane == 0) red[wave][k] = v;
-      }
       __syncthreads();
       if (tid < 6) {
         DD s = red[0][tid];
@@ -703,13 +785,10 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       }
       __syncthreads();
     }
-    // the link slots' streamed values
-    for (int e = tid; e < m; e += nthreads) {
-      const uint32_t pk = pk_l[e];
-      const int t = pk & 511, q = (pk >> 9) & 31, lt = base + t;
-      const int lx = lx0 + lt % ex, ly = ly0 + (lt / ex) % ey, lz = lz0 + lt / (ex * ey);
-      fout[q * N + (lx + g.nx * (ly + g.ny * lz))] = encode_class<S, Q>(v_l[e], (pk >> 20) & 3);
-    }
+    // the link slots' streamed values, kept for their nodes' collision (in
+    // v_l itself when one round takes them all)
+    if (total > (uint32_t)cap)
+      for (int e = tid; e < m; e += nthreads) linkval[c0 + e] = v_l[e];
     __syncthreads();  // the next round reuses the list
   }
   if (tid < 6 * sp.np) {
@@ -733,30 +812,69 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   }
   __syncthreads();
   if (last) {
-    for (int p = 0; p < sp.np; p++) integrate_particle(sp, p);
+    for (int p = 0; p < sp.np; p++) integrate_particle(sp, p, prec[p]);
     if (tid == 0) __hip_atomic_store(sp.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  // this node: S (gather only) or its collision P, and rho for the next tick's links
+  if (node) {
+    R fg[Q];
+    const GatherRows gr = gather_rows(g, Q, y, z);
+    const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
+    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+    uint32_t idx = off;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      // x - c_q in a wall layer: halfway bounce-back (gpu/fluids.jl:165)
+      const bool bb = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
+                      (cz > 0 && zl) || (cz < 0 && zh);
+      if ((own_any >> q) & 1) {
+        fg[q] = total > (uint32_t)cap ? linkval[idx] : v_l[idx];  // particle link
+        idx++;
+      } else {
+        const int64_t src = bb ? own_idx + (int64_t)L::opp[q] * g.nx
+                               : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
+        fg[q] = decode<S, Q>(fin[src], q);
+      }
+    }
+    if constexpr (MODE == kGather) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) fout[pop_index(g, Q, q, x, row)] = encode<S, Q>(fg[q], q);
+    } else {
+      R fs[Q], rho;
+      collide_node<Q, S, COLL, FM>(a, (int64_t)x + g.nx * (int64_t)row, fg, fs, rho);
+#pragma unroll
+      for (int q = 0; q < Q; q++) fout[pop_index(g, Q, q, x, row)] = encode<S, Q>(fs[q], q);
+      rho_side[(int64_t)x + g.nx * (int64_t)row] = rho;
+    }
   }
 }
 
 // -------------------------------------------------------------- step
-// One tick of the lattice: blocks [0, nbox_pad) work on the particles' link
-// boxes, the others each stream one tile outside them.  Registers are capped
-// at the plain path's needs (FP32/FP16: 64, eight waves per SIMD; FP64: 128,
-// four), so the wider box path spills rather than lowering the occupancy of
-// every tile.
-template <int Q, int S, int COLL, int FM>
+// One kernel of a run (see StepMode): blocks [0, nbox_pad) work on the
+// particles' link boxes (gathering modes only), the others each do one tile
+// outside them.  Registers are capped at the plain path's needs (FP32/FP16:
+// 64, eight waves per SIMD; FP64: 128, four), so the wider box path spills
+// rather than lowering the occupancy of every tile.
+template <int Q, int S, int COLL, int FM, int MODE>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
 step_kernel(StepArgs a) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
-  constexpr int D = L::D;
   const BoxGeom &g = a.g;
-  if ((int)blockIdx.x < a.sp.nbox_pad) {
-    if ((int)blockIdx.x < a.sp.nbox) box_work<Q, S, COLL, FM>(a, blockIdx.x);
-    return;
+  int b = blockIdx.x;
+  if constexpr (MODE != kCollide) {
+    if (b < a.sp.nbox_pad) {
+      if (b < a.sp.nbox) box_work<Q, S, COLL, FM, MODE>(a, b);
+      return;
+    }
+    b -= a.sp.nbox_pad;
   }
-  const int tile = xcd_remap(blockIdx.x - a.sp.nbox_pad, a.t.ntiles);
+  const int tile = a.t.remap ? xcd_remap(b, a.t.ntiles) : b;
   const int tpx = tile % a.t.tx;
   const int tpy = (tile / a.t.tx) % a.t.ty;
   const int tpz = tile / (a.t.tx * a.t.ty);
@@ -764,94 +882,67 @@ step_kernel(StepArgs a) {
   const int y = tpy * a.t.by + threadIdx.y;
   const int z = tpz * a.t.bz + threadIdx.z;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  // nodes of a link box belong to the box blocks
+  bool boxed = false;  // in a particle's link box
   for (int p = 0; p < a.sp.np; p++) {
     const DevParticle &P = a.sp.parts_in[p];
     const bool tile_in = tpx * a.t.bx <= P.hi[0] && (tpx + 1) * a.t.bx > P.lo[0] &&
                          tpy * a.t.by <= P.hi[1] && (tpy + 1) * a.t.by > P.lo[1] &&
                          tpz * a.t.bz <= P.hi[2] && (tpz + 1) * a.t.bz > P.lo[2];
-    if (tile_in && in_box(P, x, y, z)) return;
+    if (tile_in && in_box(P, x, y, z)) boxed = true;
   }
-  const int64_t N = g.N;
-  constexpr uint32_t es = sizeof(T);
-  const uint32_t pb = (uint32_t)(N * es);  // bytes per plane (< 2^31, checked at creation)
-  const uint32_t n = x + g.nx * (y + g.ny * z);
-  const T *fin = static_cast<const T *>(a.fin);
-  T *fout = static_cast<T *>(a.fout);
+  if (MODE != kCollide && boxed) return;  // the box blocks' node
+  const int row = y + g.ny * z;
+  const int64_t own = (int64_t)row * Q * g.nx + x;
+  const T *__restrict__ fin = static_cast<const T *>(a.fin);
+  T *__restrict__ fout = static_cast<T *>(a.fout);
 
   if (is_wall_node(g, x, y, z)) {
     // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-      plane_store<T>(fout + q * N, pb, n * es,
-                     S == kF64 ? T(0) : encode<S, Q>(static_cast<R>(-L::w(q) * a.c.rho0), q));
+    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = zero_pop<Q, S>(a.c, q);
     return;
   }
 
-  R fs[Q];
-  {
-    R f[Q];
+  R f[Q];
+  if constexpr (MODE == kCollide) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(plane_load_stream<T>(fin + q * N, pb, n * es), q);
-    R Fn[3] = {0, 0, 0};
-    if constexpr (FM == LBM_FORCE_UNIFORM) {
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[own + (int64_t)q * g.nx], q);
+  } else {
+    // gather f_q(x) = f*_q(x - c_q)
+    const GatherRows gr = gather_rows(g, Q, y, z);
+    const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
+    if (!g.walled) {
 #pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
-    } else if constexpr (FM == LBM_FORCE_FIELD) {
-      const R *Ff = static_cast<const R *>(a.Ffield);
+      for (int q = 0; q < Q; q++)
+        f[q] = decode<S, Q>(fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]], q);
+    } else {
+      const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+      const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+      const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
 #pragma unroll
-      for (int k = 0; k < D; k++) Fn[k] = Ff[k * N + n];
+      for (int q = 0; q < Q; q++) {
+        const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+        // x - c_q in a wall layer: halfway bounce-back f_q(x) = f*_{q̄}(x)
+        // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165)
+        const bool bb = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
+                        (cz > 0 && zl) || (cz < 0 && zh);
+        const int64_t src = bb ? own + (int64_t)L::opp[q] * g.nx
+                               : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
+        f[q] = decode<S, Q>(fin[src], q);
+      }
     }
-    R rho;
-    if constexpr (S == kF64)
-      collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
-    else
-      collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
   }
 
-  // Periodic neighbour offsets, in bytes.
-  const uint32_t xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
-  const uint32_t ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
-  const uint32_t zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
-  const uint32_t rx = es, ry = es * g.nx, rz = es * g.nx * g.ny;
-  const uint32_t ox[3] = {xm * rx, x * rx, xp * rx};
-  const uint32_t oy[3] = {ym * ry, y * ry, yp * ry};
-  const uint32_t oz[3] = {zm * rz, z * rz, zp * rz};
-
-  if (!g.walled) {
+  if constexpr (MODE == kGather) {
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-      plane_store<T>(fout + q * N, pb, ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1],
-                     encode<S, Q>(fs[q], q));
-    return;
-  }
-
-  // Flags of fluid nodes next to a wall layer.
-  const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-  const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-  const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
-  const bool near = xl | xh | yl | yh | zl | zh;
-
-  if (!near) {
-    // away from walls every population streams: no per-direction branches
+    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = encode<S, Q>(f[q], q);
+  } else {
+    R fs[Q], rho;
+    collide_node<Q, S, COLL, FM>(a, (int64_t)x + g.nx * (int64_t)row, f, fs, rho);
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-      plane_store<T>(fout + q * N, pb, ox[L::cx[q] + 1] + oy[L::cy[q] + 1] + oz[L::cz[q] + 1],
-                     encode<S, Q>(fs[q], q));
-    return;
-  }
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-    // x + c_q inside a wall layer: the wall keeps zero, no push.
-    const bool push = !((cx < 0 && xl) || (cx > 0 && xh) || (cy < 0 && yl) || (cy > 0 && yh) ||
-                        (cz < 0 && zl) || (cz > 0 && zh));
-    // x - c_q inside a wall layer: halfway bounce-back into own slot q
-    // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165).
-    const bool own = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
-                     (cz > 0 && zl) || (cz < 0 && zh);
-    if (push) plane_store<T>(fout + q * N, pb, ox[cx + 1] + oy[cy + 1] + oz[cz + 1], encode<S, Q>(fs[q], q));
-    if (own) plane_store<T>(fout + q * N, pb, n * es, encode<S, Q>(fs[L::opp[q]], q));
+    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = encode<S, Q>(fs[q], q);
+    // rho of the links' next gather (only the collide-only kernel reaches box nodes)
+    if (MODE == kCollide && boxed) static_cast<R *>(a.rho_side)[(int64_t)x + g.nx * (int64_t)row] = rho;
   }
 }
 
@@ -897,6 +988,7 @@ __global__ void init_uniform_kernel(AuxArgs a) {
     }
   }
   T *f = static_cast<T *>(a.f);
+  const int row = y + a.g.ny * z;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const double ci[3] = {L::cx[q] * a.c.ls, L::cy[q] * a.c.ls, L::cz[q] * a.c.ls};
@@ -907,7 +999,7 @@ __global__ void init_uniform_kernel(AuxArgs a) {
     for (int k = 0; k < D; k++) uu = uu + u[k] * u[k];
     const double first = udc * a.c.invC2s + (udc * udc) * (0.5 * a.c.invC4s);
     const double second = first - uu * (0.5 * a.c.invC2s) + 1.0;
-    f[q * a.g.N + n] = from_f64<S, Q>(second * (L::w(q) * rho), q, a.c.rho0);
+    f[pop_index(a.g, Q, q, x, row)] = from_f64<S, Q>(second * (L::w(q) * rho), q, a.c.rho0);
   }
 }
 
@@ -917,8 +1009,9 @@ __global__ void set_populations_kernel(AuxArgs a) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.g.N) return;
   T *f = static_cast<T *>(a.f);
+  const int x = n % a.g.nx, row = n / a.g.nx;
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[q * a.g.N + n] = from_f64<S, Q>(a.f64[q * a.g.N + n], q, a.c.rho0);
+  for (int q = 0; q < Q; q++) f[pop_index(a.g, Q, q, x, row)] = from_f64<S, Q>(a.f64[q * a.g.N + n], q, a.c.rho0);
 }
 
 template <int Q, int S>
@@ -929,9 +1022,10 @@ __global__ void get_populations_kernel(AuxArgs a) {
   const T *f = static_cast<const T *>(a.f);
   const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
   const bool wall = S != kF64 && is_wall_node(a.g, x, y, z);
+  const int row = y + a.g.ny * z;
 #pragma unroll
   for (int q = 0; q < Q; q++)
-    a.out64[q * a.g.N + n] = wall ? 0.0 : to_f64<S, Q>(f[q * a.g.N + n], q, a.c.rho0);
+    a.out64[q * a.g.N + n] = wall ? 0.0 : to_f64<S, Q>(f[pop_index(a.g, Q, q, x, row)], q, a.c.rho0);
 }
 
 // hydroVariablesUpdate! (gpu/fluids.jl:7-37) on the current populations,
@@ -946,9 +1040,9 @@ __global__ void hydro_kernel(AuxArgs a) {
   if (n >= a.g.N) return;
   const T *fp = static_cast<const T *>(a.f);
   double f[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) f[q] = to_f64<S, Q>(fp[q * a.g.N + n], q, a.c.rho0);
   const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = to_f64<S, Q>(fp[pop_index(a.g, Q, q, x, y + a.g.ny * z)], q, a.c.rho0);
   if (S != kF64 && is_wall_node(a.g, x, y, z)) {
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = 0.0;
