@@ -58,9 +58,8 @@ struct lbm_handle {
   void *F = nullptr;  // force field (double or float)
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
-  double *partial = nullptr;     // [box block][particle][6][hi, lo]
-  size_t partial_cap = 0;        // doubles
-  uint32_t *ticket = nullptr;    // box blocks finished in the running tick
+  double *partial = nullptr;     // 2 parities x [box block][particle][6][hi, lo]
+  size_t partial_cap = 0;        // doubles per parity
   // box blocks leading the step kernel's grid (see StepParticles)
   int32_t first_block[LBM_MAX_PARTICLES + 1] = {0};
   Tiling tiling{};
@@ -133,14 +132,20 @@ Tiling default_tiling(const lbm_handle *h) {
 // a tick writes the other parity while the step kernel may still read this one).
 DevParticle *parts_of(lbm_handle *h, int b) { return h->parts_d + b * LBM_MAX_PARTICLES; }
 
+// Nodes per axis the box blocks may own for a particle: the largest link
+// box its radius allows, grown by kBoxMargin on both sides.
+int32_t owned_extent(const lbm_handle *h, const DevParticle &p, int k) {
+  return std::min(link_box_extent(p.radius, h->cfg.dx) + 2 * kBoxMargin, h->n[k]);
+}
+
 // Box blocks for blocks of `threads` threads: enough per particle to cover
-// the largest link box its radius allows; partials for every box block.
+// the largest region it may own; partials for every box block.
 int plan_boxes(lbm_handle *h, int64_t threads) {
   int32_t blocks = 0;
   for (size_t p = 0; p < h->parts_h.size(); p++) {
     h->first_block[p] = blocks;
     int64_t vol = 1;
-    for (int k = 0; k < h->dims; k++) vol *= std::min(link_box_extent(h->parts_h[p].radius, h->cfg.dx), h->n[k]);
+    for (int k = 0; k < h->dims; k++) vol *= owned_extent(h, h->parts_h[p], k);
     blocks += (int32_t)((vol + threads - 1) / threads);
   }
   for (size_t p = h->parts_h.size(); p <= LBM_MAX_PARTICLES; p++) h->first_block[p] = blocks;
@@ -155,13 +160,14 @@ int plan_boxes(lbm_handle *h, int64_t threads) {
     HIP_TRY(hipMalloc(&h->linkval, lv), "hipMalloc link values");
     h->linkval_cap = lv;
   }
+  // (between runs no move is pending, so the partials may be reallocated)
   const size_t need = (size_t)std::max(blocks, 1) * LBM_MAX_PARTICLES * 6 * 2;
   if (need > h->partial_cap) {
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     if (h->partial) hipFree(h->partial);
     h->partial = nullptr;
     h->partial_cap = 0;
-    HIP_TRY(hipMalloc(&h->partial, need * sizeof(double)), "hipMalloc partials");
+    HIP_TRY(hipMalloc(&h->partial, 2 * need * sizeof(double)), "hipMalloc partials");
     h->partial_cap = need;
   }
   return LBM_OK;
@@ -169,13 +175,13 @@ int plan_boxes(lbm_handle *h, int64_t threads) {
 
 int64_t tile_threads(const Tiling &t) { return (int64_t)t.bx * t.by * t.bz; }
 
-// Doubles per axis of a box block's coordinate stage: the largest link-box
+// Doubles per axis of a box block's coordinate stage: the largest owned
 // extent plus one node on each side.
 int32_t coord_stride(const lbm_handle *h) {
   int32_t most = 1;
   for (const DevParticle &p : h->parts_h)
     for (int k = 0; k < h->dims; k++)
-      most = std::max(most, std::min(link_box_extent(p.radius, h->cfg.dx), h->n[k]) + 2);
+      most = std::max(most, owned_extent(h, p, k) + 2);
   return most;
 }
 
@@ -192,9 +198,9 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   sp.parts_in = parts_of(h, pin);
   sp.parts_out = parts_of(h, pin ^ 1);
   for (int k = 0; k < 3; k++) sp.coords[k] = h->coords_d[k];
-  sp.partial = h->partial;
+  sp.partial_in = h->partial + (size_t)pin * h->partial_cap;
+  sp.partial_out = h->partial + (size_t)(pin ^ 1) * h->partial_cap;
   sp.linkval = h->linkval;
-  sp.ticket = h->ticket;
   sp.np = (int32_t)h->parts_h.size();
   sp.semantics = h->cfg.semantics;
   sp.dims = h->dims;
@@ -347,7 +353,6 @@ int lbm_destroy(lbm_handle *h) {
   if (h->F) hipFree(h->F);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
-  if (h->ticket) hipFree(h->ticket);
   if (h->rho_side) hipFree(h->rho_side);
   if (h->linkval) hipFree(h->linkval);
   for (auto e : h->ev) hipEventDestroy(e);
@@ -427,10 +432,6 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   }
   if (hipMalloc(&h->parts_d, sizeof(DevParticle) * LBM_MAX_PARTICLES * 2) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "hipMalloc particles"));
-  if (hipMalloc(&h->ticket, sizeof(uint32_t) * LBM_MAX_PARTICLES) != hipSuccess)
-    return bail(fail(LBM_ENOMEM, "hipMalloc tickets"));
-  if (hipMemset(h->ticket, 0, sizeof(uint32_t) * LBM_MAX_PARTICLES) != hipSuccess)
-    return bail(fail(LBM_EDEVICE, "clear tickets"));
   h->tiling = default_tiling(h);
   if (hipEventCreate(&h->run_start) != hipSuccess || hipEventCreate(&h->run_stop) != hipSuccess)
     return bail(fail(LBM_EDEVICE, "hipEventCreate"));
@@ -720,6 +721,13 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
   a ^= 1;
   h->pcur ^= 1;
   h->cur = a;
+  if (!h->parts_h.empty()) {
+    // the last tick's move
+    StepArgs fa = step_args(h, nullptr, nullptr, h->pcur);
+    fa.sp.parts_out = parts_of(h, h->pcur);  // in place
+    e = launch_finalize(fa.sp, h->stream);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+  }
   h->tick += nsteps;
   for (int64_t k = 0; k < nsteps; k++) h->time += h->cfg.dt;
   HIP_TRY(hipEventRecord(h->run_stop, h->stream), "event record");

@@ -25,6 +25,7 @@ template <int S>
 hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
+hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
 }  // namespace lbm

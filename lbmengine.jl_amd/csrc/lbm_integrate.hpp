@@ -59,16 +59,19 @@ __device__ __forceinline__ DD wave_sum(DD v) {
 }
 constexpr int kWaveSumLane = 63;
 
-// What the kernels of a tick need for the particles.  Records are
-// double-buffered by tick parity: the kernels of tick t read parts_in while
-// the block that integrates writes the records of tick t+1 into parts_out.
+// What the kernels of a tick need for the particles.  A box kernel reads
+// the records of its tick (parts_in) and, when they are pending, first
+// applies the move of the previous box kernel from its partials
+// (partial_in); it writes its own partials (partial_out) and, from box block
+// 0, the records of the next kernel (parts_out, pending).  Records and
+// partials are double-buffered by tick parity.
 struct StepParticles {
   const DevParticle *parts_in;
   DevParticle *parts_out;
   const double *coords[3];
-  double *partial;       // [box block][particle][6][hi, lo]
-  void *linkval;         // [box block][T (Q - 1)]: link-slot values (compute type)
-  uint32_t *ticket;      // box blocks finished in the running tick
+  const double *partial_in;  // [box block][particle][6][hi, lo] of the previous box kernel
+  double *partial_out;       // the same, of this kernel
+  void *linkval;             // [box block][T (Q - 1)]: link-slot values (compute type)
   int32_t np, semantics, dims;
   // box blocks: particle p's are [first_block[p], first_block[p + 1]);
   // nbox = first_block[np] of them lead the grid, padded to nbox_pad (a
@@ -79,11 +82,16 @@ struct StepParticles {
   double coeff;          // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
   double ls;             // lattice speed dx / dt
   double cutoff;         // 2 dx (cpu-semantics node-velocity range)
-  double shell;          // 1.5 dx: link nodes lie within r +- shell of a centre
+  double shell;          // 1.5 dx: link nodes lie within r + shell of a centre
   double dt, dx;
   double x0[3];
   int32_t n[3];
 };
+
+// Nodes a box kernel owns for a particle: its link box grown by kBoxMargin on
+// every side (clamped to the lattice), so that it holds the link nodes of the
+// moved particle and of the next tick's move as well.
+constexpr int kBoxMargin = 2;
 
 // Link box of a sphere of radius r at Rg (node indices, inclusive): every
 // solid node and every fluid node with a solid neighbour lies inside.
@@ -201,23 +209,61 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
   find_nv = any_nonzero(dL, nw) || moved;
 }
 
-// Reduce particle p's box-block partials and move it (moveParticles!) from
-// its record cur of this tick, run by the whole box block that finished
-// last.  Both reduction orders are fixed, so the result is reproducible.
-__device__ inline void integrate_particle(const StepParticles &sp, int p, const DevParticle &cur) {
-  __shared__ double tot[6];
-  const int nblocks = sp.nbox;
-  __syncthreads();  // the previous particle's copy-out may still read rec
+// The move of particle P from the partials of the previous box kernel:
+// tot = forceInput (0..2) and torqueInput (3..5), then moveParticles!, the
+// link box and the node-velocity snapshot.  Run by one thread.
+__device__ inline void move_record(DevParticle &P, const double tot[6], const StepParticles &sp) {
+  const double F[3] = {tot[0], tot[1], sp.dims == 3 ? tot[2] : 0.0};
+  const double T[3] = {tot[3], sp.dims == 3 ? tot[4] : 0.0, sp.dims == 3 ? tot[5] : 0.0};
+  bool moved, find_nv;
+  const double R0[3] = {P.Rg[0], P.Rg[1], P.Rg[2]};
+  if (sp.semantics == LBM_SEMANTICS_GPU)
+    move_gpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
+  else
+    move_cpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
+  for (int k = 0; k < 3; k++) {
+    P.force[k] = F[k];
+    P.torque[k] = T[k];
+  }
+  if (moved) {
+    for (int k = 0; k < 3; k++) {
+      // the box kernels own a margin of kBoxMargin nodes around the link
+      // box for two ticks of motion: half a node per tick at most
+      if (!(fabs(P.R[k] - R0[k]) <= 0.5 * sp.dx)) P.err = 1;
+      P.Rg[k] = P.R[k];
+    }
+    int32_t err = 0;
+    link_box(P.Rg, P.radius, sp.x0, sp.dx, sp.n, sp.dims, P.lo, P.hi, &err);
+    if (err) P.err = 1;
+  }
+  if (find_nv) {
+    for (int k = 0; k < 3; k++) {
+      P.nR[k] = P.R[k];
+      P.nV[k] = P.V[k];
+      P.nW[k] = P.W[k];
+      P.ne[k] = P.e[k];
+    }
+  }
+  P.pending = 0;
+}
+
+// Apply the pending moves of the records in LDS (recs[0..np)) from
+// partial_in, over the whole block: each component is reduced over the
+// previous kernel's nbox box blocks in a fixed order, so every block that
+// does this gets the same records.
+__device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs) {
+  __shared__ double tot[LBM_MAX_PARTICLES][6];
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   // block sizes are whole waves, so every wave reaches wave_sum with 64 lanes
   const int wave = tid >> 6, nwaves = nthreads >> 6, lane = __lane_id();
-  for (int k = wave; k < 6; k += nwaves) {
+  for (int pk = wave; pk < 6 * sp.np; pk += nwaves) {
+    const int p = pk / 6, k = pk % 6;
+    if (!recs[p].pending) continue;  // uniform over the wave
     DD s{0.0, 0.0};
-    for (int b = lane; b < nblocks; b += 64) {
-      const double *src = sp.partial + (((int64_t)b * LBM_MAX_PARTICLES + p) * 6 + k) * 2;
-      s = dd_merge(s, DD{__hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)});
+    for (int b = lane; b < sp.nbox; b += 64) {
+      const double *src = sp.partial_in + (((int64_t)b * LBM_MAX_PARTICLES + p) * 6 + k) * 2;
+      s = dd_merge(s, DD{src[0], src[1]});
     }
     s = wave_sum(s);
     if (lane == kWaveSumLane) {
@@ -225,51 +271,12 @@ __device__ inline void integrate_particle(const StepParticles &sp, int p, const 
       // forceInput = -coeff * sum(s) * c and torqueInput = -coeff *
       // cross(sum(s (x - R)), c) with c = c_q dx/dt (gpu/fluids.jl:194-203)
       const double v = s.hi + s.lo;
-      tot[k] = -sp.coeff * (v * sp.ls);
-    }
-  }
-  // The record is updated in LDS by thread 0 (global memory round trips
-  // would put a chain of HBM latencies on the tick's critical path), then
-  // copied out by the whole block.
-  static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
-  constexpr int kWords = sizeof(DevParticle) / 8;
-  __shared__ uint64_t rec[kWords];
-  {
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&cur);
-    for (int w = tid; w < kWords; w += nthreads) rec[w] = src[w];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    DevParticle &P = *reinterpret_cast<DevParticle *>(rec);
-    const double F[3] = {tot[0], tot[1], sp.dims == 3 ? tot[2] : 0.0};
-    const double T[3] = {tot[3], sp.dims == 3 ? tot[4] : 0.0, sp.dims == 3 ? tot[5] : 0.0};
-    bool moved, find_nv;
-    if (sp.semantics == LBM_SEMANTICS_GPU)
-      move_gpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
-    else
-      move_cpu(P, F, T, sp.dims, sp.dt, moved, find_nv);
-    for (int k = 0; k < 3; k++) {
-      P.force[k] = F[k];
-      P.torque[k] = T[k];
-    }
-    if (moved) {
-      for (int k = 0; k < 3; k++) P.Rg[k] = P.R[k];
-      int32_t err = 0;
-      link_box(P.Rg, P.radius, sp.x0, sp.dx, sp.n, sp.dims, P.lo, P.hi, &err);
-      if (err) P.err = 1;
-    }
-    if (find_nv) {
-      for (int k = 0; k < 3; k++) {
-        P.nR[k] = P.R[k];
-        P.nV[k] = P.V[k];
-        P.nW[k] = P.W[k];
-        P.ne[k] = P.e[k];
-      }
+      tot[p][k] = -sp.coeff * (v * sp.ls);
     }
   }
   __syncthreads();
-  uint64_t *dst = reinterpret_cast<uint64_t *>(sp.parts_out + p);
-  for (int w = tid; w < kWords; w += nthreads) dst[w] = rec[w];
+  if (tid < sp.np && recs[tid].pending) move_record(recs[tid], tot[tid], sp);
+  __syncthreads();
 }
 
 }  // namespace lbm

@@ -31,6 +31,24 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
   return hipGetLastError();
 }
 
+// Applies the move that the last box kernel of a run left pending, in place.
+__global__ void __launch_bounds__(256) finalize_kernel(StepParticles sp) {
+  __shared__ DevParticle recs[LBM_MAX_PARTICLES];
+  constexpr int kWords = sizeof(DevParticle) / 8;
+  const int tid = threadIdx.x;
+  for (int w = tid; w < sp.np * kWords; w += blockDim.x)
+    reinterpret_cast<uint64_t *>(recs)[w] = reinterpret_cast<const uint64_t *>(sp.parts_in)[w];
+  __syncthreads();
+  apply_pending_moves(sp, recs);
+  for (int w = tid; w < sp.np * kWords; w += blockDim.x)
+    reinterpret_cast<uint64_t *>(sp.parts_out)[w] = reinterpret_cast<const uint64_t *>(recs)[w];
+}
+
+hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, stream, sp);
+  return hipGetLastError();
+}
+
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l) {
   if (Q == 9)
     hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, l.shmem, l.stream, a);

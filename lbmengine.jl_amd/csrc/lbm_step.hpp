@@ -438,6 +438,26 @@ __device__ __forceinline__ bool in_box(const DevParticle &P, int x, int y, int z
   return x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] && z >= P.lo[2] && z <= P.hi[2];
 }
 
+// Range along axis k of the nodes the box blocks own for particle P (its
+// link box grown by kBoxMargin, clamped to the lattice).
+__device__ __forceinline__ void owned_range(const DevParticle &P, const BoxGeom &g, int k, int32_t &lo,
+                                            int32_t &hi) {
+  const int n = k == 0 ? g.nx : (k == 1 ? g.ny : g.nz);
+  lo = P.lo[k] - kBoxMargin < 0 ? 0 : P.lo[k] - kBoxMargin;
+  hi = P.hi[k] + kBoxMargin > n - 1 ? n - 1 : P.hi[k] + kBoxMargin;
+}
+
+__device__ __forceinline__ bool owned(const DevParticle &P, const BoxGeom &g, int x, int y, int z) {
+  const int c[3] = {x, y, z};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    int32_t lo, hi;
+    owned_range(P, g, k, lo, hi);
+    if (c[k] < lo || c[k] > hi) return false;
+  }
+  return true;
+}
+
 // Index of population q of node (x, row) in the row-blocked layout, row =
 // y + ny z: the Q planes of one x-row are contiguous.
 __device__ __forceinline__ int64_t pop_index(const BoxGeom &g, int Q, int q, int x, int row) {
@@ -526,8 +546,11 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 
-  // the particle records of this tick, in LDS (read throughout)
+  // The particle records of this tick, in LDS (read throughout), with the
+  // regions this kernel's box blocks own (from the records as read); then
+  // the previous box kernel's move, applied by every box block alike.
   __shared__ DevParticle prec[LBM_MAX_PARTICLES];
+  __shared__ int32_t cb_lo[LBM_MAX_PARTICLES][3], cb_hi[LBM_MAX_PARTICLES][3];
   {
     static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
     constexpr int kWords = sizeof(DevParticle) / 8;
@@ -536,21 +559,36 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     for (int w = tid; w < sp.np * kWords; w += nthreads) dst[w] = src[w];
   }
   __syncthreads();
+  if (tid < 3 * sp.np) {
+    const int p = tid / 3, k = tid % 3;
+    owned_range(prec[p], g, k, cb_lo[p][k], cb_hi[p][k]);
+  }
+  apply_pending_moves(sp, prec);  // (synchronises the block)
 
   int pb = 0;
   while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
-  const DevParticle &B = prec[pb];
-  const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1, ez = B.hi[2] - B.lo[2] + 1;
+  const int bl[3] = {cb_lo[pb][0], cb_lo[pb][1], cb_lo[pb][2]};
+  const int ex = cb_hi[pb][0] - bl[0] + 1, ey = cb_hi[pb][1] - bl[1] + 1, ez = cb_hi[pb][2] - bl[2] + 1;
   const int base = (bid - sp.first_block[pb]) * nthreads;
   const int li = base + tid;
-  const int x = B.lo[0] + li % ex, y = B.lo[1] + (li / ex) % ey, z = B.lo[2] + li / (ex * ey);
-  // link boxes lie inside the lattice; a node in several boxes belongs to the first
+  const int x = bl[0] + li % ex, y = bl[1] + (li / ex) % ey, z = bl[2] + li / (ex * ey);
+  // a node owned for several particles belongs to the first
   bool mine = li < ex * ey * ez;
   for (int p = 0; p < pb; p++)
-    if (in_box(prec[p], x, y, z)) mine = false;
+    if (x >= cb_lo[p][0] && x <= cb_hi[p][0] && y >= cb_lo[p][1] && y <= cb_hi[p][1] && z >= cb_lo[p][2] &&
+        z <= cb_hi[p][2])
+      mine = false;
+  // box block 0 hands the moved records to the next kernel
+  if (bid == 0) {
+    constexpr int kWords = sizeof(DevParticle) / 8;
+    for (int w = tid; w < sp.np * kWords; w += nthreads) {
+      uint64_t v = reinterpret_cast<const uint64_t *>(prec)[w];
+      reinterpret_cast<uint64_t *>(sp.parts_out)[w] = v;
+    }
+  }
 
-  // Coordinates of pb's box and one node around it, in LDS: axis k from
-  // index lo_k - 1 (periodic) at cst[k][0].
+  // Coordinates of pb's owned region and one node around it, in LDS: axis k
+  // from index bl_k - 1 (periodic) at cst[k][0].
   extern __shared__ double lds_dyn[];
   const int cstride = sp.coord_stride;
   double *cst = lds_dyn;
@@ -559,14 +597,16 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     const int nn[3] = {g.nx, g.ny, g.nz};
     for (int k = 0; k < D; k++)
       for (int i = tid; i < len[k]; i += nthreads) {
-        int gi = B.lo[k] - 1 + i;
+        int gi = bl[k] - 1 + i;
         gi = gi < 0 ? gi + nn[k] : (gi >= nn[k] ? gi - nn[k] : gi);
         cst[k * cstride + i] = sp.coords[k][gi];
       }
   }
   __syncthreads();
-  // coordinate k of the node at unwrapped index i (lo_k - 1 <= i <= hi_k + 1)
-  auto crd = [&](int k, int i) { return cst[k * cstride + (i - B.lo[k] + 1)]; };
+  // this kernel's partials are to be applied by the next one
+  if (bid == 0 && tid < sp.np) sp.parts_out[tid].pending = 1;
+  // coordinate k of the node at unwrapped index i (bl_k - 1 <= i <= bl_k + extent)
+  auto crd = [&](int k, int i) { return cst[k * cstride + (i - bl[k] + 1)]; };
   const bool wall = mine && is_wall_node(g, x, y, z);
   const bool node = mine && !wall;
   const int row = y + g.ny * z;
@@ -650,7 +690,6 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   __shared__ uint32_t wave_cnt[8];
   __shared__ DD red[8][6];
   __shared__ DD blk[LBM_MAX_PARTICLES][6];
-  __shared__ uint32_t last;
   const int wave = tid >> 6, lane = __lane_id();
   const int nwaves = (nthreads + 63) >> 6;
 
@@ -672,7 +711,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     if (w < wave) off += wave_cnt[w];
     total += wave_cnt[w];
   }
-  const int lx0 = B.lo[0], ly0 = B.lo[1], lz0 = B.lo[2];
+  const int lx0 = bl[0], ly0 = bl[1], lz0 = bl[2];
   // f*_{q̄}(x) of this node's link slots, all loads in flight at once
   R bbv[Q];
 #pragma unroll
@@ -793,27 +832,9 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   }
   if (tid < 6 * sp.np) {
     const DD s = blk[tid / 6][tid % 6];
-    double *dst = sp.partial + ((int64_t)bid * LBM_MAX_PARTICLES * 6 + tid) * 2;
-    __hip_atomic_store(dst, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(dst + 1, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-
-  // Hand-over without cache flushes: the partials went out as device-coherent
-  // (write-through) stores from wave 0 (tid < 6 np <= 48), which waits until
-  // they have completed before it takes a ticket.  The block that takes the
-  // last ticket reads every partial back with device-coherent loads and moves
-  // the particles.
-  if (wave == 0) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (tid == 0) {
-      const uint32_t done = __hip_atomic_fetch_add(sp.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = done == (uint32_t)sp.nbox - 1;
-    }
-  }
-  __syncthreads();
-  if (last) {
-    for (int p = 0; p < sp.np; p++) integrate_particle(sp, p, prec[p]);
-    if (tid == 0) __hip_atomic_store(sp.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double *dst = sp.partial_out + ((int64_t)bid * LBM_MAX_PARTICLES * 6 + tid) * 2;
+    dst[0] = s.hi;
+    dst[1] = s.lo;
   }
 
   // this node: S (gather only) or its collision P, and rho for the next tick's links
@@ -882,13 +903,13 @@ step_kernel(StepArgs a) {
   const int y = tpy * a.t.by + threadIdx.y;
   const int z = tpz * a.t.bz + threadIdx.z;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  bool boxed = false;  // in a particle's link box
+  bool boxed = false;  // owned by the box blocks
   for (int p = 0; p < a.sp.np; p++) {
     const DevParticle &P = a.sp.parts_in[p];
-    const bool tile_in = tpx * a.t.bx <= P.hi[0] && (tpx + 1) * a.t.bx > P.lo[0] &&
-                         tpy * a.t.by <= P.hi[1] && (tpy + 1) * a.t.by > P.lo[1] &&
-                         tpz * a.t.bz <= P.hi[2] && (tpz + 1) * a.t.bz > P.lo[2];
-    if (tile_in && in_box(P, x, y, z)) boxed = true;
+    const bool tile_in = tpx * a.t.bx <= P.hi[0] + kBoxMargin && (tpx + 1) * a.t.bx > P.lo[0] - kBoxMargin &&
+                         tpy * a.t.by <= P.hi[1] + kBoxMargin && (tpy + 1) * a.t.by > P.lo[1] - kBoxMargin &&
+                         tpz * a.t.bz <= P.hi[2] + kBoxMargin && (tpz + 1) * a.t.bz > P.lo[2] - kBoxMargin;
+    if (tile_in && owned(P, g, x, y, z)) boxed = true;
   }
   if (MODE != kCollide && boxed) return;  // the box blocks' node
   const int row = y + g.ny * z;
