@@ -150,7 +150,10 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir,
                   int64_t cap, int64_t *count);
 
 /* LBMpropagate!(model; ticks = nsteps) (gpu/fluids.jl:228-264): nsteps calls
- * of tick! (gpu/fluids.jl:132-226).  Asynchronous on the handle's stream. */
+ * of tick! (gpu/fluids.jl:132-226).  Asynchronous on the handle's stream:
+ * one collide kernel, nsteps - 1 fused gather+collide kernels and one gather
+ * kernel (plus one small kernel applying the last particle move), so the
+ * populations before and after are the reference's distributions. */
 int lbm_run(lbm_handle *h, int64_t nsteps);
 
 /* Wait for the handle's stream; reports a particle that left the supported
@@ -160,36 +163,38 @@ int lbm_sync(lbm_handle *h);
 /* model.tick and model.time. */
 int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time);
 
-/* Block shape of the step kernel (threads along x, y, z; product 64..1024),
- * normally chosen by the autotuner; 0 resets to the default. */
+/* Block shape of the step kernel (threads along x, y, z; product 64..512, a
+ * multiple of 64), normally chosen by the autotuner; 0 resets to the
+ * default. */
 int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz);
 
 /* Current block shape of the step kernel. */
 int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz);
 
-/* Autotuner: times the stream-collide kernel over candidate block shapes
- * (reading the current populations, writing a scratch buffer, so the state
- * is untouched), keeps the fastest and returns it with its time per launch.
+/* Autotuner: times the fused step kernel over candidate block shapes and
+ * tile orders (reading the current populations, writing a scratch buffer,
+ * so the state is untouched), keeps the fastest and returns it with its time
+ * per launch.
  * No counterpart in the reference (CUDA.jl launches 8x8x8 blocks,
  * gpu/aux.jl:12-21). */
 int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms);
 
-/* lbm_run replays captured hipGraphs of `ticks_per_graph` ticks (even) and
- * launches the remainder directly; 0 launches every tick directly.  Default
- * 32.  The result is the same either way. */
+/* lbm_run replays captured hipGraphs of `ticks_per_graph` fused kernels
+ * (even) and launches the remainder directly; 0 launches every kernel
+ * directly.  Default 32.  The result is the same either way. */
 int lbm_set_graph(lbm_handle *h, int32_t ticks_per_graph);
 
 /* Measurement (no counterpart in the reference, which has no timing):
- * with timing on, lbm_run records HIP events around every stream-collide
- * kernel on the handle's stream, waits for the run and keeps the averages. */
+ * with timing on, lbm_run records HIP events around every fused step kernel
+ * on the handle's stream, waits for the run and keeps the averages. */
 int lbm_set_timing(lbm_handle *h, int on);
 
 /* Average device time per step kernel over the last timed lbm_run, in ms
  * (0 if no timed run happened). */
 int lbm_last_run_ms(lbm_handle *h, double *ms);
 
-/* Averages of the last timed lbm_run: stream-collide kernel time and whole
- * step time (all kernels of a tick), in ms.  Either pointer may be NULL. */
+/* Averages of the last timed lbm_run: fused step kernel time and whole run
+ * time per tick, in ms.  Either pointer may be NULL. */
 int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms);
 
 /* Measurement: average time (ms) of a plain device-to-device copy of `bytes`

@@ -1,22 +1,21 @@
 // lbm_step.hpp — the time step of LBMengine.jl as HIP kernels for gfx950.
 //
-// One step = tick! (gpu/fluids.jl:132-226) is one launch of step_kernel:
-//   every node   load f_i(x) (one coalesced read per q), hydro moments,
-//                BGK/TRT collision + Guo forcing, then push f*_i(x) to
-//                x + c_i; static walls handled analytically (zeroed wall
-//                layers, halfway bounce-back into own slots).
-//   tiles that   additionally test the spheres: halfway bounce-back at
-//   touch a      particle links, the Ladd moving-wall term with the
-//   particle's   squirmer slip, and the momentum exchange reduced per block
-//   link box     in double-double; the block that finishes a particle's
-//                last box tile reduces the partials in a fixed order and
-//                moves the particle (moveParticles!) for the next tick.
+// tick! (gpu/fluids.jl:132-226) = collide, stream with halfway bounce-back
+// and the Ladd term, momentum exchange, move the particles.  Here a run of n
+// ticks is n + 1 launches of step_kernel in the pull form:
+//   kCollide  P_0 = C(S_0)                      (S: the reference's
+//   kFused    S_k = G(P_{k-1}), P_k = C(S_k)     model.distributions,
+//   kGather   S_n = G(P_{n-1})                   P: after collision)
+// G gathers f_q(x) = f*_q(x - c_q) (one read per population, each node
+// writes only its own populations), with halfway bounce-back where x - c_q
+// is a wall or particle node, the Ladd moving-wall term with the squirmer
+// slip, and the momentum exchange; C is BGK/TRT with Guo forcing.  The
+// nodes around the particles are handled by the leading "box" blocks of each
+// gathering kernel (see box_work); the others stream the rest.
 //
-// Populations are SoA in the reference's Julia layout, f[q*N + n] with
-// n = x + nx*(y + ny*z).  The stored state is the reference's
-// model.distributions after streaming, so the step reads each node's own
-// populations (aligned) and scatters them (push).  FP64 storage holds f;
-// FP32 holds f - w_i rho0; FP16 holds (f - w_i rho0) * 2^14 / w_i.
+// Populations are stored row-blocked: the Q planes of each x-row are
+// contiguous, f[(row Q + q) nx + x] with row = y + ny z.  FP64 storage holds
+// f; FP32 holds f - w_i rho0; FP16 holds (f - w_i rho0) * 2^14 / w_i.
 #pragma once
 
 #include "lbm_common.hpp"
@@ -522,19 +521,22 @@ __device__ __forceinline__ typename Store<S>::T zero_pop(const CollParams &c, in
 }
 
 // -------------------------------------------------------------- boxes
-// The nodes of every particle's link box are handled by the first blocks of
-// the step kernel's grid (dispatched first, spread over all XCDs), while the
-// other blocks do the rest of the lattice.  A box node gathers like any
+// The nodes around every particle (its link box grown by kBoxMargin, the
+// owned region) are handled by the first blocks of a gathering kernel's grid
+// (dispatched first, spread over all XCDs), while the other blocks do the
+// rest of the lattice.  Every box block first applies the move left pending
+// by the previous box kernel (moveParticles!, from that kernel's partials,
+// in a fixed order, so all blocks agree).  A box node then gathers like any
 // other, except at particle links: a fluid node x whose neighbour x - c_q is
 // solid takes f*_{q̄}(x) (halfway bounce-back) plus the Ladd terms of the
 // particles the link belongs to, in particle order (gpu/fluids.jl:157-207);
 // each particle's momentum exchange is taken from the running value and
-// reduced over the block into one double-double partial per block.  The box
-// block that finishes last reduces all partials in a fixed order and moves
-// the particles (moveParticles!) into the records of the next tick.
+// reduced over the block into one double-double partial per block, which
+// the next box kernel (or the finalize kernel after a run) turns into the
+// move.
 //
-// Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's link
-// box in x-fastest order (j = bid - first_block[pb], T threads per block).
+// Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's owned
+// region in x-fastest order (j = bid - first_block[pb], T threads per block).
 template <int Q, int S, int COLL, int FM, int MODE>
 __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   using L = Lattice<Q>;
