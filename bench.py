@@ -84,7 +84,9 @@ def _pmc_means(run_dir):
     for f in files:
         for r in csv.DictReader(open(f)):
             name = r["Kernel_Name"]
-            fam = "step_kernel" if "step_kernel" in name else ("copy_kernel" if "copy_kernel" in name else None)
+            # the fused gather+collide kernel (last template argument 1) and the copy
+            fused = "step_kernel<" in name and name.replace(" ", "").endswith(",1>(lbm::StepArgs)")
+            fam = "step_kernel" if fused else ("copy_kernel" if "copy_kernel" in name else None)
             if fam:
                 acc[(fam, r["Counter_Name"])].append(float(r["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
@@ -110,7 +112,13 @@ def traffic_summary(fetch_dir, write_dir, storage, n):
             "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
             "raw_kib": {"step_fetch": f[("step_kernel", "FETCH_SIZE")], "step_write": w[("step_kernel", "WRITE_SIZE")],
                         "copy_fetch": f[("copy_kernel", "FETCH_SIZE")], "copy_write": w[("copy_kernel", "WRITE_SIZE")]},
-            "dispatches": {"step_fetch": fc[("step_kernel", "FETCH_SIZE")], "step_write": wc[("step_kernel", "WRITE_SIZE")]}}
+            "dispatches": {"step_fetch": fc[("step_kernel", "FETCH_SIZE")], "step_write": wc[("step_kernel", "WRITE_SIZE")]},
+            "made_by": [
+                f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {fetch_dir} -o step "
+                f"--output-format csv -- python3 bench.py --steps 20 --warmup 5 --no-cpu --no-variants --block 192,1,1",
+                f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {write_dir} -o step "
+                f"--output-format csv -- python3 bench.py --steps 20 --warmup 5 --no-cpu --no-variants --block 192,1,1",
+                f"python3 bench.py --traffic-from {fetch_dir} {write_dir}"]}
 
 
 def cpu_baseline(args):

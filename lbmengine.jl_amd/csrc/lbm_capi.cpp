@@ -919,7 +919,8 @@ int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms) {
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) e = launch_copy(a, b, bytes, s);  // warm-up
   if (e == hipSuccess) e = hipEventRecord(e0, s);
-  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(a, b, bytes, s);
+  // back and forth, as the step's populations go
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = (r & 1) ? launch_copy(b, a, bytes, s) : launch_copy(a, b, bytes, s);
   if (e == hipSuccess) e = hipEventRecord(e1, s);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0.0f;

@@ -5,29 +5,19 @@
 
 namespace lbm {
 
-// Plain device copy of n16 16-byte words: the measured HBM ceiling, and the
-// kernel of known traffic (n16*16 bytes read and written) that scales the
-// byte counters of the profiler.
+// Plain device copy of n16 16-byte words, one per thread: the measured HBM
+// ceiling, and the kernel of known traffic (n16*16 bytes read and written)
+// that scales the byte counters of the profiler.
 __global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                    int64_t n16) {
-  // four independent 16-byte loads in flight per thread, then four stores
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for (; i < n16; i += stride) dst[i] = src[i];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream) {
   const int64_t n16 = bytes / 16;
-  const int blocks = 256 * 32;  // 32 blocks of 256 threads per CU, grid-stride beyond
-  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4 *>(src),
-                     static_cast<uint4 *>(dst), n16);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, stream,
+                     static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
   return hipGetLastError();
 }
 
