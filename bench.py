@@ -12,6 +12,13 @@ kernel (algorithmic bytes 2*Q*sizeof(storage) per node over its average
 duration from HIP events on its stream), the CPU baseline (the oracle, a C
 restatement of the reference step, on this host's cores), FP64/FP32 variants
 of the same step, and the FP16-vs-FP64 errors after --accuracy-steps steps.
+
+The "batch" object measures the batched workload: 64 independent D3Q19 TRT
+periodic boxes, edges drawn from {48, 64, 96}, one squirmer each (radius
+6-10, beta uniform in [-5, 5], B1 = 1e-3), FP32 storage, all advanced 2000
+ticks together (one launch per tick over every box, graph-replayed), with
+each box's velocity field and swimming speed against the same box run alone
+in FP64.
 """
 from __future__ import annotations
 
@@ -61,6 +68,68 @@ def run_wall(m, steps, warmup):
     m.run(steps)
     m.sync()
     return time.perf_counter() - t0
+
+
+def batch_boxes(lh, storage, seed, count=64):
+    """The batched workload's boxes (same seed, same boxes for any storage)."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for _ in range(count):
+        n = int(rng.choice([48, 64, 96]))
+        r = float(rng.uniform(6.0, 10.0))
+        beta = float(rng.uniform(-5.0, 5.0))
+        lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
+        pos = rng.uniform(lo, hi, 3)
+        e = rng.standard_normal(3)
+        m = lh.modelInit(**workload_kwargs(storage, n))
+        lh.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2e-3 / 3, beta=beta,
+                       swimmingDirection=e / np.linalg.norm(e), coupleForces=True, coupleTorques=True)
+        models.append(m)
+    return models
+
+
+def batch_bench(lh, args):
+    """64 uneven FP32 boxes advanced together; throughput, batched kernel
+    roofline, and per-box errors against FP64 runs of each box alone."""
+    Q = 19
+    boxes = batch_boxes(lh, "f32", args.seed)
+    nodes = sum(int(np.prod(m.resolved.n[:3])) for m in boxes)
+    b = lh.LBMbatch(boxes)
+    block, tuned_ms = b.autotune(5)
+    b.run(args.batch_warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(args.batch_steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.set_timing(True)
+    b.run(200)
+    b.sync()
+    b.set_timing(False)
+    kms, sms = b.last_run_times()
+    bytes_per_launch = 2 * Q * BYTES["f32"] * nodes
+    res = {"workload": "64 D3Q19 TRT periodic boxes, edges from {48,64,96}, one squirmer each (r 6-10, "
+                       "beta U[-5,5], B1=1e-3), f32 storage, one launch per tick over all boxes",
+           "boxes": len(boxes), "nodes": nodes, "steps": args.batch_steps, "warmup": args.batch_warmup,
+           "MLUPS": round(nodes * args.batch_steps / wall / 1e6, 1),
+           "ms_per_step": round(wall / args.batch_steps * 1e3, 5), "block": list(block),
+           "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
+           "hbm_frac": round(bytes_per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if args.batch_accuracy:
+        ticks = args.batch_warmup + args.batch_steps + 200
+        ref = batch_boxes(lh, "f64", args.seed)
+        eu, es = [], []
+        for lo, hi in zip(boxes, ref):
+            hi.run(ticks)
+            hi.sync()
+            u64, v64 = hi.fluidVelocity, hi.particles[0].velocity
+            eu.append(float(np.linalg.norm(lo.fluidVelocity - u64) / np.linalg.norm(u64)))
+            es.append(float(abs(np.linalg.norm(lo.particles[0].velocity) - np.linalg.norm(v64))
+                            / np.linalg.norm(v64)))
+            del hi
+        res["accuracy_vs_f64"] = {"ticks": ticks, "u_rel_l2_max": max(eu), "u_rel_l2_median": float(np.median(eu)),
+                                  "speed_rel_max": max(es)}
+    return res
 
 
 def kernel_times(m, steps):
@@ -158,6 +227,10 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-warmup", type=int, default=100)
+    ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
     ap.add_argument("--block", default="", help="bx,by,bz of the step kernel instead of autotuning")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of this script into the traffic file and exit")
@@ -259,6 +332,9 @@ def main():
         acc["steps"] = args.accuracy_steps
         acc["swimming_speed_f64"] = float(np.linalg.norm(v64))
         out["accuracy_vs_f64"] = acc
+
+    if not args.no_batch:
+        out["batch"] = batch_bench(lh, args)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)

@@ -203,6 +203,46 @@ int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms);
  * profiler byte counters.  No counterpart in the reference. */
 int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms);
 
+/* ------------------------------------------------------------ batches
+ * A batch advances up to 256 independent boxes (handles made by lbm_create,
+ * of one lattice, storage, collision and force kind; any sizes, parameters,
+ * walls and particles) together: each step kernel of a run is one launch
+ * over every box, and the run's fused kernels replay from captured
+ * hipGraphs.  Each box ends exactly where lbm_run(box, nsteps) would leave
+ * it (same kernels, same arithmetic, bit for bit).
+ * No counterpart in the reference, which advances one LBMmodel per
+ * LBMpropagate! call (gpu/fluids.jl:228-264); a sweep there is a loop over
+ * models. */
+typedef struct lbm_batch lbm_batch;
+
+/* The batch borrows the handles (they must outlive it) and gives every box
+ * its block shape (default: box 0's, if it fits every box). */
+int lbm_batch_create(lbm_handle *const *boxes, int32_t n, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *b);
+
+/* LBMpropagate!(box; ticks = nsteps) for every box of the batch, as one
+ * sequence of batched kernels on the batch's stream.  Each box's own stream
+ * waits for it, so later calls on a box see the result. */
+int lbm_batch_run(lbm_batch *b, int64_t nsteps);
+
+/* Wait for the batch; reports LBM_ERANGE for a particle that left its box's
+ * supported region, as lbm_sync does. */
+int lbm_batch_sync(lbm_batch *b);
+
+/* Block shape shared by the boxes (the rules of lbm_set_block). */
+int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz);
+int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz);
+
+/* Autotuner of the batched fused kernel over the block shapes that fit every
+ * box and both tile orders (the boxes' state is untouched). */
+int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best_ms);
+
+/* Ticks per captured graph (0 or even; default 32), timing and the averages
+ * of the last timed run, as for one handle. */
+int lbm_batch_set_graph(lbm_batch *b, int32_t ticks_per_graph);
+int lbm_batch_set_timing(lbm_batch *b, int on);
+int lbm_batch_last_run_times(lbm_batch *b, double *kernel_ms, double *step_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -187,6 +187,7 @@ int32_t coord_stride(const lbm_handle *h) {
 
 StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   StepArgs a;
+  std::memset(&a, 0, sizeof a);  // (batches compare argument blocks bytewise)
   a.fin = fin;
   a.fout = fout;
   a.Ffield = h->F;
@@ -224,18 +225,21 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   return a;
 }
 
+// Dynamic LDS of a box block: the coordinate stage, then 2t link slots
+// (f*, value, key) and t rho.
+size_t box_shmem(const lbm_handle *h, const StepArgs &a) {
+  const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+  const size_t t = (size_t)tile_threads(a.t);
+  return 3 * (size_t)a.sp.coord_stride * 8 + 5 * t * rsize + 2 * t * 4;
+}
+
 // Launch shape of a step kernel of the given mode: in the gathering modes the
 // box blocks come first, then the tiles; the box blocks' link lists take
 // dynamic LDS.
 Launch step_launch(lbm_handle *h, const StepArgs &a, int mode) {
   const int nbox = mode == kCollide ? 0 : a.sp.nbox_pad;
   Launch l{dim3((unsigned)(nbox + h->tiling.ntiles)), dim3(h->tiling.bx, h->tiling.by, h->tiling.bz), h->stream};
-  if (nbox) {
-    const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
-    const size_t t = (size_t)tile_threads(h->tiling);
-    // coordinate stage, then 2t link slots (f*, value, key) and t rho
-    l.shmem = 3 * (size_t)a.sp.coord_stride * 8 + 5 * t * rsize + 2 * t * 4;
-  }
+  if (nbox) l.shmem = box_shmem(h, a);
   return l;
 }
 
@@ -941,6 +945,471 @@ int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz) {
   if (bx) *bx = h->tiling.bx;
   if (by) *by = h->tiling.by;
   if (bz) *bz = h->tiling.bz;
+  return LBM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// A batch advances independent lattices together: every step kernel of a
+// run covers all of them in one launch (batch_step_kernel), the launches of
+// a run are the one-lattice run's (collide, fused..., gather, finalize), and
+// the fused ones replay from captured graphs.  Each kernel reads lattice
+// i's arguments from a device table, so the graphs stay valid while the
+// lattices swap buffers.
+
+// Sections of a batch's argument table, n entries each.
+enum BatchSection { kSecCollide = 0, kSecStep0 = 1, kSecStep1 = 2, kSecFinal0 = 3, kSecFinal1 = 4, kSections = 5 };
+
+struct lbm_batch {
+  std::vector<lbm_handle *> boxes;
+  int Q = 0, storage = 0, coll = 0, force = 0;
+  Tiling shape{};  // block shape and tile order of every box (bx, by, bz, remap)
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  // argument tables, one per slot (box 0's population buffer at run start)
+  StepArgs *args_d[2] = {nullptr, nullptr};
+  std::vector<StepArgs> args_h[2];  // what the device tables hold
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int64_t graph_k = 32;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t run_start = nullptr, run_stop = nullptr;
+  double step_kernel_ms = 0.0, step_ms = 0.0;
+};
+
+namespace {
+
+int n_boxes(const lbm_batch *b) { return (int)b->boxes.size(); }
+
+void drop_batch_graphs(lbm_batch *b) {
+  for (int s = 0; s < 2; s++) {
+    if (b->graph[s]) hipGraphExecDestroy(b->graph[s]);
+    b->graph[s] = nullptr;
+  }
+}
+
+// Give box h the batch's block shape (and box-block plan for it).
+int adopt_shape(lbm_handle *h, const Tiling &shape) {
+  if (h->tiling.bx == shape.bx && h->tiling.by == shape.by && h->tiling.bz == shape.bz &&
+      h->tiling.remap == shape.remap)
+    return LBM_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  Tiling t = make_tiling(h, shape.bx, shape.by, shape.bz);
+  t.remap = shape.remap;
+  if (const int rc = plan_boxes(h, tile_threads(t))) return rc;
+  h->tiling = t;
+  drop_graph(h);
+  return LBM_OK;
+}
+
+// Box i's arguments for every section, from its current buffers and records:
+// collide reads f[cur]; fused (and gather) kernel k of the run reads
+// f[cur ^ 1 ^ (k & 1)] with records pcur ^ (k & 1), as lbm_run does; the
+// finalize after a run whose fused count has parity ph works in place on
+// records pcur ^ ph ^ 1.
+void box_sections(lbm_handle *h, StepArgs out[kSections]) {
+  const int c = h->cur, p = h->pcur;
+  out[kSecCollide] = step_args(h, h->f[c], h->f[c ^ 1], p);
+  for (int ph = 0; ph < 2; ph++) {
+    out[kSecStep0 + ph] = step_args(h, h->f[c ^ 1 ^ ph], h->f[c ^ ph], p ^ ph);
+    StepArgs fa = step_args(h, nullptr, nullptr, p ^ ph ^ 1);
+    fa.sp.parts_out = parts_of(h, p ^ ph ^ 1);
+    out[kSecFinal0 + ph] = fa;
+  }
+}
+
+// Refresh slot s's device table from the boxes' current state (a copy only
+// when something changed, after the batch's earlier work is done).
+int upload_table(lbm_batch *b, int s) {
+  const int n = n_boxes(b);
+  std::vector<StepArgs> t((size_t)n * kSections);
+  for (int i = 0; i < n; i++) {
+    StepArgs sec[kSections];
+    box_sections(b->boxes[i], sec);
+    for (int k = 0; k < kSections; k++) t[(size_t)k * n + i] = sec[k];
+  }
+  if (b->args_h[s].size() == t.size() && std::memcmp(b->args_h[s].data(), t.data(), t.size() * sizeof(StepArgs)) == 0)
+    return LBM_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  if (!b->args_h[s].empty()) drop_batch_graphs(b);  // (block counts may have changed)
+  b->args_h[s] = std::move(t);
+  // stream-ordered; the host copy stays untouched until the stream is idle
+  HIP_TRY(hipMemcpyAsync(b->args_d[s], b->args_h[s].data(), b->args_h[s].size() * sizeof(StepArgs),
+                         hipMemcpyHostToDevice, b->stream),
+          "copy batch table");
+  return LBM_OK;
+}
+
+// Grid of a batched kernel over the boxes' arguments `host` (one section):
+// box i's blocks start at a multiple of 8, so each box's tiles keep the XCD
+// order the one-lattice kernel gives them.
+Launch batch_launch(lbm_batch *b, const StepArgs *host, const StepArgs *dev, int mode, BatchArgs &ba) {
+  const int n = n_boxes(b);
+  ba.boxes = dev;
+  ba.n = n;
+  ba.start[0] = 0;
+  size_t shmem = 0;
+  for (int i = 0; i < n; i++) {
+    const StepArgs &a = host[i];
+    const int nbox = mode == kCollide ? 0 : a.sp.nbox_pad;
+    if (nbox) shmem = std::max(shmem, box_shmem(b->boxes[i], a));
+    ba.start[i + 1] = ba.start[i] + (nbox + a.t.ntiles + 7) / 8 * 8;
+  }
+  Launch l{dim3((unsigned)ba.start[n]), dim3(b->shape.bx, b->shape.by, b->shape.bz), b->stream};
+  l.shmem = shmem;
+  return l;
+}
+
+hipError_t launch_batch_mode(lbm_batch *b, int mode, const StepArgs *host, const StepArgs *dev) {
+  BatchArgs ba;
+  Launch l = batch_launch(b, host, dev, mode, ba);
+  switch (b->storage) {
+    case LBM_STORAGE_F64: return launch_batch_step<kF64>(b->Q, b->coll, b->force, mode, ba, l);
+    case LBM_STORAGE_F32: return launch_batch_step<kF32>(b->Q, b->coll, b->force, mode, ba, l);
+    default: return launch_batch_step<kF16>(b->Q, b->coll, b->force, mode, ba, l);
+  }
+}
+
+// Section k of slot s: host copy and device address.
+const StepArgs *sec_host(const lbm_batch *b, int s, int k) { return b->args_h[s].data() + (size_t)k * n_boxes(b); }
+const StepArgs *sec_dev(const lbm_batch *b, int s, int k) { return b->args_d[s] + (size_t)k * n_boxes(b); }
+
+hipError_t batch_fused(lbm_batch *b, int s, int ph) {
+  return launch_batch_mode(b, kFused, sec_host(b, s, kSecStep0 + ph), sec_dev(b, s, kSecStep0 + ph));
+}
+
+int capture_batch_graph(lbm_batch *b, int s, hipGraphExec_t *out) {
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  hipError_t e = hipSuccess;
+  for (int64_t i = 0; i < b->graph_k && e == hipSuccess; i++) e = batch_fused(b, s, (int)(i & 1));
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(e != hipSuccess ? e : e2, "capture batch graph");
+  }
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "instantiate batch graph");
+  return LBM_OK;
+}
+
+bool batch_has_particles(const lbm_batch *b) {
+  for (const lbm_handle *h : b->boxes)
+    if (!h->parts_h.empty()) return true;
+  return false;
+}
+
+// Every box populated and at the batch's block shape, and its own stream idle.
+int prepare_boxes(lbm_batch *b) {
+  for (lbm_handle *h : b->boxes) {
+    if (!h->populated) return fail(LBM_ESTATE, "a box of the batch has no populations yet");
+    if (const int rc = adopt_shape(h, b->shape)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_destroy(lbm_batch *b) {
+  if (!b) return LBM_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  drop_batch_graphs(b);
+  for (int s = 0; s < 2; s++)
+    if (b->args_d[s]) hipFree(b->args_d[s]);
+  for (auto e : b->ev) hipEventDestroy(e);
+  if (b->run_start) hipEventDestroy(b->run_start);
+  if (b->run_stop) hipEventDestroy(b->run_stop);
+  if (b->done) hipEventDestroy(b->done);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return LBM_OK;
+}
+
+int lbm_batch_create(lbm_handle *const *boxes, int32_t n, lbm_batch **out) {
+  if (!boxes || !out) return fail(LBM_EINVAL, "null argument");
+  *out = nullptr;
+  if (n < 1 || n > kMaxBatch) return fail(LBM_EINVAL, "a batch holds 1.." + std::to_string(kMaxBatch) + " boxes");
+  for (int i = 0; i < n; i++) {
+    if (!boxes[i]) return fail(LBM_EINVAL, "null box handle");
+    for (int j = 0; j < i; j++)
+      if (boxes[j] == boxes[i]) return fail(LBM_EINVAL, "a box appears twice in the batch");
+    const lbm_config &c0 = boxes[0]->cfg, &c = boxes[i]->cfg;
+    if (c.q != c0.q || c.storage != c0.storage || c.collision != c0.collision || c.force != c0.force)
+      return fail(LBM_EINVAL, "the boxes of a batch share lattice, storage, collision and force kinds");
+  }
+  lbm_batch *b = new (std::nothrow) lbm_batch;
+  if (!b) return fail(LBM_ENOMEM, "out of host memory");
+  b->boxes.assign(boxes, boxes + n);
+  b->Q = boxes[0]->Q;
+  b->storage = boxes[0]->cfg.storage;
+  b->coll = boxes[0]->cfg.collision;
+  b->force = boxes[0]->cfg.force;
+  auto bail = [&](int code) {
+    lbm_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&b->run_start) != hipSuccess || hipEventCreate(&b->run_stop) != hipSuccess)
+    return bail(fail(LBM_EDEVICE, "create batch stream and events"));
+  for (int s = 0; s < 2; s++)
+    if (hipMalloc(&b->args_d[s], (size_t)n * kSections * sizeof(StepArgs)) != hipSuccess)
+      return bail(fail(LBM_ENOMEM, "hipMalloc batch tables"));
+  // default shape: box 0's current one, if every box admits it
+  Tiling t = boxes[0]->tiling;
+  for (int i = 0; i < n; i++)
+    if (t.bx > boxes[i]->n[0] || t.by > boxes[i]->n[1] || t.bz > boxes[i]->n[2]) t = Tiling{64, 1, 1, 0, 0, 0, 0, 1};
+  b->shape = t;
+  for (lbm_handle *h : b->boxes)
+    if (const int rc = adopt_shape(h, b->shape)) return bail(rc);
+  *out = b;
+  return LBM_OK;
+}
+
+int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
+  if (nsteps == 0) return LBM_OK;
+  if (const int rc = prepare_boxes(b)) return rc;
+  const int s = b->boxes[0]->cur;
+  if (const int rc = upload_table(b, s)) return rc;
+  const int64_t nfused = nsteps - 1;
+  if (b->timing && nfused > 0 && (int64_t)b->ev.size() < 2 * nfused) {
+    const size_t old = b->ev.size();
+    b->ev.resize(2 * nfused);
+    for (size_t i = old; i < b->ev.size(); i++) HIP_TRY(hipEventCreate(&b->ev[i]), "hipEventCreate");
+  }
+  HIP_TRY(hipEventRecord(b->run_start, b->stream), "event record");
+  hipError_t e = launch_batch_mode(b, kCollide, sec_host(b, s, kSecCollide), sec_dev(b, s, kSecCollide));
+  if (e != hipSuccess) return hip_fail(e, "batch step launch");
+  int64_t k = 0;
+  if (!b->timing && b->graph_k > 0) {
+    while (nfused - k >= b->graph_k) {
+      if (!b->graph[s])
+        if (const int rc = capture_batch_graph(b, s, &b->graph[s])) return rc;
+      HIP_TRY(hipGraphLaunch(b->graph[s], b->stream), "graph launch");
+      k += b->graph_k;
+    }
+  }
+  for (int64_t i = 0; k < nfused; k++, i++) {
+    if (b->timing) hipEventRecord(b->ev[2 * i], b->stream);
+    e = batch_fused(b, s, (int)(k & 1));
+    if (e != hipSuccess) return hip_fail(e, "batch step launch");
+    if (b->timing) hipEventRecord(b->ev[2 * i + 1], b->stream);
+  }
+  const int ph = (int)(nfused & 1);
+  e = launch_batch_mode(b, kGather, sec_host(b, s, kSecStep0 + ph), sec_dev(b, s, kSecStep0 + ph));
+  if (e != hipSuccess) return hip_fail(e, "batch step launch");
+  if (batch_has_particles(b)) {
+    e = launch_finalize_batch(sec_dev(b, s, kSecFinal0 + ph), n_boxes(b), b->stream);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+  }
+  HIP_TRY(hipEventRecord(b->run_stop, b->stream), "event record");
+  HIP_TRY(hipEventRecord(b->done, b->stream), "event record");
+  for (lbm_handle *h : b->boxes) {
+    // the boxes' own later work waits for the batch
+    HIP_TRY(hipStreamWaitEvent(h->stream, b->done, 0), "stream wait");
+    // as after lbm_run: nsteps + 1 buffer swaps, nsteps record swaps
+    h->cur ^= (int)((nsteps + 1) & 1);
+    h->pcur ^= (int)(nsteps & 1);
+    h->tick += nsteps;
+    for (int64_t i = 0; i < nsteps; i++) h->time += h->cfg.dt;
+  }
+  if (b->timing) {
+    HIP_TRY(hipEventSynchronize(b->run_stop), "sync");
+    double sum = 0.0;
+    for (int64_t i = 0; i < nfused; i++) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]), "event time");
+      sum += ms;
+    }
+    float tot = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&tot, b->run_start, b->run_stop), "event time");
+    b->step_kernel_ms = nfused > 0 ? sum / (double)nfused : 0.0;
+    b->step_ms = tot / (double)nsteps;
+  }
+  return LBM_OK;
+}
+
+int lbm_batch_sync(lbm_batch *b) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  for (lbm_handle *h : b->boxes) {
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    if (const int rc = check_particles(h)) return rc;
+  }
+  return LBM_OK;
+}
+
+int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  const int64_t nt = (int64_t)bx * by * bz;
+  if (bx < 1 || by < 1 || bz < 1 || nt < 64 || nt > 512 || nt % 64)
+    return fail(LBM_EINVAL, "block shape must have 64..512 threads, a multiple of 64");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  Tiling t{};
+  t.bx = bx;
+  t.by = by;
+  t.bz = bz;
+  t.remap = 1;
+  b->shape = t;
+  drop_batch_graphs(b);
+  for (lbm_handle *h : b->boxes)
+    if (const int rc = adopt_shape(h, b->shape)) return rc;
+  return LBM_OK;
+}
+
+int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (bx) *bx = b->shape.bx;
+  if (by) *by = b->shape.by;
+  if (bz) *bz = b->shape.bz;
+  return LBM_OK;
+}
+
+int lbm_batch_set_graph(lbm_batch *b, int32_t ticks_per_graph) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (ticks_per_graph < 0 || (ticks_per_graph & 1)) return fail(LBM_EINVAL, "ticks per graph must be 0 or an even number");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  drop_batch_graphs(b);
+  b->graph_k = ticks_per_graph;
+  return LBM_OK;
+}
+
+int lbm_batch_set_timing(lbm_batch *b, int on) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  b->timing = on != 0;
+  return LBM_OK;
+}
+
+int lbm_batch_last_run_times(lbm_batch *b, double *kernel_ms, double *step_ms) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (kernel_ms) *kernel_ms = b->step_kernel_ms;
+  if (step_ms) *step_ms = b->step_ms;
+  return LBM_OK;
+}
+
+// Autotuner of a batch: times the batched fused kernel (reading every box's
+// current populations, writing one scratch buffer; the records and partials
+// it also writes are the ones every run rewrites before reading) over the
+// block shapes that fit every box and both tile orders, and keeps the
+// fastest.
+int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best_ms) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (reps < 1) reps = 1;
+  if (const int rc = prepare_boxes(b)) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  const int n = n_boxes(b);
+  int32_t mx[3] = {1, 1, 1};
+  int64_t nodes = 0;
+  size_t scratch_bytes = 0;
+  std::vector<size_t> off(n);
+  std::vector<int32_t> bxs = {16, 32, 64, 96, 128, 192, 256, 512};
+  for (int i = 0; i < n; i++) {
+    const lbm_handle *h = b->boxes[i];
+    for (int k = 0; k < 3; k++) mx[k] = std::max(mx[k], h->n[k]);
+    if (h->n[0] <= 512) bxs.push_back(h->n[0]);
+    nodes += h->N;
+    off[i] = scratch_bytes;
+    scratch_bytes += ((size_t)h->N * h->Q * elem_size(b->storage) + 255) / 256 * 256;
+  }
+  std::sort(bxs.begin(), bxs.end());
+  bxs.erase(std::unique(bxs.begin(), bxs.end()), bxs.end());
+  // shapes whose idle threads (tiles overhanging the boxes) stay within 25%
+  // of the nodes, or within 15% of the least any shape achieves
+  struct Shape {
+    int32_t bx, by, bz;
+    double waste;
+  };
+  std::vector<Shape> shapes;
+  double least = 1e300;
+  for (int32_t bx : bxs)
+    for (int32_t by : {1, 2, 4, 8, 16})
+      for (int32_t bz : {1, 2, 4}) {
+        const int64_t t = (int64_t)bx * by * bz;
+        if (t < 64 || t > 512 || t % 64 || bx > mx[0] || by > mx[1] || bz > mx[2]) continue;
+        int64_t covered = 0;
+        for (const lbm_handle *h : b->boxes) covered += (int64_t)make_tiling(h, bx, by, bz).ntiles * t;
+        const double waste = (double)covered / (double)nodes - 1.0;
+        shapes.push_back({bx, by, bz, waste});
+        least = std::min(least, waste);
+      }
+  std::vector<Tiling> cands;
+  for (const Shape &sh : shapes) {
+    if (sh.waste > std::max(0.25, least + 0.15)) continue;
+    for (int remap : {1, 0}) {
+      Tiling c{};
+      c.bx = sh.bx;
+      c.by = sh.by;
+      c.bz = sh.bz;
+      c.remap = remap;
+      cands.push_back(c);
+    }
+  }
+  if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits the boxes");
+  char *scratch = nullptr;
+  StepArgs *trial_d = nullptr;
+  HIP_TRY(hipMalloc(&scratch, scratch_bytes), "hipMalloc autotune scratch");
+  if (hipMalloc(&trial_d, (size_t)n * sizeof(StepArgs)) != hipSuccess) {
+    hipFree(scratch);
+    return fail(LBM_ENOMEM, "hipMalloc autotune table");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const Tiling saved = b->shape;
+  double best_t = 1e300;
+  Tiling best_c = cands[0];
+  int rc = LBM_OK;
+  hipError_t e = hipSuccess;
+  std::vector<StepArgs> trial(n);
+  for (const Tiling &c : cands) {
+    for (lbm_handle *h : b->boxes)
+      if ((rc = adopt_shape(h, c))) break;
+    if (rc) break;
+    b->shape = c;
+    for (int i = 0; i < n; i++) {
+      lbm_handle *h = b->boxes[i];
+      trial[i] = step_args(h, h->f[h->cur], scratch + off[i], h->pcur);
+    }
+    e = hipMemcpyAsync(trial_d, trial.data(), (size_t)n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream);
+    for (int r = 0; r < 2 && e == hipSuccess; r++) e = launch_batch_mode(b, kFused, trial.data(), trial_d);
+    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_batch_mode(b, kFused, trial.data(), trial_d);
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    if (ms / reps < best_t) {
+      best_t = ms / reps;
+      best_c = c;
+    }
+  }
+  hipStreamSynchronize(b->stream);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(trial_d);
+  hipFree(scratch);
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "batch autotune");
+  b->shape = rc ? saved : best_c;
+  drop_batch_graphs(b);
+  for (lbm_handle *h : b->boxes) {
+    const int rc2 = adopt_shape(h, b->shape);
+    if (!rc) rc = rc2;
+  }
+  if (rc) return rc;
+  if (best) {
+    best[0] = best_c.bx;
+    best[1] = best_c.by;
+    best[2] = best_c.bz;
+  }
+  if (best_ms) *best_ms = best_t;
   return LBM_OK;
 }
 

@@ -16,6 +16,8 @@ struct Launch {
 template <int S>
 hipError_t launch_step(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
+hipError_t launch_batch_step(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l);
+template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_set_populations(int Q, const AuxArgs &a, const Launch &l);
@@ -26,6 +28,7 @@ hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);
+hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
 }  // namespace lbm

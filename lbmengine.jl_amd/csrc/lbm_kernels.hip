@@ -1,46 +1,70 @@
 // lbm_kernels.hip — launchers of the step and auxiliary kernels for
-// one storage kind.  Compiled three times with -DLBM_STORE=0 (FP64),
-// 1 (FP32) and 2 (FP16).
+// one storage kind.  Compiled once per storage, -DLBM_STORE=0 (FP64),
+// 1 (FP32) or 2 (FP16), and per part: -DLBM_PART=0 builds the one-lattice
+// step and the auxiliary kernels, -DLBM_PART=1 the batched step.
 #include "lbm_dispatch.hpp"
 
 #ifndef LBM_STORE
 #error "LBM_STORE must be 0 (FP64), 1 (FP32) or 2 (FP16)"
+#endif
+#ifndef LBM_PART
+#error "LBM_PART must be 0 (one lattice and auxiliary kernels) or 1 (batched step)"
 #endif
 
 namespace lbm {
 
 constexpr int S = LBM_STORE;
 
+// (lattice, collision, force) -> the launcher instantiated for them
+#define LBM_KEY_SWITCH(FN, ...)                          \
+  switch ((Q == 19 ? 100 : 0) + coll * 10 + fm) {        \
+    case 0: return FN<9, 0, 0>(__VA_ARGS__);             \
+    case 1: return FN<9, 0, 1>(__VA_ARGS__);             \
+    case 2: return FN<9, 0, 2>(__VA_ARGS__);             \
+    case 10: return FN<9, 1, 0>(__VA_ARGS__);            \
+    case 11: return FN<9, 1, 1>(__VA_ARGS__);            \
+    case 12: return FN<9, 1, 2>(__VA_ARGS__);            \
+    case 100: return FN<19, 0, 0>(__VA_ARGS__);          \
+    case 101: return FN<19, 0, 1>(__VA_ARGS__);          \
+    case 102: return FN<19, 0, 2>(__VA_ARGS__);          \
+    case 110: return FN<19, 1, 0>(__VA_ARGS__);          \
+    case 111: return FN<19, 1, 1>(__VA_ARGS__);          \
+    case 112: return FN<19, 1, 2>(__VA_ARGS__);          \
+    default: return hipErrorInvalidValue;                \
+  }
+
+#define LBM_MODE_SWITCH(KERNEL, ARGS)                                                                        \
+  switch (mode) {                                                                                            \
+    case kCollide: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kCollide>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+    case kFused: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kFused>), l.grid, l.block, l.shmem, l.stream, ARGS); break;     \
+    case kGather: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kGather>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+    default: return hipErrorInvalidValue;                                                                    \
+  }                                                                                                          \
+  return hipGetLastError();
+
+#if LBM_PART == 1
+
 // (internal linkage: each storage's object file has its own)
 template <int Q, int COLL, int FM>
+static hipError_t launch_batch_qcf(int mode, const BatchArgs &a, const Launch &l) {
+  LBM_MODE_SWITCH(batch_step_kernel, a)
+}
+
+template <>
+hipError_t launch_batch_step<S>(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l) {
+  LBM_KEY_SWITCH(launch_batch_qcf, mode, a, l)
+}
+
+#else
+
+template <int Q, int COLL, int FM>
 static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) {
-  switch (mode) {
-    case kCollide: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kCollide>), l.grid, l.block, l.shmem, l.stream, a); break;
-    case kFused: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kFused>), l.grid, l.block, l.shmem, l.stream, a); break;
-    case kGather: hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kGather>), l.grid, l.block, l.shmem, l.stream, a); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  LBM_MODE_SWITCH(step_kernel, a)
 }
 
 template <>
 hipError_t launch_step<S>(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l) {
-  const int key = (Q == 19 ? 100 : 0) + coll * 10 + fm;
-  switch (key) {
-    case 0: return launch_step_qcf<9, 0, 0>(mode, a, l);
-    case 1: return launch_step_qcf<9, 0, 1>(mode, a, l);
-    case 2: return launch_step_qcf<9, 0, 2>(mode, a, l);
-    case 10: return launch_step_qcf<9, 1, 0>(mode, a, l);
-    case 11: return launch_step_qcf<9, 1, 1>(mode, a, l);
-    case 12: return launch_step_qcf<9, 1, 2>(mode, a, l);
-    case 100: return launch_step_qcf<19, 0, 0>(mode, a, l);
-    case 101: return launch_step_qcf<19, 0, 1>(mode, a, l);
-    case 102: return launch_step_qcf<19, 0, 2>(mode, a, l);
-    case 110: return launch_step_qcf<19, 1, 0>(mode, a, l);
-    case 111: return launch_step_qcf<19, 1, 1>(mode, a, l);
-    case 112: return launch_step_qcf<19, 1, 2>(mode, a, l);
-    default: return hipErrorInvalidValue;
-  }
+  LBM_KEY_SWITCH(launch_step_qcf, mode, a, l)
 }
 
 #define LBM_DISPATCH_Q(KERNEL, ARGS)                                                \
@@ -74,5 +98,7 @@ hipError_t launch_hydro<S>(int Q, const AuxArgs &a, const Launch &l) {
   LBM_DISPATCH_Q(hydro_kernel, a);
   return hipGetLastError();
 }
+
+#endif
 
 }  // namespace lbm

@@ -21,8 +21,8 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
   return hipGetLastError();
 }
 
-// Applies the move that the last box kernel of a run left pending, in place.
-__global__ void __launch_bounds__(256) finalize_kernel(StepParticles sp) {
+// Applies the move that the last box kernel of a run left pending.
+__device__ __forceinline__ void finalize_block(const StepParticles &sp) {
   __shared__ DevParticle recs[LBM_MAX_PARTICLES];
   constexpr int kWords = sizeof(DevParticle) / 8;
   const int tid = threadIdx.x;
@@ -34,8 +34,22 @@ __global__ void __launch_bounds__(256) finalize_kernel(StepParticles sp) {
     reinterpret_cast<uint64_t *>(sp.parts_out)[w] = reinterpret_cast<const uint64_t *>(recs)[w];
 }
 
+__global__ void __launch_bounds__(256) finalize_kernel(StepParticles sp) { finalize_block(sp); }
+
+// One block per lattice of a batch; boxes[i].sp names lattice i's records
+// and the partials of its last box kernel.
+__global__ void __launch_bounds__(256) finalize_batch_kernel(const StepArgs *boxes) {
+  const StepParticles &sp = boxes[blockIdx.x].sp;
+  if (sp.np > 0) finalize_block(sp);
+}
+
 hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream) {
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, stream, sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(finalize_batch_kernel, dim3((unsigned)n), dim3(256), 0, stream, boxes);
   return hipGetLastError();
 }
 

@@ -877,19 +877,15 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
 }
 
 // -------------------------------------------------------------- step
-// One kernel of a run (see StepMode): blocks [0, nbox_pad) work on the
-// particles' link boxes (gathering modes only), the others each do one tile
-// outside them.  Registers are capped at the plain path's needs (FP32/FP16:
-// 64, eight waves per SIMD; FP64: 128, four), so the wider box path spills
-// rather than lowering the occupancy of every tile.
+// Block b of one lattice's part of a step kernel (see StepMode): blocks
+// [0, nbox_pad) work on the particles' link boxes (gathering modes only),
+// the others each do one tile outside them.
 template <int Q, int S, int COLL, int FM, int MODE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
-step_kernel(StepArgs a) {
+__device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   const BoxGeom &g = a.g;
-  int b = blockIdx.x;
   if constexpr (MODE != kCollide) {
     if (b < a.sp.nbox_pad) {
       if (b < a.sp.nbox) box_work<Q, S, COLL, FM, MODE>(a, b);
@@ -967,6 +963,44 @@ step_kernel(StepArgs a) {
     // rho of the links' next gather (only the collide-only kernel reaches box nodes)
     if (MODE == kCollide && boxed) static_cast<R *>(a.rho_side)[(int64_t)x + g.nx * (int64_t)row] = rho;
   }
+}
+
+// One step kernel over one lattice.  Registers are capped at the plain
+// path's needs (FP32/FP16: 64, eight waves per SIMD; FP64: 128, four), so the
+// wider box path spills rather than lowering the occupancy of every tile.
+template <int Q, int S, int COLL, int FM, int MODE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
+step_kernel(StepArgs a) {
+  step_block<Q, S, COLL, FM, MODE>(a, blockIdx.x);
+}
+
+// One step kernel over a batch of independent lattices (the same lattice,
+// storage, collision and force kinds; any sizes, parameters and particles):
+// lattice i owns blocks [start[i], start[i + 1]) of the grid, its box blocks
+// (gathering modes) and tiles followed by idle blocks up to a multiple of 8.
+constexpr int kMaxBatch = 256;
+struct BatchArgs {
+  const StepArgs *boxes;  // device array of the lattices' arguments
+  int32_t n;
+  int32_t start[kMaxBatch + 1];
+};
+
+template <int Q, int S, int COLL, int FM, int MODE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
+batch_step_kernel(BatchArgs ba) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = ba.n - 1;  // the lattice whose range holds b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ba.start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const StepArgs &a = ba.boxes[lo];
+  const int local = b - ba.start[lo];
+  if (local >= (MODE != kCollide ? a.sp.nbox_pad : 0) + a.t.ntiles) return;
+  step_block<Q, S, COLL, FM, MODE>(a, local);
 }
 
 // -------------------------------------------------------- aux kernels

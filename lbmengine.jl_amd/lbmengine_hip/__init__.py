@@ -5,12 +5,12 @@ addBead!, LBMpropagate!, hydroVariablesUpdate!) over liblbm_hip.so, whose
 C ABI is include/lbm_hip.h.
 """
 from ._capi import LBMError, LIB_PATH
-from .model import (DEFAULT, LBMmodel, ResolvedModel, addBead, addSquirmer, hydroVariablesUpdate,
+from .model import (DEFAULT, LBMbatch, LBMmodel, ResolvedModel, addBead, addSquirmer, hydroVariablesUpdate,
                     julia_range, LBMpropagate, modelInit, propagate_ticks, resolve_model,
                     resolve_particle, sphere_count)
 
 __all__ = [
-    "DEFAULT", "LBMError", "LBMmodel", "LIB_PATH", "ResolvedModel", "addBead", "addSquirmer",
+    "DEFAULT", "LBMError", "LBMbatch", "LBMmodel", "LIB_PATH", "ResolvedModel", "addBead", "addSquirmer",
     "hydroVariablesUpdate", "julia_range", "LBMpropagate", "modelInit", "propagate_ticks",
     "resolve_model", "resolve_particle", "sphere_count",
 ]

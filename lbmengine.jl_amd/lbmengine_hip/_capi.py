@@ -88,6 +88,16 @@ SIGNATURES = {
     "lbm_measure_copy": [C.c_int64, C.c_int32, _dp],
     "lbm_autotune": [_vp, C.c_int32, C.POINTER(C.c_int32), _dp],
     "lbm_set_graph": [_vp, C.c_int32],
+    "lbm_batch_create": [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)],
+    "lbm_batch_destroy": [_vp],
+    "lbm_batch_run": [_vp, C.c_int64],
+    "lbm_batch_sync": [_vp],
+    "lbm_batch_set_block": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "lbm_batch_get_block": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "lbm_batch_autotune": [_vp, C.c_int32, C.POINTER(C.c_int32), _dp],
+    "lbm_batch_set_graph": [_vp, C.c_int32],
+    "lbm_batch_set_timing": [_vp, C.c_int],
+    "lbm_batch_last_run_times": [_vp, _dp, _dp],
 }
 _RESTYPES = {"lbm_last_error": C.c_char_p, "lbm_version": C.c_char_p}
 

@@ -41,8 +41,9 @@ def build(verbose: bool = False) -> Path:
     OBJ_DIR.mkdir(exist_ok=True)
     jobs = []
     for store, tag in ((0, "f64"), (1, "f32"), (2, "f16")):
-        jobs.append([HIPCC, *COMMON, "-c", f"-DLBM_STORE={store}", str(CSRC / "lbm_kernels.hip"),
-                     "-o", str(OBJ_DIR / f"lbm_kernels_{tag}.o")])
+        for part, ptag in ((0, ""), (1, "_batch")):
+            jobs.append([HIPCC, *COMMON, "-c", f"-DLBM_STORE={store}", f"-DLBM_PART={part}",
+                         str(CSRC / "lbm_kernels.hip"), "-o", str(OBJ_DIR / f"lbm_kernels_{tag}{ptag}.o")])
     jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "lbm_particles.hip"), "-o", str(OBJ_DIR / "lbm_particles.o")])
     jobs.append([HIPCC, *COMMON, "-x", "hip", "-c", str(CSRC / "lbm_capi.cpp"), "-o", str(OBJ_DIR / "lbm_capi.o")])
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))

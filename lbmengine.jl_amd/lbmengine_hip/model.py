@@ -569,6 +569,64 @@ class LBMmodel:
         return k.value, s.value
 
 
+class LBMbatch:
+    """Independent models (one lattice, storage, collision and force kind;
+    any sizes, parameters and particles) advanced together: every kernel of a
+    run covers all of them in one launch.  Each model ends where
+    LBMpropagate(model, ticks=n) would leave it.  No counterpart in the
+    reference, where a parameter sweep loops LBMpropagate! over models
+    (gpu/fluids.jl:228-264)."""
+
+    def __init__(self, models):
+        self._fn = _capi.load()
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
+        b = C.c_void_p()
+        _capi.check(self._fn["lbm_batch_create"](arr, len(self.models), C.byref(b)))
+        self._b = b
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            try:
+                self._fn["lbm_batch_destroy"](b)
+            except Exception:
+                pass
+            self._b = None
+
+    def run(self, ticks: int) -> None:
+        _capi.check(self._fn["lbm_batch_run"](self._b, int(ticks)))
+
+    def sync(self) -> None:
+        _capi.check(self._fn["lbm_batch_sync"](self._b))
+
+    def set_block(self, bx: int, by: int, bz: int = 1) -> None:
+        _capi.check(self._fn["lbm_batch_set_block"](self._b, bx, by, bz))
+
+    def get_block(self):
+        b = [C.c_int32() for _ in range(3)]
+        _capi.check(self._fn["lbm_batch_get_block"](self._b, *[C.byref(x) for x in b]))
+        return tuple(x.value for x in b)
+
+    def autotune(self, reps: int = 10):
+        """Pick the fastest block shape of the batched stream-collide kernel."""
+        best = (C.c_int32 * 3)()
+        ms = C.c_double()
+        _capi.check(self._fn["lbm_batch_autotune"](self._b, int(reps), best, C.byref(ms)))
+        return tuple(best), ms.value
+
+    def set_graph(self, ticks_per_graph: int) -> None:
+        _capi.check(self._fn["lbm_batch_set_graph"](self._b, int(ticks_per_graph)))
+
+    def set_timing(self, on: bool) -> None:
+        _capi.check(self._fn["lbm_batch_set_timing"](self._b, int(bool(on))))
+
+    def last_run_times(self):
+        k, s = C.c_double(), C.c_double()
+        _capi.check(self._fn["lbm_batch_last_run_times"](self._b, C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+
 def modelInit(**kw) -> LBMmodel:
     """modelInit (gpu/initMethods.jl:440-702) on the hip backend."""
     return LBMmodel(resolve_model(**kw))
@@ -612,9 +670,21 @@ def propagate_ticks(dt: float, simulationTime=DEFAULT, ticks=DEFAULT) -> int:
     return int(ticks)
 
 
-def LBMpropagate(model: LBMmodel, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
+def LBMpropagate(model, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
                  ticksSaved=DEFAULT) -> None:
-    """LBMpropagate! (gpu/fluids.jl:228-264): `ticks` calls of tick!."""
+    """LBMpropagate! (gpu/fluids.jl:228-264): `ticks` calls of tick!.  Given
+    an LBMbatch, every model of it advances (simulationTime is then read in
+    the time step of the batch's first model, which all must share)."""
+    if isinstance(model, LBMbatch):
+        dts = {m.resolved.dt for m in model.models}
+        if simulationTime is not DEFAULT and len(dts) != 1:
+            raise ValueError("simulationTime needs one time step across the batch; give ticks instead")
+        n = propagate_ticks(model.models[0].resolved.dt, simulationTime, ticks)
+        model.run(n)
+        model.sync()
+        if verbose:
+            print(f"\r t = {round(model.models[0].time, 2)}")
+        return
     n = propagate_ticks(model.resolved.dt, simulationTime, ticks)
     model.run(n)
     model.sync()
