@@ -138,14 +138,30 @@ int32_t owned_extent(const lbm_handle *h, const DevParticle &p, int k) {
   return std::min(link_box_extent(p.radius, h->cfg.dx) + 2 * kBoxMargin, h->n[k]);
 }
 
+// Owned regions wider than this (3D) give their box blocks the whole region
+// rather than the band, whose row table would outgrow the blocks' LDS.
+constexpr int32_t kBandMaxExtent = 96;
+
+// Most nodes a particle's box blocks take: its band holds at most as many
+// nodes as the shell one node wider on either side has volume (each node's
+// unit cell lies inside it), and never more than its owned region.
+int64_t box_nodes_bound(const lbm_handle *h, const DevParticle &p) {
+  int64_t cube = 1;
+  for (int k = 0; k < h->dims; k++) cube *= owned_extent(h, p, k);
+  if (p.whole) return cube;
+  const double r = p.radius / h->cfg.dx;
+  const double ri = std::max(r - kBandIn - 1.0, 0.0), ro = r + kBandOut + 1.0;
+  const double v = h->dims == 3 ? 4.0 / 3.0 * M_PI * (ro * ro * ro - ri * ri * ri) : M_PI * (ro * ro - ri * ri);
+  return std::min(cube, (int64_t)std::ceil(v) + 1);
+}
+
 // Box blocks for blocks of `threads` threads: enough per particle to cover
-// the largest region it may own; partials for every box block.
+// the most nodes it may take; partials for every box block.
 int plan_boxes(lbm_handle *h, int64_t threads) {
   int32_t blocks = 0;
   for (size_t p = 0; p < h->parts_h.size(); p++) {
     h->first_block[p] = blocks;
-    int64_t vol = 1;
-    for (int k = 0; k < h->dims; k++) vol *= owned_extent(h, h->parts_h[p], k);
+    const int64_t vol = box_nodes_bound(h, h->parts_h[p]);
     blocks += (int32_t)((vol + threads - 1) / threads);
   }
   for (size_t p = h->parts_h.size(); p <= LBM_MAX_PARTICLES; p++) h->first_block[p] = blocks;
@@ -225,12 +241,21 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   return a;
 }
 
-// Dynamic LDS of a box block: the coordinate stage, then 2t link slots
-// (f*, value, key) and t rho.
+// Dynamic LDS of a box block: the coordinate stage, 2t link slots (f*,
+// value, key), t rho and node words, the plane offsets and the row offsets
+// of the largest band region.
 size_t box_shmem(const lbm_handle *h, const StepArgs &a) {
   const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
   const size_t t = (size_t)tile_threads(a.t);
-  return 3 * (size_t)a.sp.coord_stride * 8 + 5 * t * rsize + 2 * t * 4;
+  const size_t cs = (size_t)a.sp.coord_stride;
+  size_t rows = 0;
+  for (const DevParticle &p : h->parts_h)
+    if (!p.whole) {
+      size_t r = 1;
+      for (int k = 1; k < h->dims; k++) r *= (size_t)owned_extent(h, p, k);
+      rows = std::max(rows, r);
+    }
+  return 3 * cs * 8 + 5 * t * rsize + 2 * t * 4 + t * 4 + (cs + 1) * 4 + rows * 2;
 }
 
 // Launch shape of a step kernel of the given mode: in the gathering modes the
@@ -575,6 +600,10 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
     d.nW[k] = d.W[k];
     d.ne[k] = d.e[k];
   }
+  int32_t widest = 1;
+  for (int k = 0; k < h->dims; k++) widest = std::max(widest, owned_extent(h, d, k));
+  if (widest > 1023) return fail(LBM_EINVAL, "particle radius too large (owned region over 1023 nodes)");
+  d.whole = h->dims == 3 && widest > kBandMaxExtent;
   int32_t err = 0;
   link_box(d.Rg, d.radius, h->x0, h->cfg.dx, h->n, h->dims, d.lo, d.hi, &err);
   if (err)

@@ -439,7 +439,7 @@ __device__ __forceinline__ bool in_box(const DevParticle &P, int x, int y, int z
 
 // Range along axis k of the nodes the box blocks own for particle P (its
 // link box grown by kBoxMargin, clamped to the lattice).
-__device__ __forceinline__ void owned_range(const DevParticle &P, const BoxGeom &g, int k, int32_t &lo,
+__host__ __device__ inline void owned_range(const DevParticle &P, const BoxGeom &g, int k, int32_t &lo,
                                             int32_t &hi) {
   const int n = k == 0 ? g.nx : (k == 1 ? g.ny : g.nz);
   lo = P.lo[k] - kBoxMargin < 0 ? 0 : P.lo[k] - kBoxMargin;
@@ -455,6 +455,109 @@ __device__ __forceinline__ bool owned(const DevParticle &P, const BoxGeom &g, in
     if (c[k] < lo || c[k] > hi) return false;
   }
   return true;
+}
+
+// Band of nodes the box blocks take for a particle: the nodes of its owned
+// region whose distance from its centre as read at the kernel's start (in
+// node units) lies in [r - kBandIn, r + kBandOut]; the tiles take every other
+// node.  Link fluid nodes lie within [r, r + sqrt(2)) of their tick's centre
+// and a move is below half a node, so the band holds this kernel's links and
+// the nodes whose rho the next kernel's links read, with a quarter-node
+// margin either side.  Tiles and box blocks evaluate the same arithmetic on
+// the same record, so they agree on every node.  A particle whose record is
+// marked whole gives its box blocks its whole owned region instead (radii
+// whose band rows would not fit the blocks' LDS).
+constexpr double kBandIn = 1.25, kBandOut = 2.75;
+
+struct Band {
+  double c[3];         // centre in node units
+  double rin2, rout2;  // squared band radii in node units
+  int32_t lo[3], hi[3];  // owned region (inclusive)
+  int32_t whole;
+};
+
+__host__ __device__ inline void band_of(const DevParticle &P, const StepParticles &sp, const BoxGeom &g, Band &b) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 3; k++) {
+    b.c[k] = k < sp.dims ? (P.Rg[k] - sp.x0[k]) / sp.dx : 0.0;
+    owned_range(P, g, k, b.lo[k], b.hi[k]);
+  }
+  const double r = P.radius / sp.dx;
+  const double ri = r - kBandIn > 0.0 ? r - kBandIn : 0.0, ro = r + kBandOut;
+  b.rin2 = ri * ri;
+  b.rout2 = ro * ro;
+  b.whole = P.whole;
+}
+
+// The band's nodes on row (y, z): x in [a0, a0 + na) and [b0, b0 + nb).
+__host__ __device__ inline void band_row(const Band &b, int y, int z, int &a0, int &na, int &b0, int &nb) {
+#pragma clang fp contract(off)
+  a0 = b0 = 0;
+  na = nb = 0;
+  if (y < b.lo[1] || y > b.hi[1] || z < b.lo[2] || z > b.hi[2]) return;
+  if (b.whole) {
+    a0 = b.lo[0];
+    na = b.hi[0] - b.lo[0] + 1;
+    return;
+  }
+  const double dy = y - b.c[1], dz = z - b.c[2];
+  const double rho2 = dy * dy + dz * dz;
+  if (rho2 > b.rout2) return;
+  const double ao = sqrt(b.rout2 - rho2);
+  int lo = (int)ceil(b.c[0] - ao), hi = (int)floor(b.c[0] + ao);
+  lo = lo < b.lo[0] ? b.lo[0] : lo;
+  hi = hi > b.hi[0] ? b.hi[0] : hi;
+  if (rho2 < b.rin2) {
+    // minus the hole |x - c| < ai
+    const double ai = sqrt(b.rin2 - rho2);
+    const int l1 = (int)floor(b.c[0] - ai), h0 = (int)ceil(b.c[0] + ai);
+    a0 = lo;
+    na = (l1 < hi ? l1 : hi) - lo + 1;
+    b0 = h0 > lo ? h0 : lo;
+    nb = hi - b0 + 1;  // (l1 < c0 < h0: the two never overlap)
+  } else {
+    a0 = lo;
+    na = hi - lo + 1;
+  }
+  na = na > 0 ? na : 0;
+  nb = nb > 0 ? nb : 0;
+}
+
+__host__ __device__ inline bool in_band(const Band &b, int x, int y, int z) {
+  int a0, na, b0, nb;
+  band_row(b, y, z, a0, na, b0, nb);
+  return (x >= a0 && x < a0 + na) || (x >= b0 && x < b0 + nb);
+}
+
+// Node gi of a band region of ey x ez rows, from the tables: plane_pre[k] =
+// band nodes of planes below k (ez + 1 entries), row_off[k ey + j] = band
+// nodes of plane k's rows below j.  gi must be below plane_pre[ez].
+__host__ __device__ inline void band_node(const Band &b, const int32_t *plane_pre, const uint16_t *row_off, int ey,
+                                          int ez, int gi, int &x, int &y, int &z) {
+  int zl = 0, zh = ez - 1;  // last plane starting at or before gi
+  while (zl < zh) {
+    const int mid = (zl + zh + 1) >> 1;
+    if (plane_pre[mid] <= gi)
+      zl = mid;
+    else
+      zh = mid - 1;
+  }
+  const int off = gi - plane_pre[zl];
+  const uint16_t *ro = row_off + zl * ey;
+  int yl = 0, yh = ey - 1;  // last row starting at or before off
+  while (yl < yh) {
+    const int mid = (yl + yh + 1) >> 1;
+    if (ro[mid] <= off)
+      yl = mid;
+    else
+      yh = mid - 1;
+  }
+  const int k = off - ro[yl];
+  y = b.lo[1] + yl;
+  z = b.lo[2] + zl;
+  int a0, na, b0, nb;
+  band_row(b, y, z, a0, na, b0, nb);
+  x = k < na ? a0 + k : b0 + (k - na);
 }
 
 // Index of population q of node (x, row) in the row-blocked layout, row =
@@ -521,10 +624,10 @@ __device__ __forceinline__ typename Store<S>::T zero_pop(const CollParams &c, in
 }
 
 // -------------------------------------------------------------- boxes
-// The nodes around every particle (its link box grown by kBoxMargin, the
-// owned region) are handled by the first blocks of a gathering kernel's grid
-// (dispatched first, spread over all XCDs), while the other blocks do the
-// rest of the lattice.  Every box block first applies the move left pending
+// The nodes of every particle's band (see Band) are handled by the first
+// blocks of a gathering kernel's grid (dispatched first, spread over all
+// XCDs), while the other blocks do the rest of the lattice.  Every box block
+// first applies the move left pending
 // by the previous box kernel (moveParticles!, from that kernel's partials,
 // in a fixed order, so all blocks agree).  A box node then gathers like any
 // other, except at particle links: a fluid node x whose neighbour x - c_q is
@@ -535,8 +638,9 @@ __device__ __forceinline__ typename Store<S>::T zero_pop(const CollParams &c, in
 // the next box kernel (or the finalize kernel after a run) turns into the
 // move.
 //
-// Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's owned
-// region in x-fastest order (j = bid - first_block[pb], T threads per block).
+// Box block bid of particle pb covers nodes [j T, (j + 1) T) of pb's band
+// listed row by row in x-fastest order (j = bid - first_block[pb], T threads
+// per block); a node in the bands of several particles belongs to the first.
 template <int Q, int S, int COLL, int FM, int MODE>
 __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   using L = Lattice<Q>;
@@ -552,7 +656,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // regions this kernel's box blocks own (from the records as read); then
   // the previous box kernel's move, applied by every box block alike.
   __shared__ DevParticle prec[LBM_MAX_PARTICLES];
-  __shared__ int32_t cb_lo[LBM_MAX_PARTICLES][3], cb_hi[LBM_MAX_PARTICLES][3];
+  __shared__ Band bands[LBM_MAX_PARTICLES];
   {
     static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
     constexpr int kWords = sizeof(DevParticle) / 8;
@@ -561,25 +665,75 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     for (int w = tid; w < sp.np * kWords; w += nthreads) dst[w] = src[w];
   }
   __syncthreads();
-  if (tid < 3 * sp.np) {
-    const int p = tid / 3, k = tid % 3;
-    owned_range(prec[p], g, k, cb_lo[p][k], cb_hi[p][k]);
-  }
+  if (tid < sp.np) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
   apply_pending_moves(sp, prec);  // (synchronises the block)
 
   int pb = 0;
   while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
-  const int bl[3] = {cb_lo[pb][0], cb_lo[pb][1], cb_lo[pb][2]};
-  const int ex = cb_hi[pb][0] - bl[0] + 1, ey = cb_hi[pb][1] - bl[1] + 1, ez = cb_hi[pb][2] - bl[2] + 1;
-  const int base = (bid - sp.first_block[pb]) * nthreads;
-  const int li = base + tid;
-  const int x = bl[0] + li % ex, y = bl[1] + (li / ex) % ey, z = bl[2] + li / (ex * ey);
-  // a node owned for several particles belongs to the first
-  bool mine = li < ex * ey * ez;
-  for (int p = 0; p < pb; p++)
-    if (x >= cb_lo[p][0] && x <= cb_hi[p][0] && y >= cb_lo[p][1] && y <= cb_hi[p][1] && z >= cb_lo[p][2] &&
-        z <= cb_hi[p][2])
+  const Band &B = bands[pb];
+  const int bl[3] = {B.lo[0], B.lo[1], B.lo[2]};
+  const int ex = B.hi[0] - bl[0] + 1, ey = B.hi[1] - bl[1] + 1, ez = B.hi[2] - bl[2] + 1;
+
+  // Dynamic LDS: coordinate stage (3 axes of coord_stride doubles), link
+  // list (2 T slots of f*, value and key), rho and node of each lane, and the
+  // band's row offsets (coord_stride^2) and plane offsets (coord_stride + 1).
+  extern __shared__ double lds_dyn[];
+  const int cstride = sp.coord_stride;
+  const int cap = 2 * nthreads;
+  double *cst = lds_dyn;
+  R *fs_l = reinterpret_cast<R *>(cst + 3 * cstride);  // f*_{q̄}(x) of the slot
+  R *v_l = fs_l + cap;                               // streamed value of the slot so far
+  R *rho_l = v_l + cap;                              // rho(x) of each lane's node
+  uint32_t *pk_l = reinterpret_cast<uint32_t *>(rho_l + nthreads);
+  uint32_t *node_l = pk_l + cap;                     // lane's node - bl, 10 bits per axis
+  int32_t *plane_pre = reinterpret_cast<int32_t *>(node_l + nthreads);
+  uint16_t *row_off = reinterpret_cast<uint16_t *>(plane_pre + cstride + 1);
+
+  // Band nodes per row of pb's region, then offsets within each plane and of
+  // each plane (a whole region needs no table).
+  const int nrows = B.whole ? 0 : ey * ez;
+  for (int r = tid; r < nrows; r += nthreads) {
+    int a0, na, b0, nb;
+    band_row(B, bl[1] + r % ey, bl[2] + r / ey, a0, na, b0, nb);
+    row_off[r] = (uint16_t)(na + nb);
+  }
+  __syncthreads();
+  if (!B.whole && tid < ez) {
+    int sum = 0;
+    for (int j = 0; j < ey; j++) {
+      const int c = row_off[tid * ey + j];
+      row_off[tid * ey + j] = (uint16_t)sum;
+      sum += c;
+    }
+    plane_pre[tid + 1] = sum;
+  }
+  __syncthreads();
+  if (!B.whole && tid == 0) {
+    plane_pre[0] = 0;
+    for (int k = 1; k <= ez; k++) plane_pre[k] += plane_pre[k - 1];
+  }
+  __syncthreads();
+  // this lane's band node: plane, then row, then place in the row
+  const int gi = (bid - sp.first_block[pb]) * nthreads + tid;
+  int x = bl[0], y = bl[1], z = bl[2];
+  bool mine = gi < (B.whole ? ex * ey * ez : plane_pre[ez]);
+  if (mine && B.whole) {
+    x = bl[0] + gi % ex;
+    y = bl[1] + (gi / ex) % ey;
+    z = bl[2] + gi / (ex * ey);
+    for (int p = 0; p < pb; p++)
+      if (in_band(bands[p], x, y, z)) mine = false;
+  } else if (mine) {
+    band_node(B, plane_pre, row_off, ey, ez, gi, x, y, z);
+    // (a node off its region would mean inconsistent tables: never touch it)
+    if (x < bl[0] || x > B.hi[0]) {
+      x = bl[0];
       mine = false;
+    }
+    for (int p = 0; p < pb; p++)
+      if (in_band(bands[p], x, y, z)) mine = false;
+  }
+  node_l[tid] = (uint32_t)(x - bl[0]) | (uint32_t)(y - bl[1]) << 10 | (uint32_t)(z - bl[2]) << 20;
   // box block 0 hands the moved records to the next kernel
   if (bid == 0) {
     constexpr int kWords = sizeof(DevParticle) / 8;
@@ -591,9 +745,6 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
 
   // Coordinates of pb's owned region and one node around it, in LDS: axis k
   // from index bl_k - 1 (periodic) at cst[k][0].
-  extern __shared__ double lds_dyn[];
-  const int cstride = sp.coord_stride;
-  double *cst = lds_dyn;
   {
     const int len[3] = {ex + 2, ey + 2, D == 3 ? ez + 2 : 1};
     const int nn[3] = {g.nx, g.ny, g.nz};
@@ -681,13 +832,8 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // f*_{q̄}(x); the particles then add their Ladd terms in particle order
   // and take their momentum exchange from the running value.  The list holds
   // 2 T slots at a time; more are taken in further rounds.
-  const int cap = 2 * nthreads;
-  R *fs_l = reinterpret_cast<R *>(cst + 3 * cstride);  // f*_{q̄}(x) of the slot
-  R *v_l = fs_l + cap;                               // streamed value of the slot so far
-  R *rho_l = v_l + cap;                              // rho(x) of each lane's node
   // final values of the block's link slots, in list order
   R *linkval = static_cast<R *>(sp.linkval) + (int64_t)bid * nthreads * (Q - 1);
-  uint32_t *pk_l = reinterpret_cast<uint32_t *>(rho_l + nthreads);
   // pk: lane | q << 9 | (c + 1) << 14, 16, 18 | weight class << 20 | owners << 22
   __shared__ uint32_t wave_cnt[8];
   __shared__ DD red[8][6];
@@ -757,8 +903,9 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
         const uint32_t pk = e < m ? pk_l[e] : 0u;
         if ((pk >> (22 + p)) & 1) {
-        const int t = pk & 511, lt = base + t;
-        const int lx = lx0 + lt % ex, ly = ly0 + (lt / ex) % ey, lz = lz0 + lt / (ex * ey);
+        const int t = pk & 511;
+        const uint32_t nd = node_l[t];
+        const int lx = lx0 + (int)(nd & 1023), ly = ly0 + (int)((nd >> 10) & 1023), lz = lz0 + (int)(nd >> 20);
         const int c0 = (int)((pk >> 14) & 3) - 1, c1 = (int)((pk >> 16) & 3) - 1,
                   c2 = (int)((pk >> 18) & 3) - 1;
         const double wq = ((pk >> 20) & 3) == 1 ? L::w(1) : L::w(Q - 1);
@@ -907,7 +1054,11 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
     const bool tile_in = tpx * a.t.bx <= P.hi[0] + kBoxMargin && (tpx + 1) * a.t.bx > P.lo[0] - kBoxMargin &&
                          tpy * a.t.by <= P.hi[1] + kBoxMargin && (tpy + 1) * a.t.by > P.lo[1] - kBoxMargin &&
                          tpz * a.t.bz <= P.hi[2] + kBoxMargin && (tpz + 1) * a.t.bz > P.lo[2] - kBoxMargin;
-    if (tile_in && owned(P, g, x, y, z)) boxed = true;
+    if (tile_in) {
+      Band bp;
+      band_of(P, a.sp, g, bp);
+      if (in_band(bp, x, y, z)) boxed = true;
+    }
   }
   if (MODE != kCollide && boxed) return;  // the box blocks' node
   const int row = y + g.ny * z;
