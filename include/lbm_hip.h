@@ -243,6 +243,42 @@ int lbm_batch_set_graph(lbm_batch *b, int32_t ticks_per_graph);
 int lbm_batch_set_timing(lbm_batch *b, int on);
 int lbm_batch_last_run_times(lbm_batch *b, double *kernel_ms, double *step_ms);
 
+/* ------------------------------------------------------------ adjoint
+ * Gradient of a swimming-speed loss by the discrete adjoint of the step
+ * (D3Q19, GPU semantics; any storage, collision and force kind):
+ *
+ *   L = (mean over ticks T-window+1..T of |V_swimmer| - target_speed)^2
+ *       + force_penalty * sum F^2
+ *
+ * from the handle's current state over `ticks` ticks, with respect to every
+ * particle's B1 and B2 and the Guo force field.  The forward run saves the
+ * populations and particle records every `checkpoint_every` ticks; the
+ * reverse sweep recomputes each segment, keeping its states, and runs the
+ * adjoint tick kernel backwards through it.  The link set is a
+ * piecewise-constant function of the positions and carries no derivative.
+ * No counterpart in the reference (whose step is not differentiable:
+ * gpu/fluids.jl:132-226 runs on Float64 arrays in place). */
+typedef struct lbm_adjoint_config {
+  int64_t ticks;            /* forward ticks T                                   */
+  int64_t checkpoint_every; /* K: states saved every K ticks                     */
+  int64_t window;           /* the speed is averaged over the last `window` ticks */
+  int32_t particle;         /* id of the swimmer                                 */
+  int32_t reserved;         /* 0                                                 */
+  double target_speed;
+  double force_penalty;
+} lbm_adjoint_config;
+
+/* Runs the forward ticks (the handle ends ticks ahead, as after lbm_run) and
+ * the reverse sweep.  Outputs (each may be NULL): *loss; dB1, dB2 (one per
+ * particle, in id order); dF ((N, 3), Julia layout; requires a force field).
+ * Device memory: about (ticks/K + K + 2) population buffers. */
+int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, double *dB1, double *dB2,
+                    double *dF);
+
+/* Wall times (ms) of the last lbm_adjoint_run: forward, reverse (with the
+ * recomputation). */
+int lbm_adjoint_last_times(lbm_handle *h, double *forward_ms, double *reverse_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,12 +6,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "lbm_adjoint.hpp"
+#include "lbm_adjoint_move.hpp"
 #include "lbm_dispatch.hpp"
 
 using namespace lbm;
@@ -73,6 +77,7 @@ struct lbm_handle {
   hipEvent_t run_start = nullptr, run_stop = nullptr;
   int64_t timed_steps = 0;
   double step_kernel_ms = 0.0, step_ms = 0.0;
+  double adj_forward_ms = 0.0, adj_reverse_ms = 0.0;  // last lbm_adjoint_run
   // captured blocks of graph_k ticks, one per starting ping-pong buffer
   // graphs of graph_k fused kernels, per starting population and particle parity
   hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -1439,6 +1444,258 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
     best[2] = best_c.bz;
   }
   if (best_ms) *best_ms = best_t;
+  return LBM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ adjoint
+namespace {
+
+// Device buffers of one adjoint run, freed on every exit.
+struct AdjBuffers {
+  std::vector<void *> ptrs;
+  ~AdjBuffers() {
+    for (void *p : ptrs) hipFree(p);
+  }
+  template <class Tp>
+  hipError_t alloc(Tp **out, size_t bytes) {
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e == hipSuccess) ptrs.push_back(p);
+    *out = static_cast<Tp *>(p);
+    return e;
+  }
+};
+
+// Stream-ordered copies on the handle's stream, waited for.
+hipError_t copy_sync(lbm_handle *h, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return e;
+}
+
+// Indices of the particle state in the adjoint's vectors (lbm_adjoint_move.hpp).
+enum { kStR = 0, kStV = 3, kStW = 6, kStE = 9, kStNR = 12, kStNV = 15, kStNW = 18, kStNE = 21 };
+
+}  // namespace
+
+extern "C" {
+
+int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, double *dB1, double *dB2,
+                    double *dF) {
+  if (!h || !cfg) return fail(LBM_EINVAL, "null argument");
+  if (h->dims != 3 || h->Q != 19) return fail(LBM_EINVAL, "the adjoint is built for D3Q19");
+  if (h->cfg.semantics != LBM_SEMANTICS_GPU) return fail(LBM_EINVAL, "the adjoint follows the GPU semantics");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
+  const int np = (int)h->parts_h.size();
+  const int64_t T = cfg->ticks, K = cfg->checkpoint_every, Wn = cfg->window;
+  if (T < 1 || K < 1 || Wn < 1 || Wn > T) return fail(LBM_EINVAL, "need ticks >= 1, checkpoint_every >= 1, 1 <= window <= ticks");
+  if (cfg->particle < 0 || cfg->particle >= np) return fail(LBM_EINVAL, "bad swimmer id");
+  if (dF && h->cfg.force != LBM_FORCE_FIELD) return fail(LBM_EINVAL, "dF needs a model with a force field");
+  const int sw = cfg->particle;
+  const size_t N = (size_t)h->N;
+  const size_t pop_bytes = N * h->Q * elem_size(h->cfg.storage);
+  const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+  const int threads = 256;
+  const int64_t nblocks = ((int64_t)N + threads - 1) / threads;
+  const int64_t nck = (T + K - 1) / K, seg_len = std::min(K, T);
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+
+  AdjBuffers buf;
+  char *ck = nullptr, *seg = nullptr, *fin = nullptr;
+  void *lam[2] = {nullptr, nullptr};
+  double *gF = nullptr, *partial = nullptr, *sums_d = nullptr;
+  DevParticle *recs_d = nullptr;
+  if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess || buf.alloc(&seg, (size_t)seg_len * pop_bytes) != hipSuccess ||
+      buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
+      buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
+      buf.alloc(&partial, (size_t)nblocks * np * kAdjSums * sizeof(double)) != hipSuccess ||
+      buf.alloc(&sums_d, (size_t)np * kAdjSums * sizeof(double)) != hipSuccess ||
+      buf.alloc(&recs_d, LBM_MAX_PARTICLES * sizeof(DevParticle)) != hipSuccess)
+    return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
+  std::vector<std::vector<DevParticle>> ck_recs(nck, std::vector<DevParticle>(np));
+  std::vector<std::vector<DevParticle>> seg_recs(seg_len + 1, std::vector<DevParticle>(np));
+  std::vector<DevParticle> fin_recs(np);
+  auto save_recs = [&](std::vector<DevParticle> &dst) {
+    return copy_sync(h, dst.data(), parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToHost);
+  };
+  auto load_recs = [&](const std::vector<DevParticle> &src) {
+    return copy_sync(h, parts_of(h, h->pcur), src.data(), np * sizeof(DevParticle), hipMemcpyHostToDevice);
+  };
+  auto run = [&](int64_t n) -> int {
+    if (const int rc = lbm_run(h, n)) return rc;
+    return LBM_OK;
+  };
+  const auto t_start = std::chrono::steady_clock::now();
+
+  // ---- forward: checkpoints, and the swimmer's speed over the window
+  const int64_t ws = T - Wn;  // V_t for t in (ws, T]
+  std::vector<double> speeds;
+  std::vector<std::array<double, 3>> vel;
+  for (int64_t t = 0; t < T;) {
+    if (t % K == 0) {
+      HIP_TRY(hipMemcpyAsync(ck + (size_t)(t / K) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice, h->stream),
+              "save checkpoint");
+      HIP_TRY(save_recs(ck_recs[t / K]), "save checkpoint records");
+    }
+    if (t >= ws) {
+      if (const int rc = run(1)) return rc;
+      std::vector<DevParticle> r(np);
+      HIP_TRY(save_recs(r), "read particles");
+      std::array<double, 3> v{r[sw].V[0], r[sw].V[1], r[sw].V[2]};
+      vel.push_back(v);
+      speeds.push_back(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      t++;
+    } else {
+      const int64_t n = std::min((t / K + 1) * K, ws) - t;
+      if (const int rc = run(n)) return rc;
+      t += n;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (const int rc = check_particles(h)) return rc;
+  HIP_TRY(hipMemcpyAsync(fin, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice, h->stream), "save final state");
+  HIP_TRY(save_recs(fin_recs), "save final records");
+  const int64_t fin_tick = h->tick;
+  const double fin_time = h->time;
+  double mean = 0.0;
+  for (double s : speeds) mean += s;
+  mean /= (double)Wn;
+  double Fsq = 0.0;
+  std::vector<double> Fh;
+  if (h->cfg.force == LBM_FORCE_FIELD) {
+    Fh.resize(N * 3);
+    if (h->cfg.storage == LBM_STORAGE_F64) {
+      HIP_TRY(copy_sync(h, Fh.data(), h->F, N * 3 * sizeof(double), hipMemcpyDeviceToHost), "read force field");
+    } else {
+      std::vector<float> Ff(N * 3);
+      HIP_TRY(copy_sync(h, Ff.data(), h->F, N * 3 * sizeof(float), hipMemcpyDeviceToHost), "read force field");
+      for (size_t i = 0; i < Ff.size(); i++) Fh[i] = Ff[i];
+    }
+    for (double v : Fh) Fsq += v * v;
+  }
+  const double L = (mean - cfg->target_speed) * (mean - cfg->target_speed) + cfg->force_penalty * Fsq;
+  const auto t_fwd = std::chrono::steady_clock::now();
+
+  // ---- reverse
+  const double gscale = 2.0 * (mean - cfg->target_speed) / (double)Wn;
+  auto direct = [&](const DevParticle &p, double (&mu)[kPState]) {  // dL/dV_t of the speed term
+    const double s = std::sqrt(p.V[0] * p.V[0] + p.V[1] * p.V[1] + p.V[2] * p.V[2]);
+    if (s > 0.0)
+      for (int k = 0; k < 3; k++) mu[kStV + k] += gscale * p.V[k] / s;
+  };
+  std::vector<std::array<double, kPState>> mu(np);
+  for (auto &m : mu) m.fill(0.0);
+  {
+    double m[kPState];
+    for (int i = 0; i < kPState; i++) m[i] = 0.0;
+    direct(fin_recs[sw], m);
+    for (int i = 0; i < kPState; i++) mu[sw][i] = m[i];
+  }
+  std::vector<double> gB1(np, 0.0), gB2(np, 0.0);
+  int a = 0;
+  HIP_TRY(hipMemsetAsync(lam[a], 0, N * h->Q * rsize, h->stream), "clear adjoint");
+  HIP_TRY(hipMemsetAsync(gF, 0, N * 3 * sizeof(double), h->stream), "clear force gradient");
+  AdjArgs aa;
+  std::memset(&aa, 0, sizeof aa);
+  aa.gF = gF;
+  aa.Ffield = h->F;
+  aa.g = geom(h);
+  aa.c = coll(h);
+  aa.parts = recs_d;
+  for (int k = 0; k < 3; k++) aa.coords[k] = h->coords_d[k];
+  aa.partial = partial;
+  aa.np = np;
+  {
+    double coeff = h->cfg.dx * h->cfg.dx * h->cfg.dx / h->cfg.dt;
+    aa.coeff = coeff;
+  }
+  aa.ls = h->cfg.dx / h->cfg.dt;
+  aa.shell = 1.5 * h->cfg.dx;
+  std::vector<double> sums((size_t)np * kAdjSums);
+  const Launch l{dim3((unsigned)nblocks), dim3(threads), h->stream};
+  for (int64_t c = nck - 1; c >= 0; c--) {
+    const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
+    HIP_TRY(hipMemcpyAsync(h->f[h->cur], ck + (size_t)c * pop_bytes, pop_bytes, hipMemcpyDeviceToDevice, h->stream),
+            "restore checkpoint");
+    HIP_TRY(load_recs(ck_recs[c]), "restore checkpoint records");
+    for (int64_t k = t0; k < t1; k++) {
+      HIP_TRY(hipMemcpyAsync(seg + (size_t)(k - t0) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice,
+                             h->stream),
+              "keep segment state");
+      HIP_TRY(save_recs(seg_recs[k - t0]), "keep segment records");
+      if (const int rc = run(1)) return rc;
+    }
+    HIP_TRY(save_recs(seg_recs[t1 - t0]), "keep segment records");
+    for (int64_t k = t1 - 1; k >= t0; k--) {
+      const std::vector<DevParticle> &Pk = seg_recs[k - t0], &Pk1 = seg_recs[k - t0 + 1];
+      std::vector<std::array<double, kPState>> mu_in(np);
+      for (int p = 0; p < np; p++) {
+        double mo[kPState], mi[kPState], phi[3], tau[3];
+        for (int i = 0; i < kPState; i++) mo[i] = mu[p][i];
+        move_vjp(Pk[p], Pk1[p].force, Pk1[p].torque, h->cfg.dt, mo, mi, phi, tau);
+        for (int i = 0; i < kPState; i++) mu_in[p][i] = mi[i];
+        for (int k2 = 0; k2 < 3; k2++) {
+          aa.phi[p][k2] = phi[k2];
+          aa.tau[p][k2] = tau[k2];
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(recs_d, Pk.data(), np * sizeof(DevParticle), hipMemcpyHostToDevice, h->stream),
+              "upload records");
+      aa.S = seg + (size_t)(k - t0) * pop_bytes;
+      aa.lam_in = lam[a];
+      aa.lam_out = lam[a ^ 1];
+      hipError_t e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
+      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks, np, sums_d, h->stream);
+      if (e != hipSuccess) return hip_fail(e, "adjoint launch");
+      HIP_TRY(copy_sync(h, sums.data(), sums_d, sums.size() * sizeof(double), hipMemcpyDeviceToHost), "adjoint sums");
+      a ^= 1;
+      for (int p = 0; p < np; p++) {
+        const double *s = &sums[(size_t)p * kAdjSums];
+        gB1[p] += s[kAdjB1];
+        gB2[p] += s[kAdjB2];
+        for (int k2 = 0; k2 < 3; k2++) {
+          mu_in[p][kStNR + k2] += s[kAdjNR + k2];
+          mu_in[p][kStNV + k2] += s[kAdjNV + k2];
+          mu_in[p][kStNW + k2] += s[kAdjNW + k2];
+          mu_in[p][kStNE + k2] += s[kAdjNE + k2];
+          mu_in[p][kStR + k2] += s[kAdjR + k2];
+        }
+        mu[p] = mu_in[p];
+      }
+      if (k > ws) {  // V_k is one of the averaged speeds
+        double m[kPState];
+        for (int i = 0; i < kPState; i++) m[i] = 0.0;
+        direct(Pk[sw], m);
+        for (int i = 0; i < kPState; i++) mu[sw][i] += m[i];
+      }
+    }
+  }
+  // the state the forward run reached
+  HIP_TRY(hipMemcpyAsync(h->f[h->cur], fin, pop_bytes, hipMemcpyDeviceToDevice, h->stream), "restore final state");
+  HIP_TRY(load_recs(fin_recs), "restore final records");
+  h->tick = fin_tick;
+  h->time = fin_time;
+  if (dF) {
+    HIP_TRY(copy_sync(h, dF, gF, N * 3 * sizeof(double), hipMemcpyDeviceToHost), "read force gradient");
+    for (size_t i = 0; i < N * 3; i++) dF[i] += 2.0 * cfg->force_penalty * Fh[i];
+  }
+  if (loss) *loss = L;
+  for (int p = 0; p < np; p++) {
+    if (dB1) dB1[p] = gB1[p];
+    if (dB2) dB2[p] = gB2[p];
+  }
+  const auto t_end = std::chrono::steady_clock::now();
+  h->adj_forward_ms = std::chrono::duration<double, std::milli>(t_fwd - t_start).count();
+  h->adj_reverse_ms = std::chrono::duration<double, std::milli>(t_end - t_fwd).count();
+  return LBM_OK;
+}
+
+int lbm_adjoint_last_times(lbm_handle *h, double *forward_ms, double *reverse_ms) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (forward_ms) *forward_ms = h->adj_forward_ms;
+  if (reverse_ms) *reverse_ms = h->adj_reverse_ms;
   return LBM_OK;
 }
 

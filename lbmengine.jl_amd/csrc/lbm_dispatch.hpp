@@ -31,4 +31,8 @@ hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);
 hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
+struct AdjArgs;
+hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l);
+hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream);
+
 }  // namespace lbm

@@ -120,7 +120,7 @@ __host__ __device__ inline void link_box(const double Rg[3], double r, const dou
 // Largest link-box extent along one dimension for radius r.
 inline int32_t link_box_extent(double r, double dx) { return (int32_t)ceil(2.0 * r / dx) + 7; }
 
-__device__ inline bool any_nonzero(const double *v, int n) {
+__host__ __device__ inline bool any_nonzero(const double *v, int n) {
   for (int k = 0; k < n; k++)
     if (v[k] != 0.0) return true;
   return false;
@@ -129,7 +129,7 @@ __device__ inline bool any_nonzero(const double *v, int n) {
 // moveParticles!, gpu/particles.jl:7-92.  The half-kick branch
 // (model.tick == 0) is unreachable there because tick! increments the tick
 // first (gpu/fluids.jl:217), so only the full kick is restated.
-__device__ inline void move_gpu(DevParticle &p, const double F[3], const double T[3], int dims,
+__host__ __device__ inline void move_gpu(DevParticle &p, const double F[3], const double T[3], int dims,
                                 double dt, bool &moved, bool &find_nv) {
 #pragma clang fp contract(off)
   moved = false;

@@ -139,7 +139,7 @@ struct StepArgs {
 // operations, without FMA contraction, so the step reproduces the FP64
 // CPU restatement to the last bit wherever both sum in the same order.
 template <int Q, int COLL, int FM>
-__device__ __forceinline__ void collide_ref(const double (&f)[Q], const double *Fn,
+__host__ __device__ inline void collide_ref(const double (&f)[Q], const double *Fn,
                                             const CollParams &c, double (&fs)[Q],
                                             double &rho_out) {
 #pragma clang fp contract(off)
@@ -362,7 +362,7 @@ __device__ __forceinline__ bool solid_at(const double *const coords[3], const do
 // getNodeVelocity (gpu/aux.jl:229-263) or bulkVelocity
 // (cpu/particles.jl:20-43), from the node-velocity snapshot.
 template <int D>
-__device__ __forceinline__ void node_velocity(const double (&xc)[3], const DevParticle &p, int semantics,
+__host__ __device__ inline void node_velocity(const double (&xc)[3], const DevParticle &p, int semantics,
                                               double cutoff, double (&out)[3]) {
 #pragma clang fp contract(off)
   double r[3] = {0.0, 0.0, 0.0};

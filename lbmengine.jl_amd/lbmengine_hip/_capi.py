@@ -98,6 +98,8 @@ SIGNATURES = {
     "lbm_batch_set_graph": [_vp, C.c_int32],
     "lbm_batch_set_timing": [_vp, C.c_int],
     "lbm_batch_last_run_times": [_vp, _dp, _dp],
+    "lbm_adjoint_run": [_vp, _vp, _dp, _dp, _dp, _dp],
+    "lbm_adjoint_last_times": [_vp, _dp, _dp],
 }
 _RESTYPES = {"lbm_last_error": C.c_char_p, "lbm_version": C.c_char_p}
 

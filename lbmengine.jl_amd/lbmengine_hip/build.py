@@ -1,8 +1,9 @@
 """Build liblbm_hip.so for gfx950 with hipcc, in-tree.
 
-The library is compiled from lbmengine.jl_amd/csrc: the kernel translation
-unit once per population storage (FP64, FP32, FP16), the particle kernels,
-and the host runtime that implements include/lbm_hip.h.
+The library is compiled from lbmengine.jl_amd/csrc: the step kernels once
+per population storage (FP64, FP32, FP16) and part (one lattice, batch), the
+particle kernels, the adjoint kernels, and the host runtime that implements
+include/lbm_hip.h.
 """
 from __future__ import annotations
 
@@ -45,6 +46,7 @@ def build(verbose: bool = False) -> Path:
             jobs.append([HIPCC, *COMMON, "-c", f"-DLBM_STORE={store}", f"-DLBM_PART={part}",
                          str(CSRC / "lbm_kernels.hip"), "-o", str(OBJ_DIR / f"lbm_kernels_{tag}{ptag}.o")])
     jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "lbm_particles.hip"), "-o", str(OBJ_DIR / "lbm_particles.o")])
+    jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "lbm_adjoint.hip"), "-o", str(OBJ_DIR / "lbm_adjoint.o")])
     jobs.append([HIPCC, *COMMON, "-x", "hip", "-c", str(CSRC / "lbm_capi.cpp"), "-o", str(OBJ_DIR / "lbm_capi.o")])
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:

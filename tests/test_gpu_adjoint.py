@@ -1,0 +1,109 @@
+"""Discrete adjoint of the step against central finite differences of the
+forward step (FP64 runs of the same loss on perturbed parameters):
+
+    L = (mean |V| over the last `window` ticks - target)^2 + penalty * sum F^2
+
+with respect to the squirmer's B1, B2 and the Guo force field, along seeded
+directions.  Steps are relative (1e-6 of each parameter's scale); the link
+set is piecewise constant, and runs whose +/- perturbations change it
+would show as a failure here (none is expected at these step sizes)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KW3 = dict(dims=3, velocities="D3Q19", latticeParameter=1.0, relaxationTimeRatio=0.8, kineticParameter=3 / 16)
+
+
+def setup(n, r, seed, f_sigma=1e-7):
+    rng = np.random.default_rng(seed)
+    F = rng.normal(0.0, f_sigma, (n, n, n, 3))
+    e = rng.standard_normal(3)
+    pos = rng.uniform(-1.0, 1.0, 3)
+    return F, pos, e / np.linalg.norm(e)
+
+
+def make(gpu, storage, n, r, B1, B2, F, pos, e):
+    m = gpu.modelInit(**KW3, xlims=(-n // 2, n // 2 - 1), storage=storage, forceDensity=np.asfortranarray(F))
+    gpu.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2 * B1 / 3, beta=B2 / B1, swimmingDirection=e,
+                    coupleForces=True, coupleTorques=True)
+    return m
+
+
+def directional_checks(gpu, storage, n, r, ticks, K, window, seed, ndirs, target=1e-4, penalty=1e3):
+    from lbmengine_hip import adjoint
+    F, pos, e = setup(n, r, seed)
+    B1, B2 = 1e-3, 2e-3
+    m = make(gpu, storage, n, r, B1, B2, F, pos, e)
+    g = adjoint.swimming_loss_gradient(m, ticks=ticks, checkpoint_every=K, window=window, target_speed=target,
+                                       force_penalty=penalty)
+    assert m.tick == ticks
+    opts = dict(ticks=ticks, window=window, target_speed=target, force_penalty=penalty)
+    rng = np.random.default_rng(seed + 100)
+    h = 1e-6
+    out = []
+    for d in range(ndirs):
+        if d == 0:
+            dB1, dB2, dFd = 1.0, 0.0, np.zeros_like(F)
+        elif d == 1:
+            dB1, dB2, dFd = 0.0, 1.0, np.zeros_like(F)
+        else:
+            dB1, dB2, dFd = rng.standard_normal(), rng.standard_normal(), rng.standard_normal(F.shape)
+        # steps relative to each parameter's scale
+        sB1, sB2, sF = h * B1 * dB1, h * B2 * dB2, h * 1e-7 * dFd
+        Lp = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 + sB1, B2 + sB2, F + sF, pos, e),
+                                   force_field=F + sF, **opts)
+        Lm = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 - sB1, B2 - sB2, F - sF, pos, e),
+                                   force_field=F - sF, **opts)
+        fd = (Lp - Lm) / 2.0
+        ad = g.dB1[0] * sB1 + g.dB2[0] * sB2 + float(np.sum(g.dF * sF))
+        out.append((ad, fd))
+    return g, out
+
+
+def test_fp64_adjoint_against_finite_differences(gpu):
+    g, pairs = directional_checks(gpu, "f64", n=24, r=5.0, ticks=40, K=16, window=10, seed=3, ndirs=5)
+    assert np.isfinite(g.loss) and g.loss > 0
+    for ad, fd in pairs:
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+
+
+def test_fp32_adjoint_against_fp64_finite_differences(gpu):
+    g, pairs = directional_checks(gpu, "f32", n=40, r=7.0, ticks=300, K=64, window=100, seed=5, ndirs=4)
+    for ad, fd in pairs:
+        assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(gpu):
+    from lbmengine_hip import adjoint
+    F, pos, e = setup(20, 4.5, 9)
+    res = []
+    for K in (7, 30):
+        m = make(gpu, "f64", 20, 4.5, 1e-3, -1e-3, F, pos, e)
+        res.append(adjoint.swimming_loss_gradient(m, ticks=30, checkpoint_every=K, window=8))
+    a, b = res
+    assert a.loss == b.loss
+    assert np.array_equal(a.dB1, b.dB1) and np.array_equal(a.dB2, b.dB2)
+    assert np.array_equal(a.dF, b.dF)
+
+
+def test_autograd_wrapper_matches_the_adjoint(gpu):
+    import torch
+    from lbmengine_hip import adjoint
+    n, r = 20, 4.5
+    F, pos, e = setup(n, r, 11)
+
+    def make_model(B1, B2, Fa):
+        return make(gpu, "f64", n, r, float(B1[0]), float(B2[0]), Fa, pos, e)
+
+    B1 = torch.tensor([1e-3], dtype=torch.float64, requires_grad=True)
+    B2 = torch.tensor([5e-4], dtype=torch.float64, requires_grad=True)
+    Ft = torch.tensor(F, requires_grad=True)
+    L = adjoint.swimming_speed_loss(B1, B2, Ft, make_model, ticks=20, checkpoint_every=8, window=5,
+                                    force_penalty=10.0)
+    (3.0 * L).backward()
+    ref = adjoint.swimming_loss_gradient(make_model(B1.detach(), B2.detach(), F), ticks=20, checkpoint_every=8,
+                                         window=5, force_penalty=10.0)
+    assert L.item() == ref.loss
+    assert B1.grad.item() == 3.0 * ref.dB1[0] and B2.grad.item() == 3.0 * ref.dB2[0]
+    assert np.array_equal(Ft.grad.numpy(), 3.0 * ref.dF)
