@@ -19,6 +19,14 @@ periodic boxes, edges drawn from {48, 64, 96}, one squirmer each (radius
 ticks together (one launch per tick over every box, graph-replayed), with
 each box's velocity field and swimming speed against the same box run alone
 in FP64.
+
+The "adjoint" object measures the gradient workload: D3Q19 TRT 96^3
+periodic, one squirmer of radius 10 (B1 = 1e-3, beta = 0), a Guo force field
+drawn N(0, 1e-7), FP32 forward and adjoint over 2000 ticks with checkpoints
+every 100; loss = (mean speed over the last 500 ticks - 1e-4)^2 + sum F^2;
+gradients with respect to B1, B2 and the force field, checked against FP64
+central finite differences (steps 1e-6 of each parameter's scale) along 16
+seeded directions.
 """
 from __future__ import annotations
 
@@ -132,6 +140,66 @@ def batch_bench(lh, args):
     return res
 
 
+def adjoint_model(lh, storage, n, r, B1, B2, F, pos, e):
+    m = lh.modelInit(**workload_kwargs(storage, n), forceDensity=np.asfortranarray(F))
+    lh.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2 * B1 / 3, beta=B2 / B1, swimmingDirection=e,
+                   coupleForces=True, coupleTorques=True)
+    return m
+
+
+def adjoint_bench(lh, args):
+    """The gradient workload: adjoint time and the check against FP64
+    central differences."""
+    from lbmengine_hip import adjoint
+    n, r, ticks, K, window = 96, 10.0, args.adjoint_ticks, 100, min(500, args.adjoint_ticks)
+    rng = np.random.default_rng(args.seed + 5)
+    F = rng.normal(0.0, 1e-7, (n, n, n, 3))
+    lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
+    pos = rng.uniform(lo, hi, 3)
+    e = rng.standard_normal(3)
+    e /= np.linalg.norm(e)
+    B1, B2 = 1e-3, 0.0
+    opts = dict(ticks=ticks, window=window, target_speed=1e-4, force_penalty=1.0)
+    m = adjoint_model(lh, "f32", n, r, B1, B2, F, pos, e)
+    g = adjoint.swimming_loss_gradient(m, checkpoint_every=K, **opts)
+    del m
+    res = {"workload": f"D3Q19 TRT {n}^3 periodic, squirmer r={r:g} B1=1e-3 beta=0, Guo force field N(0,1e-7), "
+                       f"f32 forward and adjoint, {ticks} ticks, checkpoints every {K}, speed averaged over the "
+                       f"last {window}",
+           "loss": g.loss, "dL_dB1": float(g.dB1[0]), "dL_dB2": float(g.dB2[0]),
+           "dL_dF_norm": float(np.linalg.norm(g.dF)),
+           "forward_ms": round(g.forward_ms, 1), "reverse_ms": round(g.reverse_ms, 1),
+           "reverse_ms_per_tick": round(g.reverse_ms / ticks, 4)}
+    errs, steps, switched = [], [], 0
+    for d in range(args.adjoint_dirs):
+        d1, d2, dF = rng.standard_normal(), rng.standard_normal(), rng.standard_normal(F.shape)
+        h = 1e-6
+        while True:
+            s1, s2, sF = h * B1 * d1, h * B1 * d2, h * 1e-7 * dF
+            ta, tb = [], []
+            mp = adjoint_model(lh, "f64", n, r, B1 + s1, B2 + s2, F + sF, pos, e)
+            Lp = adjoint.swimming_loss(mp, force_field=F + sF, trajectory=ta, **opts)
+            Lm = adjoint.swimming_loss(adjoint_model(lh, "f64", n, r, B1 - s1, B2 - s2, F - sF, pos, e),
+                                       force_field=F - sF, trajectory=tb, **opts)
+            # a link switch between the two runs is a jump of the geometry,
+            # not part of the derivative: shrink the step until there is none
+            if adjoint.link_switches(mp, ta, tb) == 0 or h < 1e-10:
+                break
+            switched += 1
+            h /= 10.0
+        fd = (Lp - Lm) / 2.0
+        ad = g.dB1[0] * s1 + g.dB2[0] * s2 + float(np.sum(g.dF * sF))
+        errs.append(abs(ad - fd) / abs(fd))
+        steps.append(h)
+    if errs:
+        res["fd_check"] = {"directions": len(errs), "step": "1e-6 of each parameter's scale, divided by 10 "
+                                                             "while the two runs' link sets differ at some tick",
+                           "steps_shrunk": switched, "smallest_step": min(steps),
+                           "max_rel_err": max(errs), "median_rel_err": float(np.median(errs)),
+                           "rel_errs": [float(x) for x in errs]}
+    return res
+
+
 def kernel_times(m, steps):
     """Average device time of the step kernel and of a whole tick, from HIP
     events recorded on the kernel's stream around every step kernel."""
@@ -231,6 +299,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-ticks", type=int, default=2000)
+    ap.add_argument("--adjoint-dirs", type=int, default=16)
     ap.add_argument("--block", default="", help="bx,by,bz of the step kernel instead of autotuning")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of this script into the traffic file and exit")
@@ -335,6 +406,8 @@ def main():
 
     if not args.no_batch:
         out["batch"] = batch_bench(lh, args)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(lh, args)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)

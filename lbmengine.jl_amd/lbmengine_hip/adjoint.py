@@ -65,19 +65,55 @@ def swimming_loss_gradient(model, *, ticks: int, checkpoint_every: int = 100, wi
 
 
 def swimming_loss(model, *, ticks: int, window: int | None = None, particle: int = 0,
-                  target_speed: float = 1e-4, force_penalty: float = 0.0, force_field=None) -> float:
-    """The same loss by a plain forward run (for finite differences)."""
+                  target_speed: float = 1e-4, force_penalty: float = 0.0, force_field=None,
+                  trajectory: list | None = None) -> float:
+    """The same loss by a plain forward run (for finite differences).  With
+    `trajectory`, every particle's position before each tick is appended to
+    it (the geometry the tick used)."""
     w = ticks if window is None else window
-    model.run(ticks - w)
     speeds = []
-    for _ in range(w):
+    if trajectory is None:
+        model.run(ticks - w)
+        start = ticks - w
+    else:
+        start = 0
+    for t in range(start, ticks):
+        if trajectory is not None:
+            trajectory.append([np.array(p.position) for p in model.particles])
         model.run(1)
-        speeds.append(float(np.linalg.norm(model.particles[particle].velocity)))
+        if t >= ticks - w:
+            speeds.append(float(np.linalg.norm(model.particles[particle].velocity)))
     model.sync()
     L = (np.mean(speeds) - target_speed) ** 2
     if force_field is not None:
         L += force_penalty * float(np.sum(np.asarray(force_field) ** 2))
     return float(L)
+
+
+def link_switches(model, traj_a: list, traj_b: list) -> int:
+    """Ticks at which two runs' link sets differ: a node solid in one run's
+    sphere (getSphere's test on the tick's position) and not in the other's.
+    Finite differences across such a tick see the jump of the piecewise-
+    constant geometry, which the gradient does not contain."""
+    rm = model.resolved
+    coords = [np.asarray(c, dtype=np.float64) for c in rm.coords[: rm.dims]]
+    radii = [p.particleParams["radius"] for p in model.particles]
+    count = 0
+    for pa, pb in zip(traj_a, traj_b):
+        differ = False
+        for Ra, Rb, r in zip(pa, pb, radii):
+            r2 = r * r
+            # nodes within r + 2 of the first run's centre
+            sel = [np.nonzero(np.abs(c - Ra[k]) <= r + 2.0)[0] for k, c in enumerate(coords)]
+            grids = np.meshgrid(*[coords[k][sel[k]] for k in range(rm.dims)], indexing="ij")
+            da = sum((g + (-Ra[k])) ** 2 for k, g in enumerate(grids)) if rm.dims == 2 else \
+                ((grids[0] + (-Ra[0])) ** 2 + (grids[1] + (-Ra[1])) ** 2) + (grids[2] + (-Ra[2])) ** 2
+            db = sum((g + (-Rb[k])) ** 2 for k, g in enumerate(grids)) if rm.dims == 2 else \
+                ((grids[0] + (-Rb[0])) ** 2 + (grids[1] + (-Rb[1])) ** 2) + (grids[2] + (-Rb[2])) ** 2
+            if np.any((da < r2) != (db < r2)):
+                differ = True
+        count += int(differ)
+    return count
 
 
 def _autograd_function():

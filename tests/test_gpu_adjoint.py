@@ -4,9 +4,10 @@ forward step (FP64 runs of the same loss on perturbed parameters):
     L = (mean |V| over the last `window` ticks - target)^2 + penalty * sum F^2
 
 with respect to the squirmer's B1, B2 and the Guo force field, along seeded
-directions.  Steps are relative (1e-6 of each parameter's scale); the link
-set is piecewise constant, and runs whose +/- perturbations change it
-would show as a failure here (none is expected at these step sizes)."""
+directions.  Steps are relative (1e-6 of each parameter's scale).  The link
+set is piecewise constant in the positions: where the +/- runs' link sets
+differ at some tick (adjoint.link_switches), the difference quotient holds
+a jump the derivative does not, and the step is divided by 10."""
 import numpy as np
 import pytest
 
@@ -49,12 +50,20 @@ def directional_checks(gpu, storage, n, r, ticks, K, window, seed, ndirs, target
             dB1, dB2, dFd = 0.0, 1.0, np.zeros_like(F)
         else:
             dB1, dB2, dFd = rng.standard_normal(), rng.standard_normal(), rng.standard_normal(F.shape)
-        # steps relative to each parameter's scale
-        sB1, sB2, sF = h * B1 * dB1, h * B2 * dB2, h * 1e-7 * dFd
-        Lp = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 + sB1, B2 + sB2, F + sF, pos, e),
-                                   force_field=F + sF, **opts)
-        Lm = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 - sB1, B2 - sB2, F - sF, pos, e),
-                                   force_field=F - sF, **opts)
+        # steps relative to each parameter's scale, shrunk while the two
+        # runs' link sets differ at some tick (a jump, not a derivative)
+        hd = h
+        while True:
+            sB1, sB2, sF = hd * B1 * dB1, hd * B2 * dB2, hd * 1e-7 * dFd
+            ta, tb = [], []
+            mp = make(gpu, "f64", n, r, B1 + sB1, B2 + sB2, F + sF, pos, e)
+            Lp = adjoint.swimming_loss(mp, force_field=F + sF, trajectory=ta, **opts)
+            Lm = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 - sB1, B2 - sB2, F - sF, pos, e),
+                                       force_field=F - sF, trajectory=tb, **opts)
+            if adjoint.link_switches(mp, ta, tb) == 0:
+                break
+            hd /= 10.0
+            assert hd > 1e-10, "link switches at every step size"
         fd = (Lp - Lm) / 2.0
         ad = g.dB1[0] * sB1 + g.dB2[0] * sB2 + float(np.sum(g.dF * sF))
         out.append((ad, fd))
