@@ -163,6 +163,8 @@ def adjoint_bench(lh, args):
     m = adjoint_model(lh, "f32", n, r, B1, B2, F, pos, e)
     g = adjoint.swimming_loss_gradient(m, checkpoint_every=K, **opts)
     del m
+    # the gradient holds the starting populations fixed; so do the runs below
+    f0 = adjoint_model(lh, "f64", n, r, B1, B2, F, pos, e).distributions
     res = {"workload": f"D3Q19 TRT {n}^3 periodic, squirmer r={r:g} B1=1e-3 beta=0, Guo force field N(0,1e-7), "
                        f"f32 forward and adjoint, {ticks} ticks, checkpoints every {K}, speed averaged over the "
                        f"last {window}",
@@ -178,9 +180,11 @@ def adjoint_bench(lh, args):
             s1, s2, sF = h * B1 * d1, h * B1 * d2, h * 1e-7 * dF
             ta, tb = [], []
             mp = adjoint_model(lh, "f64", n, r, B1 + s1, B2 + s2, F + sF, pos, e)
+            mm = adjoint_model(lh, "f64", n, r, B1 - s1, B2 - s2, F - sF, pos, e)
+            mp.distributions = f0
+            mm.distributions = f0
             Lp = adjoint.swimming_loss(mp, force_field=F + sF, trajectory=ta, **opts)
-            Lm = adjoint.swimming_loss(adjoint_model(lh, "f64", n, r, B1 - s1, B2 - s2, F - sF, pos, e),
-                                       force_field=F - sF, trajectory=tb, **opts)
+            Lm = adjoint.swimming_loss(mm, force_field=F - sF, trajectory=tb, **opts)
             # a link switch between the two runs is a jump of the geometry,
             # not part of the derivative: shrink the step until there is none
             if adjoint.link_switches(mp, ta, tb) == 0 or h < 1e-10:

@@ -1,4 +1,5 @@
-"""ctypes driver of oracle/liblbm_oracle.so — TEST INFRASTRUCTURE.
+"""ctypes driver of oracle/liblbm_oracle.so (and of the dual-number
+restatement oracle/liblbm_oracle_dual.so) — TEST INFRASTRUCTURE.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
 module, as the checker.  It takes the same resolved model as the hip backend
@@ -187,4 +188,53 @@ def oracle_model(**kw) -> OracleModel:
     return OracleModel(resolve_model(**kw))
 
 
-__all__ = ["OracleModel", "oracle_model", "build_oracle", "DEFAULT", "propagate_ticks"]
+ORACLE_DUAL_LIB = ORACLE_DIR / "liblbm_oracle_dual.so"
+_DUAL = None
+
+
+def load_dual():
+    global _DUAL
+    if _DUAL is None:
+        if not ORACLE_DUAL_LIB.exists():
+            build_oracle()
+        _DUAL = C.CDLL(str(ORACLE_DUAL_LIB))
+        fn = _DUAL.lbmd_loss_jvp
+        dp = C.POINTER(C.c_double)
+        fn.argtypes = [C.POINTER(_capi.LbmConfig), dp, dp, C.c_int32, C.POINTER(_capi.LbmParticle), dp, dp, dp,
+                       C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double, dp, dp, dp]
+        fn.restype = C.c_int
+    return _DUAL
+
+
+def dual_loss_jvp(rm: ResolvedModel, particles, f0, *, ticks: int, window: int, F=None, dB1=None, dB2=None,
+                  dF=None, swimmer: int = 0, target: float = 1e-4, penalty: float = 0.0):
+    """Loss (mean swimmer speed over the last `window` ticks - target)^2 +
+    penalty sum F^2 from populations f0 and its derivative along (dB1, dB2,
+    dF), by the dual-number restatement of the oracle's tick.  Returns
+    (loss, dloss, speeds)."""
+    lib = load_dual()
+    cfg, keep = rm.config()
+    np_ = len(particles)
+    parr = (_capi.LbmParticle * np_)(*particles)
+    f0 = np.asfortranarray(f0, dtype=np.float64)
+
+    def opt(a, n):
+        if a is None:
+            return None
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        assert a.size == n
+        keep.append(a)
+        return _capi.as_dptr(a)
+
+    d = rm.dims
+    loss, dl = C.c_double(), C.c_double()
+    speeds = np.zeros(window)
+    rc = lib.lbmd_loss_jvp(C.byref(cfg), _capi.as_dptr(f0), opt(F, rm.N * d), np_, parr, opt(dB1, np_),
+                           opt(dB2, np_), opt(dF, rm.N * d), int(ticks), int(window), int(swimmer), float(target),
+                           float(penalty), C.byref(loss), C.byref(dl), _capi.as_dptr(speeds))
+    if rc != 0:
+        raise _capi.LBMError(rc, "lbmd_loss_jvp rejected its arguments")
+    return loss.value, dl.value, speeds
+
+
+__all__ = ["OracleModel", "oracle_model", "build_oracle", "dual_loss_jvp", "DEFAULT", "propagate_ticks"]

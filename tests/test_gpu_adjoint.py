@@ -4,7 +4,10 @@ forward step (FP64 runs of the same loss on perturbed parameters):
     L = (mean |V| over the last `window` ticks - target)^2 + penalty * sum F^2
 
 with respect to the squirmer's B1, B2 and the Guo force field, along seeded
-directions.  Steps are relative (1e-6 of each parameter's scale).  The link
+directions, and against the dual-number restatement of the oracle's tick.
+All hold the starting populations fixed (the model's initial state is an
+input; its Guo-shifted equilibrium is not differentiated).  Steps are
+relative (1e-6 of each parameter's scale).  The link
 set is piecewise constant in the positions: where the +/- runs' link sets
 differ at some tick (adjoint.link_switches), the difference quotient holds
 a jump the derivative does not, and the step is divided by 10."""
@@ -36,6 +39,8 @@ def directional_checks(gpu, storage, n, r, ticks, K, window, seed, ndirs, target
     F, pos, e = setup(n, r, seed)
     B1, B2 = 1e-3, 2e-3
     m = make(gpu, storage, n, r, B1, B2, F, pos, e)
+    # the gradient holds the starting populations fixed: so do the runs below
+    f0 = make(gpu, "f64", n, r, B1, B2, F, pos, e).distributions
     g = adjoint.swimming_loss_gradient(m, ticks=ticks, checkpoint_every=K, window=window, target_speed=target,
                                        force_penalty=penalty)
     assert m.tick == ticks
@@ -57,9 +62,11 @@ def directional_checks(gpu, storage, n, r, ticks, K, window, seed, ndirs, target
             sB1, sB2, sF = hd * B1 * dB1, hd * B2 * dB2, hd * 1e-7 * dFd
             ta, tb = [], []
             mp = make(gpu, "f64", n, r, B1 + sB1, B2 + sB2, F + sF, pos, e)
+            mm = make(gpu, "f64", n, r, B1 - sB1, B2 - sB2, F - sF, pos, e)
+            mp.distributions = f0
+            mm.distributions = f0
             Lp = adjoint.swimming_loss(mp, force_field=F + sF, trajectory=ta, **opts)
-            Lm = adjoint.swimming_loss(make(gpu, "f64", n, r, B1 - sB1, B2 - sB2, F - sF, pos, e),
-                                       force_field=F - sF, trajectory=tb, **opts)
+            Lm = adjoint.swimming_loss(mm, force_field=F - sF, trajectory=tb, **opts)
             if adjoint.link_switches(mp, ta, tb) == 0:
                 break
             hd /= 10.0
@@ -81,6 +88,33 @@ def test_fp32_adjoint_against_fp64_finite_differences(gpu):
     g, pairs = directional_checks(gpu, "f32", n=40, r=7.0, ticks=300, K=64, window=100, seed=5, ndirs=4)
     for ad, fd in pairs:
         assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+
+
+def test_adjoint_against_the_dual_number_oracle(gpu, oracle):
+    """FP64 adjoint against the forward-mode derivative of the oracle's tick
+    (dual numbers) from the same populations, along B1, B2, a force-field
+    direction and a mixed one."""
+    from lbmengine_hip import adjoint
+    from lbmengine_hip.model import resolve_particle
+    n, r, T, W, target, pen = 16, 3.6, 16, 6, 1e-4, 1e3
+    F, pos, e = setup(n, r, 21, f_sigma=1e-6)
+    B1, B2 = 1e-3, -2e-3
+    m = make(gpu, "f64", n, r, B1, B2, F, pos, e)
+    f0 = m.distributions
+    p = resolve_particle(m.resolved, kind="squirmer", radius=r, position=pos, swimmingSpeed=2 * B1 / 3,
+                         beta=B2 / B1, swimmingDirection=e, coupleForces=True, coupleTorques=True)
+    g = adjoint.swimming_loss_gradient(m, ticks=T, checkpoint_every=5, window=W, target_speed=target,
+                                       force_penalty=pen)
+    Fj = np.asfortranarray(F).reshape(-1, order="F")
+    gF = np.asfortranarray(g.dF).reshape(-1, order="F")
+    rng = np.random.default_rng(4)
+    dFd = rng.standard_normal(Fj.shape) * 1e-6
+    for a, b, dF in ((B1, 0.0, None), (0.0, B2, None), (0.0, 0.0, dFd), (0.4 * B1, -0.3 * B2, dFd)):
+        L, dl, _ = oracle.dual_loss_jvp(m.resolved, [p], f0, ticks=T, window=W, F=Fj, dB1=[a], dB2=[b], dF=dF,
+                                        target=target, penalty=pen)
+        ad = g.dB1[0] * a + g.dB2[0] * b + (0.0 if dF is None else float(np.dot(gF, dF)))
+        assert abs(L - g.loss) <= 1e-12 * L
+        assert abs(ad - dl) <= 1e-8 * abs(dl), (a, b, ad, dl)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient(gpu):
