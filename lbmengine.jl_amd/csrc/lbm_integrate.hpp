@@ -251,7 +251,18 @@ __device__ inline void move_record(DevParticle &P, const double tot[6], const St
 // partial_in, over the whole block: each component is reduced over the
 // previous kernel's nbox box blocks in a fixed order, so every block that
 // does this gets the same records.
-__device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs) {
+// The first partial a lane reduces in apply_pending_moves (component pk =
+// wave of box block b = lane), loaded before the records are in: the
+// partials' addresses depend on the kernel arguments alone.
+__device__ inline DD prefetch_partial(const StepParticles &sp) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int pk = tid >> 6, b = __lane_id();
+  if (pk >= 6 * sp.np || b >= sp.nbox) return DD{0.0, 0.0};
+  const double *src = sp.partial_in + (((int64_t)b * LBM_MAX_PARTICLES + pk / 6) * 6 + pk % 6) * 2;
+  return DD{src[0], src[1]};
+}
+
+__device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs, const DD *pre = nullptr) {
   __shared__ double tot[LBM_MAX_PARTICLES][6];
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
@@ -262,6 +273,10 @@ __device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle 
     if (!recs[p].pending) continue;  // uniform over the wave
     DD s{0.0, 0.0};
     for (int b = lane; b < sp.nbox; b += 64) {
+      if (pre && pk == wave && b == lane) {
+        s = dd_merge(s, *pre);
+        continue;
+      }
       const double *src = sp.partial_in + (((int64_t)b * LBM_MAX_PARTICLES + p) * 6 + k) * 2;
       s = dd_merge(s, DD{src[0], src[1]});
     }

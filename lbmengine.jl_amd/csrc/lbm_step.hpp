@@ -651,12 +651,17 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const StepParticles &sp = a.sp;
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  // the box blocks' chain of dependent loads is the kernel's critical path:
+  // let their waves issue ahead of the tiles' on a shared CU
+  __builtin_amdgcn_s_setprio(3);
 
   // The particle records of this tick, in LDS (read throughout), with the
   // regions this kernel's box blocks own (from the records as read); then
   // the previous box kernel's move, applied by every box block alike.
   __shared__ DevParticle prec[LBM_MAX_PARTICLES];
   __shared__ Band bands[LBM_MAX_PARTICLES];
+  // the records and the previous kernel's partials load together
+  const DD pre = prefetch_partial(sp);
   {
     static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
     constexpr int kWords = sizeof(DevParticle) / 8;
@@ -665,14 +670,41 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     for (int w = tid; w < sp.np * kWords; w += nthreads) dst[w] = src[w];
   }
   __syncthreads();
-  if (tid < sp.np) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
-  apply_pending_moves(sp, prec);  // (synchronises the block)
-
   int pb = 0;
   while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
+  // pb's owned region (from the records as read) and the coordinates to
+  // stage for it, loaded now so that they arrive during the move
+  int bl[3], ext[3];
+  for (int k = 0; k < 3; k++) {
+    int32_t lo, hi;
+    owned_range(prec[pb], g, k, lo, hi);
+    bl[k] = lo;
+    ext[k] = hi - lo + 1;
+  }
+  constexpr int kCoordRegs = 4;
+  double cval[kCoordRegs];
+  {
+    const int len[3] = {ext[0] + 2, ext[1] + 2, D == 3 ? ext[2] + 2 : 1};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    const int tot3 = len[0] + len[1] + len[2];
+#pragma unroll
+    for (int j = 0; j < kCoordRegs; j++) {
+      const int i = tid + j * nthreads;
+      cval[j] = 0.0;
+      if (i < tot3) {
+        const int k = i < len[0] ? 0 : (i < len[0] + len[1] ? 1 : 2);
+        const int ii = i - (k == 0 ? 0 : (k == 1 ? len[0] : len[0] + len[1]));
+        int gi = bl[k] - 1 + ii;
+        gi = gi < 0 ? gi + nn[k] : (gi >= nn[k] ? gi - nn[k] : gi);
+        cval[j] = sp.coords[k][gi];
+      }
+    }
+  }
+  if (tid < sp.np) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
+  apply_pending_moves(sp, prec, &pre);  // (synchronises the block)
+
   const Band &B = bands[pb];
-  const int bl[3] = {B.lo[0], B.lo[1], B.lo[2]};
-  const int ex = B.hi[0] - bl[0] + 1, ey = B.hi[1] - bl[1] + 1, ez = B.hi[2] - bl[2] + 1;
+  const int ex = ext[0], ey = ext[1], ez = ext[2];
 
   // Dynamic LDS: coordinate stage (3 axes of coord_stride doubles), link
   // list (2 T slots of f*, value and key), rho and node of each lane, and the
@@ -744,16 +776,29 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   }
 
   // Coordinates of pb's owned region and one node around it, in LDS: axis k
-  // from index bl_k - 1 (periodic) at cst[k][0].
+  // from index bl_k - 1 (periodic) at cst[k][0] (the loads above; a region
+  // too long for them loads the rest here).
   {
     const int len[3] = {ex + 2, ey + 2, D == 3 ? ez + 2 : 1};
     const int nn[3] = {g.nx, g.ny, g.nz};
-    for (int k = 0; k < D; k++)
-      for (int i = tid; i < len[k]; i += nthreads) {
-        int gi = bl[k] - 1 + i;
+    const int tot3 = len[0] + len[1] + len[2];
+    for (int i = tid; i < tot3; i += nthreads) {
+      const int k = i < len[0] ? 0 : (i < len[0] + len[1] ? 1 : 2);
+      const int ii = i - (k == 0 ? 0 : (k == 1 ? len[0] : len[0] + len[1]));
+      const int j = (i - tid) / nthreads;
+      double v;
+      if (j < kCoordRegs) {
+        v = cval[0];
+#pragma unroll
+        for (int jj = 1; jj < kCoordRegs; jj++)
+          if (j == jj) v = cval[jj];
+      } else {
+        int gi = bl[k] - 1 + ii;
         gi = gi < 0 ? gi + nn[k] : (gi >= nn[k] ? gi - nn[k] : gi);
-        cst[k * cstride + i] = sp.coords[k][gi];
+        v = sp.coords[k][gi];
       }
+      cst[k * cstride + ii] = v;
+    }
   }
   __syncthreads();
   // this kernel's partials are to be applied by the next one
@@ -860,11 +905,28 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     total += wave_cnt[w];
   }
   const int lx0 = bl[0], ly0 = bl[1], lz0 = bl[2];
-  // f*_{q̄}(x) of this node's link slots, all loads in flight at once
+  // The node's Q loads at once: f*_{q̄}(x) at its link slots (bounce-back),
+  // f*_q(x - c_q) elsewhere (halfway bounce-back off a wall layer, gpu/
+  // fluids.jl:165); the link slots go through the list below, the others
+  // wait in registers for the node's update.  (w_q = w_q̄, so either index
+  // decodes the bounced value.)
   R bbv[Q];
+  {
+    const GatherRows gr = gather_rows(g, Q, y, z);
+    const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
+    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
 #pragma unroll
-  for (int q = 0; q < Q; q++)
-    bbv[q] = ((own_any >> q) & 1) ? decode<S, Q>(fin[own_idx + (int64_t)L::opp[q] * g.nx], L::opp[q]) : R(0);
+    for (int q = 0; q < Q; q++) {
+      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const bool bb = ((own_any >> q) & 1) || (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) ||
+                      (cy < 0 && yh) || (cz > 0 && zl) || (cz < 0 && zh);
+      const int64_t src = bb ? own_idx + (int64_t)L::opp[q] * g.nx
+                             : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
+      bbv[q] = node ? decode<S, Q>(fin[src], q) : R(0);
+    }
+  }
   for (uint32_t c0 = 0; c0 < total; c0 += cap) {
     const int m = (int)min((uint32_t)cap, total - c0);
     uint32_t idx = off;
@@ -898,9 +960,12 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       const int passes = (m + nthreads - 1) / nthreads;
       for (int it = 0; it < passes; it++) {
         const int e = it * nthreads + tid;
+        // double-double sums: the result does not depend on how the slots
+        // fall into blocks and passes (any block shape gives the same bits)
         DD acc[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] = DD{0.0, 0.0};
+        auto add = [](DD &a, double x) { dd_add(a, x); };
         const uint32_t pk = e < m ? pk_l[e] : 0u;
         if ((pk >> (22 + p)) & 1) {
         const int t = pk & 511;
@@ -932,28 +997,28 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
 #pragma clang fp contract(off)
           // s = f*_{q̄}(x) + f_q(x) (gpu/fluids.jl:193); the loads are
           // -coeff * sum(s) * c_q and -coeff * cross(sum(s (x - R)), c_q),
-          // formed from these exact sums by the integrator
+          // formed from these sums by the integrator
           const double s = static_cast<double>(fs_l[e]) + static_cast<double>(v) + shift;
           const double X[3] = {crd(0, lx), crd(1, ly), D == 3 ? crd(2, lz) : 0.0};
           double pr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < D; k++) pr[k] = s * (X[k] + (-P.R[k]));
           if (P.couple_forces) {
-            if (c0) dd_add(acc[0], c0 > 0 ? s : -s);
-            if (c1) dd_add(acc[1], c1 > 0 ? s : -s);
-            if (D == 3 && c2) dd_add(acc[2], c2 > 0 ? s : -s);
+            if (c0) add(acc[0], c0 > 0 ? s : -s);
+            if (c1) add(acc[1], c1 > 0 ? s : -s);
+            if (D == 3 && c2) add(acc[2], c2 > 0 ? s : -s);
           }
           if (P.couple_torques) {
             if constexpr (D == 2) {
-              if (c1) dd_add(acc[3], c1 > 0 ? pr[0] : -pr[0]);
-              if (c0) dd_add(acc[3], c0 > 0 ? -pr[1] : pr[1]);
+              if (c1) add(acc[3], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) add(acc[3], c0 > 0 ? -pr[1] : pr[1]);
             } else {
-              if (c2) dd_add(acc[3], c2 > 0 ? pr[1] : -pr[1]);
-              if (c1) dd_add(acc[3], c1 > 0 ? -pr[2] : pr[2]);
-              if (c2) dd_add(acc[4], c2 > 0 ? -pr[0] : pr[0]);
-              if (c0) dd_add(acc[4], c0 > 0 ? pr[2] : -pr[2]);
-              if (c1) dd_add(acc[5], c1 > 0 ? pr[0] : -pr[0]);
-              if (c0) dd_add(acc[5], c0 > 0 ? -pr[1] : pr[1]);
+              if (c2) add(acc[3], c2 > 0 ? pr[1] : -pr[1]);
+              if (c1) add(acc[3], c1 > 0 ? -pr[2] : pr[2]);
+              if (c2) add(acc[4], c2 > 0 ? -pr[0] : pr[0]);
+              if (c0) add(acc[4], c0 > 0 ? pr[2] : -pr[2]);
+              if (c1) add(acc[5], c1 > 0 ? pr[0] : -pr[0]);
+              if (c0) add(acc[5], c0 > 0 ? -pr[1] : pr[1]);
             }
           }
         }
@@ -989,25 +1054,14 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // this node: S (gather only) or its collision P, and rho for the next tick's links
   if (node) {
     R fg[Q];
-    const GatherRows gr = gather_rows(g, Q, y, z);
-    const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
-    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
     uint32_t idx = off;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-      // x - c_q in a wall layer: halfway bounce-back (gpu/fluids.jl:165)
-      const bool bb = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
-                      (cz > 0 && zl) || (cz < 0 && zh);
       if ((own_any >> q) & 1) {
         fg[q] = total > (uint32_t)cap ? linkval[idx] : v_l[idx];  // particle link
         idx++;
       } else {
-        const int64_t src = bb ? own_idx + (int64_t)L::opp[q] * g.nx
-                               : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
-        fg[q] = decode<S, Q>(fin[src], q);
+        fg[q] = bbv[q];
       }
     }
     if constexpr (MODE == kGather) {
