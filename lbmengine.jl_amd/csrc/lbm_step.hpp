@@ -314,6 +314,143 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
   }
 }
 
+// FP16 storage: the same collision on the stored values h_q = d_q / s_q
+// (s_q = w_q / kF16Scale) directly: the moments are sums of the stored
+// halves (mixed-precision FMAs take them as they are), and each population
+// is updated as h_q + delta_q / s_q and rounded once to FP16.  The FMAs are
+// written out and contraction is off, so the result is the same bits
+// wherever the routine is inlined.
+template <int Q, int COLL, int FM>
+__device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float *Fn, const CollParams &c,
+                                            _Float16 (&hs)[Q], float &rho_out) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  constexpr float s0 = static_cast<float>(L::w(0) / kF16Scale);
+  constexpr float s1 = static_cast<float>(L::w(1) / kF16Scale);
+  constexpr float s2 = static_cast<float>(L::w(Q - 1) / kF16Scale);
+  constexpr float is0 = static_cast<float>(kF16Scale * L::invw(0));
+  constexpr float is1 = static_cast<float>(kF16Scale * L::invw(1));
+  constexpr float is2 = static_cast<float>(kF16Scale * L::invw(Q - 1));
+  const float ls = static_cast<float>(c.ls);
+  const float invC2s = static_cast<float>(c.invC2s);
+  const float invC4s = static_cast<float>(c.invC4s);
+  // class sums of the stored values and of their first moments
+  float S1 = 0.0f, S2 = 0.0f, A1[3] = {0.0f, 0.0f, 0.0f}, A2[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int q = 1; q < Q; q++) {
+    const float v = static_cast<float>(h[q]);
+    const int cq[3] = {L::cx[q], L::cy[q], L::cz[q]};
+    if (L::wclass[q] == 1) {
+      S1 = fmaf(v, 1.0f, S1);
+#pragma unroll
+      for (int k = 0; k < D; k++)
+        if (cq[k]) A1[k] = fmaf(v, cq[k] > 0 ? 1.0f : -1.0f, A1[k]);
+    } else {
+      S2 = fmaf(v, 1.0f, S2);
+#pragma unroll
+      for (int k = 0; k < D; k++)
+        if (cq[k]) A2[k] = fmaf(v, cq[k] > 0 ? 1.0f : -1.0f, A2[k]);
+    }
+  }
+  const float h0 = static_cast<float>(h[0]);
+  const float drho = fmaf(s2, S2, fmaf(s1, S1, s0 * h0));
+  float j[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < D; k++) j[k] = fmaf(s2, A2[k], s1 * A1[k]);
+  float F[3] = {0.0f, 0.0f, 0.0f};
+  if constexpr (FM != LBM_FORCE_NONE) {
+    const float half_dt = static_cast<float>(0.5 * c.dt);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      F[k] = Fn[k];
+      j[k] = fmaf(half_dt, F[k], j[k]);
+    }
+  }
+  const float rho = static_cast<float>(c.rho0) + drho;
+  const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
+  float u[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < D; k++) u[k] = j[k] * inv;
+  rho_out = rho;
+  float uu = 0.0f;
+#pragma unroll
+  for (int k = 0; k < D; k++) uu = fmaf(u[k], u[k], uu);
+  const float dt = static_cast<float>(c.dt);
+  const float wp = static_cast<float>(c.dt / c.tau);
+  const float wm = COLL == LBM_COLLISION_TRT ? static_cast<float>(c.dt / c.tau_minus) : wp;
+  const float gp = 1.0f - 0.5f * wp, gm = 1.0f - 0.5f * wm;
+  float uF = 0.0f;
+  if constexpr (FM != LBM_FORCE_NONE) {
+#pragma unroll
+    for (int k = 0; k < D; k++) uF = fmaf(u[k], F[k], uF);
+  }
+  const float base = (-0.5f * invC2s) * uu;  // -u.u/(2 c_s^2)
+  {
+    // rest population (self-conjugate)
+    const float w0 = static_cast<float>(L::w(0));
+    const float dfe = w0 * fmaf(rho, base, drho);
+    float S = 0.0f;
+    if constexpr (FM != LBM_FORCE_NONE) S = (w0 * dt * gp) * (-uF * invC2s);
+    const float delta = fmaf(-wp, fmaf(s0, h0, -dfe), S);
+    hs[0] = static_cast<_Float16>(fmaf(delta, is0, h0));
+  }
+  // conjugate pairs (q, q̄), q < q̄: symmetric and antisymmetric parts
+#pragma unroll
+  for (int q = 1; q < Q; q++) {
+    const int o = L::opp[q];
+    if (o < q) continue;
+    const bool c1 = L::wclass[q] == 1;
+    const float w = static_cast<float>(L::w(q));
+    const float sq = c1 ? s1 : s2, isq = c1 ? is1 : is2;
+    float cu = 0.0f;
+    if (L::cx[q]) cu = L::cx[q] > 0 ? cu + u[0] : cu - u[0];
+    if (L::cy[q]) cu = L::cy[q] > 0 ? cu + u[1] : cu - u[1];
+    if (D == 3 && L::cz[q]) cu = L::cz[q] > 0 ? cu + u[2] : cu - u[2];
+    cu = ls * cu;
+    const float esym = w * fmaf(rho, fmaf(0.5f * invC4s, cu * cu, base), drho);
+    const float eanti = (w * rho * invC2s) * cu;
+    float Ssym = 0.0f, Santi = 0.0f;
+    if constexpr (FM != LBM_FORCE_NONE) {
+      float cF = 0.0f;
+      if (L::cx[q]) cF = L::cx[q] > 0 ? cF + F[0] : cF - F[0];
+      if (L::cy[q]) cF = L::cy[q] > 0 ? cF + F[1] : cF - F[1];
+      if (D == 3 && L::cz[q]) cF = L::cz[q] > 0 ? cF + F[2] : cF - F[2];
+      cF = ls * cF;
+      Ssym = (w * dt * gp) * fmaf(cu * cF, invC4s, -uF * invC2s);
+      Santi = (w * dt * gm) * (cF * invC2s);
+    }
+    const float hq = static_cast<float>(h[q]), ho = static_cast<float>(h[o]);
+    const float psym = fmaf(0.5f * sq, hq + ho, -esym);
+    const float panti = fmaf(0.5f * sq, hq - ho, -eanti);
+    const float dsym = fmaf(-wp, psym, Ssym);
+    const float danti = fmaf(-wm, panti, Santi);
+    hs[q] = static_cast<_Float16>(fmaf(dsym + danti, isq, hq));
+    hs[o] = static_cast<_Float16>(fmaf(dsym - danti, isq, ho));
+  }
+}
+
+// Collision from stored values to stored values (the tiles' path).
+template <int Q, int S, int COLL, int FM>
+__device__ __forceinline__ void collide_stored(const typename Store<S>::T (&raw)[Q], const typename Store<S>::R *Fn,
+                                               const CollParams &c, typename Store<S>::T (&out)[Q],
+                                               typename Store<S>::R &rho) {
+  using R = typename Store<S>::R;
+  if constexpr (S == kF16) {
+    collide_h16<Q, COLL, FM>(raw, Fn, c, out, rho);
+  } else {
+    R f[Q], fs[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(raw[q], q);
+    if constexpr (S == kF64)
+      collide_ref<Q, COLL, FM>(f, Fn, c, fs, rho);
+    else
+      collide_dev<Q, COLL, FM>(f, Fn, c, fs, rho);
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = encode<S, Q>(fs[q], q);
+  }
+}
+
 // ------------------------------------------------------- node geometry
 struct NodeIdx {
   int x, y, z;
@@ -591,14 +728,12 @@ __device__ __forceinline__ GatherRows gather_rows(const BoxGeom &g, int Q, int y
 enum StepMode { kCollide = 0, kFused = 1, kGather = 2 };
 
 // Collision of a node's populations f (S), into fs (P), with rho.
-template <int Q, int S, int COLL, int FM>
-__device__ __forceinline__ void collide_node(const StepArgs &a, int64_t n,
-                                             const typename Store<S>::R (&f)[Q],
-                                             typename Store<S>::R (&fs)[Q], typename Store<S>::R &rho) {
-  using L = Lattice<Q>;
+// The Guo force at node n (zero without forcing).
+template <int Q, int S, int FM>
+__device__ __forceinline__ void node_force(const StepArgs &a, int64_t n, typename Store<S>::R (&Fn)[3]) {
   using R = typename Store<S>::R;
-  constexpr int D = L::D;
-  R Fn[3] = {0, 0, 0};
+  constexpr int D = Lattice<Q>::D;
+  Fn[0] = Fn[1] = Fn[2] = R(0);
   if constexpr (FM == LBM_FORCE_UNIFORM) {
 #pragma unroll
     for (int k = 0; k < D; k++) Fn[k] = static_cast<R>(a.c.F[k]);
@@ -607,6 +742,15 @@ __device__ __forceinline__ void collide_node(const StepArgs &a, int64_t n,
 #pragma unroll
     for (int k = 0; k < D; k++) Fn[k] = Ff[k * a.g.N + n];
   }
+}
+
+template <int Q, int S, int COLL, int FM>
+__device__ __forceinline__ void collide_node(const StepArgs &a, int64_t n,
+                                             const typename Store<S>::R (&f)[Q],
+                                             typename Store<S>::R (&fs)[Q], typename Store<S>::R &rho) {
+  using R = typename Store<S>::R;
+  R Fn[3];
+  node_force<Q, S, FM>(a, n, Fn);
   if constexpr (S == kF64)
     collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
   else
@@ -1102,18 +1246,59 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   const int y = tpy * a.t.by + threadIdx.y;
   const int z = tpz * a.t.bz + threadIdx.z;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  bool boxed = false;  // owned by the box blocks
-  for (int p = 0; p < a.sp.np; p++) {
-    const DevParticle &P = a.sp.parts_in[p];
-    const bool tile_in = tpx * a.t.bx <= P.hi[0] + kBoxMargin && (tpx + 1) * a.t.bx > P.lo[0] - kBoxMargin &&
-                         tpy * a.t.by <= P.hi[1] + kBoxMargin && (tpy + 1) * a.t.by > P.lo[1] - kBoxMargin &&
-                         tpz * a.t.bz <= P.hi[2] + kBoxMargin && (tpz + 1) * a.t.bz > P.lo[2] - kBoxMargin;
-    if (tile_in) {
-      Band bp;
-      band_of(P, a.sp, g, bp);
-      if (in_band(bp, x, y, z)) boxed = true;
+  // whether this tile overlaps particle P's owned region: the six bounds
+  // in one go, compared without short-circuits (which would make six
+  // dependent loads of them)
+  auto tile_touches = [&](const DevParticle &P) {
+    const int32_t lo0 = P.lo[0], lo1 = P.lo[1], lo2 = P.lo[2], hi0 = P.hi[0], hi1 = P.hi[1], hi2 = P.hi[2];
+    return (tpx * a.t.bx <= hi0 + kBoxMargin) & ((tpx + 1) * a.t.bx > lo0 - kBoxMargin) &
+           (tpy * a.t.by <= hi1 + kBoxMargin) & ((tpy + 1) * a.t.by > lo1 - kBoxMargin) &
+           (tpz * a.t.bz <= hi2 + kBoxMargin) & ((tpz + 1) * a.t.bz > lo2 - kBoxMargin);
+  };
+  // a band node of some particle belongs to the box blocks
+  auto band_node_of_any = [&]() {
+    bool in = false;
+    for (int p = 0; p < a.sp.np; p++) {
+      const DevParticle &P = a.sp.parts_in[p];
+      if (tile_touches(P)) {
+        Band bp;
+        band_of(P, a.sp, g, bp);
+        if (in_band(bp, x, y, z)) in = true;
+      }
+    }
+    return in;
+  };
+  if constexpr (MODE == kFused) {
+    if (!g.walled) {
+      // The run's repeated kernel, no walls: the node's Q gathers go out
+      // first and the particles' regions (scalar, one load each) are
+      // checked while they are in flight.  A tile that touches none of
+      // them finishes here; one that does takes the general path below.
+      const int row = y + g.ny * z;
+      const int64_t own = (int64_t)row * Q * g.nx + x;
+      const T *__restrict__ fin = static_cast<const T *>(a.fin);
+      T *__restrict__ fout = static_cast<T *>(a.fout);
+      const GatherRows gr = gather_rows(g, Q, y, z);
+      const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
+      T raw[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        raw[q] = fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]];
+      bool near = false;
+      for (int p = 0; p < a.sp.np; p++) near = near | tile_touches(a.sp.parts_in[p]);
+      if (!near) {
+        R Fn[3], rho;
+        node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)row, Fn);
+        T out[Q];
+        collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
+#pragma unroll
+        for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = out[q];
+        return;
+      }
     }
   }
+  bool boxed = false;  // owned by the box blocks
+  boxed = band_node_of_any();
   if (MODE != kCollide && boxed) return;  // the box blocks' node
   const int row = y + g.ny * z;
   const int64_t own = (int64_t)row * Q * g.nx + x;
@@ -1127,18 +1312,17 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
     return;
   }
 
-  R f[Q];
+  T raw[Q];
   if constexpr (MODE == kCollide) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(fin[own + (int64_t)q * g.nx], q);
+    for (int q = 0; q < Q; q++) raw[q] = fin[own + (int64_t)q * g.nx];
   } else {
     // gather f_q(x) = f*_q(x - c_q)
     const GatherRows gr = gather_rows(g, Q, y, z);
     const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
     if (!g.walled) {
 #pragma unroll
-      for (int q = 0; q < Q; q++)
-        f[q] = decode<S, Q>(fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]], q);
+      for (int q = 0; q < Q; q++) raw[q] = fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]];
     } else {
       const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
       const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
@@ -1152,19 +1336,21 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
                         (cz > 0 && zl) || (cz < 0 && zh);
         const int64_t src = bb ? own + (int64_t)L::opp[q] * g.nx
                                : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
-        f[q] = decode<S, Q>(fin[src], q);
+        raw[q] = fin[src];
       }
     }
   }
 
   if constexpr (MODE == kGather) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = encode<S, Q>(f[q], q);
+    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = raw[q];
   } else {
-    R fs[Q], rho;
-    collide_node<Q, S, COLL, FM>(a, (int64_t)x + g.nx * (int64_t)row, f, fs, rho);
+    R Fn[3], rho;
+    node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)row, Fn);
+    T out[Q];
+    collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
 #pragma unroll
-    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = encode<S, Q>(fs[q], q);
+    for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = out[q];
     // rho of the links' next gather (only the collide-only kernel reaches box nodes)
     if (MODE == kCollide && boxed) static_cast<R *>(a.rho_side)[(int64_t)x + g.nx * (int64_t)row] = rho;
   }
