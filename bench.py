@@ -44,6 +44,8 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 
 METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
 HBM_PEAK_GBS = 8000.0
+# the short run the profiles are taken on (no variants, batch, adjoint or CPU leg)
+PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --block 192,1,2"
 BYTES = {"f64": 8, "f32": 4, "f16": 2}
 
 
@@ -256,9 +258,9 @@ def traffic_summary(fetch_dir, write_dir, storage, n):
             "dispatches": {"step_fetch": fc[("step_kernel", "FETCH_SIZE")], "step_write": wc[("step_kernel", "WRITE_SIZE")]},
             "made_by": [
                 f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {fetch_dir} -o step "
-                f"--output-format csv -- python3 bench.py --steps 20 --warmup 5 --no-cpu --no-variants --block 192,1,1",
+                f"--output-format csv -- python3 bench.py {PROFILE_ARGS}",
                 f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {write_dir} -o step "
-                f"--output-format csv -- python3 bench.py --steps 20 --warmup 5 --no-cpu --no-variants --block 192,1,1",
+                f"--output-format csv -- python3 bench.py {PROFILE_ARGS}",
                 f"python3 bench.py --traffic-from {fetch_dir} {write_dir}"]}
 
 
