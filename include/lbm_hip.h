@@ -142,6 +142,61 @@ int lbm_get_particle(lbm_handle *h, int32_t id, lbm_particle *out);
 int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3],
                           double torque[3]);
 
+/* ---- particle-particle interactions (molecularDynamicsTick!,
+ * gpu/particles.jl:100-197) ----
+ * model.particleInteractions with each pair list expanded, in the order
+ * the reference applies them: every term adds to forceInput from the
+ * positions before the tick's moves, then every particle moves. */
+#define LBM_INTERACTION_LINEAR 0        /* addLinearBond!    gpu/initMethods.jl:229-245, particles.jl:110-121 */
+#define LBM_INTERACTION_BISTABLE 1      /* addBistableBond!  gpu/initMethods.jl:282-308, particles.jl:123-134 */
+#define LBM_INTERACTION_POLAR 2         /* addPolarBond!     gpu/initMethods.jl:247-273, particles.jl:136-158 */
+#define LBM_INTERACTION_DIPOLE 3        /* addDipoles!, one pair   gpu/initMethods.jl:316-359, particles.jl:160-177 */
+#define LBM_INTERACTION_LENNARD_JONES 4 /* addLennardJones!/addWCA!, one pair  gpu/initMethods.jl:361-414, particles.jl:179-197 */
+
+#define LBM_MAX_INTERACTIONS 256
+
+typedef struct lbm_interaction {
+  int32_t kind;      /* LBM_INTERACTION_*                                   */
+  int32_t ids[3];    /* 0-based particle indices id1, id2 (and id3, POLAR)   */
+  double params[4];  /* LINEAR: equilibriumDisplacement, stiffness
+                        BISTABLE: trapRadius, a, b
+                        POLAR: equilibriumAngle, stiffness
+                        DIPOLE: dipoleConstant, scalar-field flag (1: B is
+                                the number field[0], 2D)
+                        LENNARD_JONES: epsilon, sigma, cutoff             */
+  double field[3];   /* DIPOLE: the magnetic field B (constant in time)     */
+} lbm_interaction;
+
+/* Replaces the handle's interaction terms with terms[0..n) (n = 0 clears
+ * them).  GPU semantics only, as the CPU backend has no interactions. */
+int lbm_set_interactions(lbm_handle *h, const lbm_interaction *terms, int32_t n);
+
+/* ---- per-tick particle log (saveData's particlesTrj.csv,
+ * gpu/fluids.jl:252-259, gpu/aux.jl:319-362) ----
+ * With capacity > 0 every tick's particle move also writes each particle's
+ * position, velocity and angular velocity after the move into a device
+ * ring of `capacity` ticks; 0 turns the log off and frees the ring.  Ticks
+ * run before the log was on are not logged. */
+int lbm_set_trajectory_log(lbm_handle *h, int64_t capacity);
+
+/* Logged states of particle id for ticks first .. first + count - 1, as
+ * out[count][9] = (position[3], velocity[3], angular velocity[3]) with
+ * unused components 0.  Every tick must be one the ring still holds: later
+ * than the particle's addition and than the log's start, at most the
+ * current tick, and within the last `capacity` ticks.  Waits for the
+ * handle's stream. */
+int lbm_get_trajectory(lbm_handle *h, int32_t id, int64_t first, int64_t count, double *out);
+
+/* writeTrajectories (gpu/aux.jl:271-317) as text: fluidTrj_<tick>.csv with
+ * columns tick, time, id_x, id_y[, id_z], coordinate_*, massDensity,
+ * fluidVelocity_*, f1..fQ, one row per node in column-major order; id_* are
+ * 1-based indices.  rho is (N...), u (N..., dims) and f (N..., Q), column
+ * major, FP64 host arrays.  Host code only: no handle or device.  Floats
+ * are written as the shortest decimal that reads back to the same double. */
+int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                        const double *const coords[3], const double *rho, const double *u, int32_t q,
+                        const double *f);
+
 /* Boundary links of particle id in the reference's findall order over its
  * streamingInvasionRegions (gpu/aux.jl:187-198, gpu/fluids.jl:181): link
  * (node, j) means node is fluid and node + c_j is solid.  Writes at most cap

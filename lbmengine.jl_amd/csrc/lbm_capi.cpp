@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <charconv>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -78,6 +79,13 @@ struct lbm_handle {
   int64_t timed_steps = 0;
   double step_kernel_ms = 0.0, step_ms = 0.0;
   double adj_forward_ms = 0.0, adj_reverse_ms = 0.0;  // last lbm_adjoint_run
+  // particle-particle interaction terms (device copy of inter_h)
+  std::vector<lbm_interaction> inter_h;
+  lbm_interaction *inter_d = nullptr;
+  // per-tick particle log: [LBM_MAX_PARTICLES][traj_cap][9]
+  double *traj_d = nullptr;
+  int64_t traj_cap = 0;
+  int64_t traj_from = 0;  // tick at which the log was turned on
   // captured blocks of graph_k ticks, one per starting ping-pong buffer
   // graphs of graph_k fused kernels, per starting population and particle parity
   hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -223,6 +231,10 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   sp.partial_in = h->partial + (size_t)pin * h->partial_cap;
   sp.partial_out = h->partial + (size_t)(pin ^ 1) * h->partial_cap;
   sp.linkval = h->linkval;
+  sp.inter = h->inter_d;
+  sp.ninter = (int32_t)h->inter_h.size();
+  sp.traj = h->traj_d;
+  sp.traj_cap = h->traj_cap;
   sp.np = (int32_t)h->parts_h.size();
   sp.semantics = h->cfg.semantics;
   sp.dims = h->dims;
@@ -389,6 +401,8 @@ int lbm_destroy(lbm_handle *h) {
   if (h->partial) hipFree(h->partial);
   if (h->rho_side) hipFree(h->rho_side);
   if (h->linkval) hipFree(h->linkval);
+  if (h->inter_d) hipFree(h->inter_d);
+  if (h->traj_d) hipFree(h->traj_d);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -661,6 +675,191 @@ int lbm_get_particle_load(lbm_handle *h, int32_t id, double force[3], double tor
     force[k] = d.force[k];
     torque[k] = d.torque[k];
   }
+  return LBM_OK;
+}
+
+int lbm_set_interactions(lbm_handle *h, const lbm_interaction *terms, int32_t n) {
+  if (!h || (n > 0 && !terms)) return fail(LBM_EINVAL, "null argument");
+  if (n < 0 || n > LBM_MAX_INTERACTIONS) return fail(LBM_EINVAL, "interaction count out of range");
+  if (n > 0 && h->cfg.semantics != LBM_SEMANTICS_GPU)
+    return fail(LBM_EINVAL, "particle interactions exist only in the GPU semantics");
+  const int np = (int)h->parts_h.size();
+  for (int t = 0; t < n; t++) {
+    const lbm_interaction &I = terms[t];
+    if (I.kind < LBM_INTERACTION_LINEAR || I.kind > LBM_INTERACTION_LENNARD_JONES)
+      return fail(LBM_EINVAL, "unknown interaction kind");
+    const int nid = I.kind == LBM_INTERACTION_POLAR ? 3 : 2;
+    for (int k = 0; k < nid; k++)
+      if (I.ids[k] < 0 || I.ids[k] >= np) return fail(LBM_EINVAL, "interaction names a particle that does not exist");
+    if (I.kind == LBM_INTERACTION_DIPOLE && I.params[1] != 0.0 && h->dims != 2)
+      return fail(LBM_EINVAL, "a scalar magnetic field is a 2D field");
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (!h->inter_d)
+    HIP_TRY(hipMalloc(&h->inter_d, sizeof(lbm_interaction) * LBM_MAX_INTERACTIONS), "allocate interactions");
+  if (n > 0)
+    HIP_TRY(hipMemcpy(h->inter_d, terms, sizeof(lbm_interaction) * n, hipMemcpyHostToDevice), "copy interactions");
+  h->inter_h.assign(terms, terms + n);
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_set_trajectory_log(lbm_handle *h, int64_t capacity) {
+  if (!h) return fail(LBM_EINVAL, "null argument");
+  if (capacity < 0 || capacity > (int64_t(1) << 24)) return fail(LBM_EINVAL, "log capacity out of range");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (h->traj_d) HIP_TRY(hipFree(h->traj_d), "free log");
+  h->traj_d = nullptr;
+  h->traj_cap = 0;
+  if (capacity > 0) {
+    HIP_TRY(hipMalloc(&h->traj_d, sizeof(double) * 9 * LBM_MAX_PARTICLES * capacity), "allocate log");
+    h->traj_cap = capacity;
+  }
+  h->traj_from = h->tick;
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_get_trajectory(lbm_handle *h, int32_t id, int64_t first, int64_t count, double *out) {
+  if (!h || (count > 0 && !out)) return fail(LBM_EINVAL, "null argument");
+  if (!h->traj_d) return fail(LBM_ESTATE, "the trajectory log is off");
+  if (id < 0 || id >= (int)h->parts_h.size()) return fail(LBM_EINVAL, "bad particle id");
+  if (count < 0) return fail(LBM_EINVAL, "negative count");
+  if (count == 0) return LBM_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  DevParticle d;
+  HIP_TRY(hipMemcpy(&d, parts_of(h, h->pcur) + id, sizeof d, hipMemcpyDeviceToHost), "copy particle");
+  // every tick moves every particle once: tick t's state is move
+  // d.moves - (tick - t)
+  const int64_t last = first + count - 1;
+  const int64_t m0 = d.moves - (h->tick - first);
+  if (first <= h->traj_from || m0 < 1 || last > h->tick || h->tick - first >= h->traj_cap)
+    return fail(LBM_EINVAL, "ticks outside what the trajectory log holds");
+  const double *ring = h->traj_d + (size_t)id * h->traj_cap * 9;
+  int64_t done = 0;
+  while (done < count) {
+    const int64_t slot = (m0 - 1 + done) % h->traj_cap;
+    const int64_t run = std::min(count - done, h->traj_cap - slot);
+    HIP_TRY(hipMemcpy(out + done * 9, ring + slot * 9, sizeof(double) * 9 * run, hipMemcpyDeviceToHost),
+            "copy log");
+    done += run;
+  }
+  return LBM_OK;
+}
+
+namespace {
+
+// A double as Julia prints it: the shortest decimal that reads back to the
+// same value, plain for 1e-4 <= |x| < 1e6 (with ".0" on whole numbers) and
+// d.ddde<exp> outside.
+int julia_double(double x, char *out) {
+  if (std::isnan(x)) return std::sprintf(out, "NaN");
+  if (std::isinf(x)) return std::sprintf(out, x < 0 ? "-Inf" : "Inf");
+  char *o = out;
+  if (std::signbit(x)) *o++ = '-';
+  x = std::fabs(x);
+  if (x == 0.0) {
+    std::memcpy(o, "0.0", 3);
+    return (int)(o - out) + 3;
+  }
+  char sci[40];
+  const auto r = std::to_chars(sci, sci + sizeof sci, x, std::chars_format::scientific);
+  *r.ptr = '\0';
+  char digits[32];
+  int nd = 0;
+  const char *c = sci;
+  for (; *c && *c != 'e'; c++)
+    if (*c != '.') digits[nd++] = *c;
+  const int e10 = std::atoi(c + 1);
+  const int pt = e10 + 1;  // digits before the decimal point
+  if (pt > -4 && pt <= 6) {
+    if (pt <= 0) {
+      *o++ = '0';
+      *o++ = '.';
+      for (int i = 0; i < -pt; i++) *o++ = '0';
+      for (int i = 0; i < nd; i++) *o++ = digits[i];
+    } else if (pt < nd) {
+      for (int i = 0; i < pt; i++) *o++ = digits[i];
+      *o++ = '.';
+      for (int i = pt; i < nd; i++) *o++ = digits[i];
+    } else {
+      for (int i = 0; i < nd; i++) *o++ = digits[i];
+      for (int i = nd; i < pt; i++) *o++ = '0';
+      *o++ = '.';
+      *o++ = '0';
+    }
+  } else {
+    *o++ = digits[0];
+    *o++ = '.';
+    if (nd == 1) *o++ = '0';
+    for (int i = 1; i < nd; i++) *o++ = digits[i];
+    o += std::sprintf(o, "e%d", e10);
+  }
+  return (int)(o - out);
+}
+
+}  // namespace
+
+int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                        const double *const coords[3], const double *rho, const double *u, int32_t q,
+                        const double *f) {
+  if (!path || !n || !coords || !rho || !u || !f) return fail(LBM_EINVAL, "null argument");
+  if (dims != 2 && dims != 3) return fail(LBM_EINVAL, "dims must be 2 or 3");
+  int64_t N = 1;
+  for (int k = 0; k < dims; k++) {
+    if (n[k] < 1 || !coords[k]) return fail(LBM_EINVAL, "bad lattice size or coordinates");
+    N *= n[k];
+  }
+  if (q < 1) return fail(LBM_EINVAL, "bad population count");
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return fail(LBM_EINVAL, std::string("cannot open ") + path);
+  static const char *ax[3] = {"x", "y", "z"};
+  std::string line = "tick,time";
+  for (int k = 0; k < dims; k++) line += std::string(",id_") + ax[k];
+  for (int k = 0; k < dims; k++) line += std::string(",coordinate_") + ax[k];
+  line += ",massDensity";
+  for (int k = 0; k < dims; k++) line += std::string(",fluidVelocity_") + ax[k];
+  for (int i = 1; i <= q; i++) line += ",f" + std::to_string(i);
+  line += "\n";
+  std::fwrite(line.data(), 1, line.size(), fp);
+  // tick and time open every row
+  char head[96];
+  int hl = std::sprintf(head, "%lld,", (long long)tick);
+  hl += julia_double(time, head + hl);
+  std::vector<char> buf;
+  buf.reserve(1 << 20);
+  char tmp[64];
+  for (int64_t x = 0; x < N; x++) {
+    buf.insert(buf.end(), head, head + hl);
+    int64_t idx[3], r = x;
+    for (int k = 0; k < dims; k++) {
+      idx[k] = r % n[k];
+      r /= n[k];
+    }
+    for (int k = 0; k < dims; k++) {
+      const int l = std::sprintf(tmp, ",%lld", (long long)(idx[k] + 1));
+      buf.insert(buf.end(), tmp, tmp + l);
+    }
+    auto put = [&](double v) {
+      tmp[0] = ',';
+      const int l = julia_double(v, tmp + 1);
+      buf.insert(buf.end(), tmp, tmp + l + 1);
+    };
+    for (int k = 0; k < dims; k++) put(coords[k][idx[k]]);
+    put(rho[x]);
+    for (int k = 0; k < dims; k++) put(u[(int64_t)k * N + x]);
+    for (int i = 0; i < q; i++) put(f[(int64_t)i * N + x]);
+    buf.push_back('\n');
+    if (buf.size() > (1u << 20)) {
+      if (std::fwrite(buf.data(), 1, buf.size(), fp) != buf.size()) {
+        std::fclose(fp);
+        return fail(LBM_EINVAL, std::string("write failed: ") + path);
+      }
+      buf.clear();
+    }
+  }
+  const bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+  if (std::fclose(fp) != 0 || !ok) return fail(LBM_EINVAL, std::string("write failed: ") + path);
   return LBM_OK;
 }
 
@@ -1488,6 +1687,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   if (h->dims != 3 || h->Q != 19) return fail(LBM_EINVAL, "the adjoint is built for D3Q19");
   if (h->cfg.semantics != LBM_SEMANTICS_GPU) return fail(LBM_EINVAL, "the adjoint follows the GPU semantics");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
+  if (!h->inter_h.empty()) return fail(LBM_EINVAL, "the adjoint does not differentiate particle interactions");
   const int np = (int)h->parts_h.size();
   const int64_t T = cfg->ticks, K = cfg->checkpoint_every, Wn = cfg->window;
   if (T < 1 || K < 1 || Wn < 1 || Wn > T) return fail(LBM_EINVAL, "need ticks >= 1, checkpoint_every >= 1, 1 <= window <= ticks");

@@ -62,6 +62,7 @@ struct DevParticle {
   int32_t kind, couple_forces, couple_torques, err;
   int32_t pending;   // 1: the move from the partials of the last box kernel is still to apply
   int32_t whole;     // 1: the box blocks take its whole owned region, not a band (large radii)
+  int64_t moves;     // moves applied since the particle was added (the per-tick log's index)
   double radius, B1, B2, inv_mass, inv_I;
   double R[3], V[3], W[3], e[3];       // state
   double Rg[3];                        // geometry snapshot

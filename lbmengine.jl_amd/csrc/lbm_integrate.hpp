@@ -72,6 +72,10 @@ struct StepParticles {
   const double *partial_in;  // [box block][particle][6][hi, lo] of the previous box kernel
   double *partial_out;       // the same, of this kernel
   void *linkval;             // [box block][T (Q - 1)]: link-slot values (compute type)
+  const lbm_interaction *inter;  // particle-particle interaction terms, in application order
+  double *traj;              // per-tick log: [particle][traj_cap][9], or null
+  int64_t traj_cap;
+  int32_t ninter;
   int32_t np, semantics, dims;
   // box blocks: particle p's are [first_block[p], first_block[p + 1]);
   // nbox = first_block[np] of them lead the grid, padded to nbox_pad (a
@@ -209,10 +213,192 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
   find_nv = any_nonzero(dL, nw) || moved;
 }
 
-// The move of particle P from the partials of the previous box kernel:
-// tot = forceInput (0..2) and torqueInput (3..5), then moveParticles!, the
-// link box and the node-velocity snapshot.  Run by one thread.
-__device__ inline void move_record(DevParticle &P, const double tot[6], const StepParticles &sp) {
+// ------------------------------------------- particle-particle interactions
+// x^n (n >= 1) as Julia's ^(::Float64, ::Integer) evaluates it (Base.Math's
+// pow_body: powering by squaring with each product's rounding error carried
+// by fma, so the result is within about an ulp of the exact power); n == 3
+// is x*x*x, as Julia's literal cube is.
+__host__ __device__ inline double julia_pow(double x, int n) {
+#pragma clang fp contract(off)
+  if (n == 3) return x * x * x;
+  double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+  while (n > 1) {
+    if (n & 1) {
+      const double err = fma(y, xnlo, x * ynlo);
+      const double hi = x * y;
+      ynlo = fma(x, y, -hi) + err;
+      y = hi;
+    }
+    const double err = x * 2.0 * xnlo;
+    const double hi = x * x;
+    xnlo = fma(x, x, -hi) + err;
+    x = hi;
+    n >>= 1;
+  }
+  const double err = fma(y, xnlo, x * ynlo);
+  return (isfinite(x) && isfinite(err)) ? fma(x, y, err) : x * y;
+}
+
+// norm of a short host vector (LinearAlgebra's generic_norm2 below its BLAS
+// cutoff): the squares summed in index order
+__host__ __device__ inline double norm_seq(const double *v, int n) {
+#pragma clang fp contract(off)
+  double s = v[0] * v[0];
+  for (int k = 1; k < n; k++) s = s + v[k] * v[k];
+  return sqrt(s);
+}
+
+// The reference's cross (gpu/aux.jl:139-157): 3-vectors as usual; two
+// 2-vectors give the scalar z component; a scalar w with a 2-vector V gives
+// (-w V_2, w V_1) whichever side the scalar is on.
+__host__ __device__ inline void cross3(const double *v, const double *w, double *o) {
+#pragma clang fp contract(off)
+  o[0] = v[1] * w[2] - v[2] * w[1];
+  o[1] = -v[0] * w[2] + v[2] * w[0];
+  o[2] = v[0] * w[1] - v[1] * w[0];
+}
+__host__ __device__ inline double cross2(const double *v, const double *w) {
+#pragma clang fp contract(off)
+  return v[0] * w[1] - v[1] * w[0];
+}
+__host__ __device__ inline void cross_sv(double s, const double *V, double *o) {
+#pragma clang fp contract(off)
+  o[0] = -s * V[1];
+  o[1] = s * V[0];
+}
+
+// One term of molecularDynamicsTick! (gpu/particles.jl:110-197) from the
+// positions R1, R2 (and R3) before the tick's moves.  Pair terms give the
+// force on id1 in fa (id2 takes -fa); the polar term gives force321 (on id1)
+// in fa and force123 (on id3) in fb (id2 takes -(fa + fb)).  Returns false
+// when the term adds nothing (Lennard-Jones beyond its cutoff).
+__host__ __device__ inline bool interaction_term(const lbm_interaction &I, const double *R1, const double *R2,
+                                                 const double *R3, int d, double fa[3], double fb[3]) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 3; k++) fa[k] = fb[k] = 0.0;
+  if (I.kind == LBM_INTERACTION_POLAR) {
+    // gpu/particles.jl:136-158
+    double d12[3] = {0, 0, 0}, d32[3] = {0, 0, 0}, u12[3] = {0, 0, 0}, u32[3] = {0, 0, 0};
+    for (int k = 0; k < d; k++) {
+      d12[k] = R1[k] - R2[k];
+      d32[k] = R3[k] - R2[k];
+    }
+    const double n12 = norm_seq(d12, d), n32 = norm_seq(d32, d);
+    for (int k = 0; k < d; k++) {
+      u12[k] = d12[k] / n12;
+      u32[k] = d32[k] / n32;
+    }
+    double c = u12[0] * u32[0];
+    for (int k = 1; k < d; k++) c = c + u12[k] * u32[k];
+    if (fabs(c) > 1.0) c = c > 0.0 ? 1.0 : -1.0;
+    const double angle = acos(c);
+    const double s = I.params[1] * (angle - I.params[0]);
+    if (d == 3) {
+      double td[3], tq[3], c1[3], c3[3];
+      cross3(u32, u12, td);
+      const double an = norm_seq(td, 3);
+      if (an > 0.0)
+        for (int k = 0; k < 3; k++) td[k] = td[k] / an;
+      for (int k = 0; k < 3; k++) tq[k] = s * td[k];
+      cross3(tq, u12, c1);
+      cross3(tq, u32, c3);
+      for (int k = 0; k < 3; k++) {
+        fa[k] = -c1[k] / n12;
+        fb[k] = c3[k] / n32;
+      }
+    } else {
+      double td = cross2(u32, u12);
+      const double an = fabs(td);
+      if (an > 0.0) td = td / an;
+      const double tq = s * td;
+      double c1[2], c3[2];
+      cross_sv(tq, u12, c1);
+      cross_sv(tq, u32, c3);
+      for (int k = 0; k < 2; k++) {
+        fa[k] = -c1[k] / n12;
+        fb[k] = c3[k] / n32;
+      }
+    }
+    return true;
+  }
+  double d21[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+  for (int k = 0; k < d; k++) d21[k] = R2[k] - R1[k];
+  const double disp = norm_seq(d21, d);
+  if (I.kind == LBM_INTERACTION_LENNARD_JONES && !(disp < I.params[2])) return false;
+  for (int k = 0; k < d; k++) u[k] = d21[k] / disp;
+  double s;
+  switch (I.kind) {
+    case LBM_INTERACTION_LINEAR:  // gpu/particles.jl:110-121
+      s = I.params[1] * (disp - I.params[0]);
+      break;
+    case LBM_INTERACTION_BISTABLE: {  // gpu/particles.jl:123-134
+      const double x = disp - I.params[0];
+      s = I.params[1] * julia_pow(x, 3) - I.params[2] * x;
+      break;
+    }
+    case LBM_INTERACTION_LENNARD_JONES: {  // gpu/particles.jl:179-197
+      const double sd = I.params[1] / disp;
+      s = -4.0 * I.params[0] / disp * (12.0 * julia_pow(sd, 12) - 6.0 * julia_pow(sd, 6));
+      break;
+    }
+    default: {  // LBM_INTERACTION_DIPOLE, gpu/particles.jl:160-177
+      const double *B = I.field;
+      const double pre = -I.params[0] / julia_pow(disp, 4);
+      double ccb[3] = {0, 0, 0}, BB, s2;
+      if (d == 3) {
+        double cb[3];
+        cross3(u, B, cb);
+        cross3(cb, B, ccb);
+        BB = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+        s2 = cb[0] * cb[0] + cb[1] * cb[1] + cb[2] * cb[2];
+      } else if (I.params[1] != 0.0) {  // scalar field B = field[0]
+        double cb[2];
+        cross_sv(B[0], u, cb);
+        cross_sv(B[0], cb, ccb);
+        BB = B[0] * B[0];
+        s2 = cb[0] * cb[0] + cb[1] * cb[1];
+      } else {
+        const double cb = cross2(u, B);
+        cross_sv(cb, B, ccb);
+        BB = B[0] * B[0] + B[1] * B[1];
+        s2 = cb * cb;
+      }
+      for (int k = 0; k < d; k++) fa[k] = pre * (2.0 * ccb[k] - 2.0 * BB * u[k] + 5.0 * u[k] * s2);
+      return true;
+    }
+  }
+  for (int k = 0; k < d; k++) fa[k] = s * u[k];
+  return true;
+}
+
+// forceInput of particle p after every interaction term, in order, from the
+// records' positions (all before the tick's moves): F holds the momentum
+// exchange on entry.
+__device__ inline void add_interaction_forces(const StepParticles &sp, const DevParticle *recs, int p, double F[3]) {
+#pragma clang fp contract(off)
+  const int d = sp.dims;
+  for (int t = 0; t < sp.ninter; t++) {
+    const lbm_interaction &I = sp.inter[t];
+    const int i1 = I.ids[0], i2 = I.ids[1];
+    const bool polar = I.kind == LBM_INTERACTION_POLAR;
+    const int i3 = polar ? I.ids[2] : -1;
+    if (p != i1 && p != i2 && p != i3) continue;
+    double fa[3], fb[3];
+    if (!interaction_term(I, recs[i1].R, recs[i2].R, polar ? recs[i3].R : nullptr, d, fa, fb)) continue;
+    if (p == i1)
+      for (int k = 0; k < d; k++) F[k] = F[k] + fa[k];
+    if (polar && p == i3)
+      for (int k = 0; k < d; k++) F[k] = F[k] + fb[k];
+    if (p == i2)
+      for (int k = 0; k < d; k++) F[k] = F[k] - (polar ? fa[k] + fb[k] : fa[k]);
+  }
+}
+
+// The move of particle P (index p) from the partials of the previous box
+// kernel: tot = forceInput (0..2) and torqueInput (3..5), then
+// moveParticles!, the link box and the node-velocity snapshot; with `log`,
+// the state after the move also goes to the per-tick log.  Run by one thread.
+__device__ inline void move_record(DevParticle &P, int p, const double tot[6], const StepParticles &sp, bool log) {
   const double F[3] = {tot[0], tot[1], sp.dims == 3 ? tot[2] : 0.0};
   const double T[3] = {tot[3], sp.dims == 3 ? tot[4] : 0.0, sp.dims == 3 ? tot[5] : 0.0};
   bool moved, find_nv;
@@ -245,6 +431,15 @@ __device__ inline void move_record(DevParticle &P, const double tot[6], const St
     }
   }
   P.pending = 0;
+  P.moves += 1;
+  if (log && sp.traj) {
+    double *o = sp.traj + ((int64_t)p * sp.traj_cap + (P.moves - 1) % sp.traj_cap) * 9;
+    for (int k = 0; k < 3; k++) {
+      o[k] = P.R[k];
+      o[3 + k] = P.V[k];
+      o[6 + k] = P.W[k];
+    }
+  }
 }
 
 // Apply the pending moves of the records in LDS (recs[0..np)) from
@@ -262,7 +457,10 @@ __device__ inline DD prefetch_partial(const StepParticles &sp) {
   return DD{src[0], src[1]};
 }
 
-__device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs, const DD *pre = nullptr) {
+// With `log` (one block per lattice and kernel), the moved states also go to
+// the per-tick log.
+__device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs, const DD *pre = nullptr,
+                                           bool log = false) {
   __shared__ double tot[LBM_MAX_PARTICLES][6];
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
@@ -290,7 +488,13 @@ __device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle 
     }
   }
   __syncthreads();
-  if (tid < sp.np && recs[tid].pending) move_record(recs[tid], tot[tid], sp);
+  if (sp.ninter > 0) {
+    // molecularDynamicsTick!: every interaction acts on the positions before
+    // any particle of the tick moves
+    if (tid < sp.np && recs[tid].pending) add_interaction_forces(sp, recs, tid, tot[tid]);
+    __syncthreads();
+  }
+  if (tid < sp.np && recs[tid].pending) move_record(recs[tid], tid, tot[tid], sp, log);
   __syncthreads();
 }
 

@@ -29,7 +29,7 @@ __device__ __forceinline__ void finalize_block(const StepParticles &sp) {
   for (int w = tid; w < sp.np * kWords; w += blockDim.x)
     reinterpret_cast<uint64_t *>(recs)[w] = reinterpret_cast<const uint64_t *>(sp.parts_in)[w];
   __syncthreads();
-  apply_pending_moves(sp, recs);
+  apply_pending_moves(sp, recs, nullptr, true);
   for (int w = tid; w < sp.np * kWords; w += blockDim.x)
     reinterpret_cast<uint64_t *>(sp.parts_out)[w] = reinterpret_cast<const uint64_t *>(recs)[w];
 }

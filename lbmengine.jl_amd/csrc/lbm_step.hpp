@@ -845,7 +845,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     }
   }
   if (tid < sp.np) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
-  apply_pending_moves(sp, prec, &pre);  // (synchronises the block)
+  apply_pending_moves(sp, prec, &pre, bid == 0);  // (synchronises the block)
 
   const Band &B = bands[pb];
   const int ex = ext[0], ey = ext[1], ez = ext[2];

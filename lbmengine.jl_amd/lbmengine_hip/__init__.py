@@ -8,9 +8,12 @@ from ._capi import LBMError, LIB_PATH
 from .model import (DEFAULT, LBMbatch, LBMmodel, ResolvedModel, addBead, addSquirmer, hydroVariablesUpdate,
                     julia_range, LBMpropagate, modelInit, propagate_ticks, resolve_model,
                     resolve_particle, sphere_count)
+from .interactions import (addBistableBond, addDipoles, addLennardJones, addLinearBond, addPolarBond,
+                           addWCA)
 
 __all__ = [
     "DEFAULT", "LBMError", "LBMbatch", "LBMmodel", "LIB_PATH", "ResolvedModel", "addBead", "addSquirmer",
+    "addBistableBond", "addDipoles", "addLennardJones", "addLinearBond", "addPolarBond", "addWCA",
     "hydroVariablesUpdate", "julia_range", "LBMpropagate", "modelInit", "propagate_ticks",
     "resolve_model", "resolve_particle", "sphere_count",
 ]

@@ -21,6 +21,8 @@ SEMANTICS = {"gpu": 0, "cpu": 1}
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 BEAD, SQUIRMER = 0, 1
 MAX_PARTICLES = 8
+INTERACTION = {"linear": 0, "bistable": 1, "polar": 2, "dipole": 3, "lennardJones": 4}
+MAX_INTERACTIONS = 256
 
 LIB_PATH = Path(os.environ.get("LBM_HIP_LIB", Path(__file__).resolve().parent.parent / "liblbm_hip.so"))
 
@@ -57,6 +59,13 @@ class LbmParticle(C.Structure):
     ]
 
 
+class LbmInteraction(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("ids", C.c_int32 * 3),
+        ("params", C.c_double * 4), ("field", C.c_double * 3),
+    ]
+
+
 _dp = C.POINTER(C.c_double)
 _vp = C.c_void_p
 
@@ -77,6 +86,11 @@ SIGNATURES = {
     "lbm_get_particle_load": [_vp, C.c_int32, _dp, _dp],
     "lbm_get_links": [_vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int64,
                       C.POINTER(C.c_int64)],
+    "lbm_set_interactions": [_vp, C.POINTER(LbmInteraction), C.c_int32],
+    "lbm_set_trajectory_log": [_vp, C.c_int64],
+    "lbm_get_trajectory": [_vp, C.c_int32, C.c_int64, C.c_int64, _dp],
+    "lbm_write_fluid_csv": [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.c_int64, C.c_double,
+                            C.POINTER(_dp), _dp, _dp, C.c_int32, _dp],
     "lbm_run": [_vp, C.c_int64],
     "lbm_sync": [_vp],
     "lbm_get_tick": [_vp, C.POINTER(C.c_int64), _dp],

@@ -219,8 +219,6 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
     forcingScheme = "guo" if forcingScheme is DEFAULT else _sym(forcingScheme)
     if forcingScheme != "guo":
         raise NotImplementedError(f"forcing scheme {forcingScheme} is not implemented in the hip backend")
-    if saveData:
-        raise NotImplementedError("saveData is not implemented in the hip backend")
 
     if massDensity is DEFAULT:
         init_rho = 1.0
@@ -285,6 +283,9 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
         else:
             raise ValueError("force density does not have consistent dimensions!")
         schemes.append(forcingScheme)
+
+    if saveData:
+        schemes.append("saveData")
 
     storage, semantics = _sym(storage), _sym(semantics)
     assert storage in _capi.STORAGE, f"storage must be one of {list(_capi.STORAGE)}"
@@ -456,6 +457,7 @@ class LBMmodel:
         _capi.check(self._fn["lbm_create"](C.byref(cfg), C.byref(h)))
         self._h = h
         self.particles: list[_Particle] = []
+        self.particleInteractions: list = []
         self.schemes = list(rm.schemes)
         if rm.force_mode == _capi.FORCE_FIELD:
             F = np.asfortranarray(rm.force_field, dtype=np.float64)
@@ -467,6 +469,9 @@ class LBMmodel:
         else:
             f0 = np.asfortranarray(rm.initial_populations())
             _capi.check(self._fn["lbm_set_populations"](h, _capi.as_dptr(f0)))
+        if "saveData" in self.schemes:
+            from . import savedata
+            savedata.start(self)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -560,6 +565,9 @@ class LBMmodel:
     def run(self, ticks: int) -> None:
         _capi.check(self._fn["lbm_run"](self._h, int(ticks)))
 
+    def _set_interactions(self, terms, n: int) -> None:
+        _capi.check(self._fn["lbm_set_interactions"](self._h, terms, int(n)))
+
     def set_timing(self, on: bool) -> None:
         _capi.check(self._fn["lbm_set_timing"](self._h, int(bool(on))))
 
@@ -640,6 +648,9 @@ def _add(model: LBMmodel, kind: str, kw) -> None:
     for s in ("ladd", "bounceBack"):
         if s not in model.schemes:
             model.schemes.append(s)
+    if "saveData" in model.schemes:
+        from . import savedata
+        savedata.write_particle(model, pid.value)
 
 
 def addSquirmer(model: LBMmodel, **kw) -> None:
@@ -676,6 +687,8 @@ def LBMpropagate(model, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
     an LBMbatch, every model of it advances (simulationTime is then read in
     the time step of the batch's first model, which all must share)."""
     if isinstance(model, LBMbatch):
+        if any("saveData" in m.schemes for m in model.models):
+            raise NotImplementedError("saveData runs one model at a time: call LBMpropagate on the model")
         dts = {m.resolved.dt for m in model.models}
         if simulationTime is not DEFAULT and len(dts) != 1:
             raise ValueError("simulationTime needs one time step across the batch; give ticks instead")
@@ -686,7 +699,11 @@ def LBMpropagate(model, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
             print(f"\r t = {round(model.models[0].time, 2)}")
         return
     n = propagate_ticks(model.resolved.dt, simulationTime, ticks)
-    model.run(n)
-    model.sync()
+    if "saveData" in model.schemes:
+        from . import savedata
+        savedata.propagate(model, n, None if ticksSaved is DEFAULT else ticksSaved)
+    else:
+        model.run(n)
+        model.sync()
     if verbose:
         print(f"\r t = {round(model.time, 2)}")

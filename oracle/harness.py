@@ -80,6 +80,10 @@ class _OParticle:
     def swimmingDirection(self):
         return np.array(self._state().swimming_direction[: self._m.dims])
 
+    @property
+    def particleParams(self) -> dict:
+        return dict(radius=self._state().radius)
+
     def load(self):
         F, T = np.zeros(3), np.zeros(3)
         _check(self._m._fn["lbm_get_particle_load"](self._m._h, self.id, _capi.as_dptr(F), _capi.as_dptr(T)))
@@ -113,6 +117,7 @@ class OracleModel:
         _check(self._fn["lbm_create"](C.byref(cfg), C.byref(h)))
         self._h = h
         self.particles = []
+        self.particleInteractions = []
         if rm.force_mode == _capi.FORCE_FIELD:
             F = np.asfortranarray(rm.force_field, dtype=np.float64)
             _check(self._fn["lbm_set_force_field"](h, _capi.as_dptr(F)))
@@ -133,6 +138,14 @@ class OracleModel:
     @property
     def dims(self):
         return self.resolved.dims
+
+    @property
+    def spaceTime(self) -> dict:
+        rm = self.resolved
+        return dict(latticeParameter=rm.dx, timeStep=rm.dt, dims=rm.dims)
+
+    def _set_interactions(self, terms, n: int) -> None:
+        _check(self._fn["lbm_set_interactions"](self._h, terms, int(n)))
 
     def add(self, kind: str, **kw):
         p = resolve_particle(self.resolved, kind=kind, **kw)

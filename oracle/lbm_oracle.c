@@ -19,6 +19,7 @@
  *   getNodeVelocity       gpu/aux.jl:229-263, gpu/kernels.jl:376-415
  *   bulkVelocity          cpu/particles.jl:20-43
  *   moveParticles!        gpu/particles.jl:7-92     cpu/particles.jl:45-100
+ *   molecularDynamicsTick!, applyInteraction!  gpu/particles.jl:100-197
  *   findInitialConditions gpu/initMethods.jl:417-438
  *   wallNodes             gpu/misc.jl:7-31
  *   D2Q9, D3Q19           gpu/misc.jl:195-210,230-248
@@ -97,6 +98,8 @@ typedef struct lbmo_model {
   int guo, bounce_back, ladd;
   int np;
   opart parts[LBM_MAX_PARTICLES];
+  int ninter;
+  lbm_interaction inter[LBM_MAX_INTERACTIONS]; /* particleInteractions, pairs expanded */
   int64_t tick;
   double time;
   int c[OMAXQ][3];
@@ -498,6 +501,155 @@ static void move_particle_cpu(lbmo_model *m, opart *pp) {
   if (find_nv) rebuild_node_velocity(m, pp);
 }
 
+/* ------------------------------------------------- molecular dynamics */
+
+/* x^n for integer n >= 1 as Julia's ^(::Float64, ::Integer) computes it:
+ * Base.Math.pow_body, binary powering whose products keep their rounding
+ * errors (two_mul via fma) and fold them back at the end; n == 3 is the
+ * literal cube x*x*x.  The result is within about an ulp of x^n. */
+static double jpow(double x, int n) {
+  if (n == 3) return x * x * x;
+  double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+  while (n > 1) {
+    if (n & 1) {
+      double err = fma(y, xnlo, x * ynlo);
+      double hi = x * y;
+      double lo = fma(x, y, -hi);
+      y = hi;
+      ynlo = lo + err;
+    }
+    double err = x * 2.0 * xnlo;
+    double hi = x * x;
+    double lo = fma(x, x, -hi);
+    x = hi;
+    xnlo = lo + err;
+    n >>= 1;
+  }
+  double err = fma(y, xnlo, x * ynlo);
+  return (isfinite(x) && isfinite(err)) ? fma(x, y, err) : x * y;
+}
+
+/* `v |> Array |> norm` (LinearAlgebra.generic_norm2 for short vectors) */
+static double vnorm(const double *v, int d) {
+  double s = v[0] * v[0];
+  for (int k = 1; k < d; k++) s = s + v[k] * v[k];
+  return sqrt(s);
+}
+
+/* cross, gpu/aux.jl:139-157 */
+static void vcross3(const double *v, const double *w, double *o) {
+  o[0] = v[1] * w[2] - v[2] * w[1];
+  o[1] = -v[0] * w[2] + v[2] * w[0];
+  o[2] = v[0] * w[1] - v[1] * w[0];
+}
+static double vcross2(const double *v, const double *w) { return v[0] * w[1] - v[1] * w[0]; }
+static void vcross_real(double om, const double *V, double *o) { /* both argument orders */
+  o[0] = -om * V[1];
+  o[1] = om * V[0];
+}
+
+static void add_force(lbmo_model *m, int id, const double *f, double sgn) {
+  for (int k = 0; k < m->d; k++)
+    m->parts[id].force[k] = sgn > 0 ? m->parts[id].force[k] + f[k] : m->parts[id].force[k] - f[k];
+}
+
+/* applyInteraction!, gpu/particles.jl:110-197 (one expanded term) */
+static void apply_interaction(lbmo_model *m, const lbm_interaction *I) {
+  const int d = m->d;
+  const double *R1 = m->parts[I->ids[0]].p.position, *R2 = m->parts[I->ids[1]].p.position;
+  if (I->kind == LBM_INTERACTION_POLAR) { /* :136-158 */
+    const double *R3 = m->parts[I->ids[2]].p.position;
+    double disp12[3], disp32[3], u12[3], u32[3];
+    for (int k = 0; k < d; k++) disp12[k] = R1[k] - R2[k];
+    const double n12 = vnorm(disp12, d);
+    for (int k = 0; k < d; k++) u12[k] = disp12[k] / n12;
+    for (int k = 0; k < d; k++) disp32[k] = R3[k] - R2[k];
+    const double n32 = vnorm(disp32, d);
+    for (int k = 0; k < d; k++) u32[k] = disp32[k] / n32;
+    double x = 0.0;
+    for (int k = 0; k < d; k++) x = k == 0 ? u12[0] * u32[0] : x + u12[k] * u32[k];
+    const double angle = acos(fabs(x) > 1 ? (x > 0 ? 1.0 : -1.0) : x);
+    double f321[3] = {0, 0, 0}, f123[3] = {0, 0, 0}, both[3];
+    if (d == 3) {
+      double td[3], torque[3], c[3];
+      vcross3(u32, u12, td);
+      const double an = vnorm(td, 3);
+      if (an > 0)
+        for (int k = 0; k < 3; k++) td[k] = td[k] / an;
+      for (int k = 0; k < 3; k++) torque[k] = I->params[1] * (angle - I->params[0]) * td[k];
+      vcross3(torque, u12, c);
+      for (int k = 0; k < 3; k++) f321[k] = -c[k] / n12;
+      vcross3(torque, u32, c);
+      for (int k = 0; k < 3; k++) f123[k] = c[k] / n32;
+    } else {
+      double td = vcross2(u32, u12), c[2];
+      const double an = fabs(td);
+      if (an > 0) td = td / an;
+      const double torque = I->params[1] * (angle - I->params[0]) * td;
+      vcross_real(torque, u12, c);
+      for (int k = 0; k < 2; k++) f321[k] = -c[k] / n12;
+      vcross_real(torque, u32, c);
+      for (int k = 0; k < 2; k++) f123[k] = c[k] / n32;
+    }
+    add_force(m, I->ids[0], f321, 1);
+    add_force(m, I->ids[2], f123, 1);
+    for (int k = 0; k < d; k++) both[k] = f321[k] + f123[k];
+    add_force(m, I->ids[1], both, -1);
+    return;
+  }
+  double disp21[3], u[3], f21[3] = {0, 0, 0};
+  for (int k = 0; k < d; k++) disp21[k] = R2[k] - R1[k];
+  const double disp = vnorm(disp21, d);
+  switch (I->kind) {
+    case LBM_INTERACTION_LINEAR: /* :110-121 */
+      for (int k = 0; k < d; k++) u[k] = disp21[k] / disp;
+      for (int k = 0; k < d; k++) f21[k] = I->params[1] * (disp - I->params[0]) * u[k];
+      break;
+    case LBM_INTERACTION_BISTABLE: { /* :123-134 */
+      for (int k = 0; k < d; k++) u[k] = disp21[k] / disp;
+      const double x = disp - I->params[0];
+      for (int k = 0; k < d; k++) f21[k] = (I->params[1] * jpow(x, 3) - I->params[2] * x) * u[k];
+      break;
+    }
+    case LBM_INTERACTION_DIPOLE: { /* :160-177, one pair */
+      const double *B = I->field;
+      for (int k = 0; k < d; k++) u[k] = disp21[k] / disp;
+      double t1[3] = {0, 0, 0}, BB, cc;
+      if (d == 3) {
+        double c1[3];
+        vcross3(u, B, c1);
+        vcross3(c1, B, t1);
+        BB = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+        cc = c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2];
+      } else if (I->params[1] != 0.0) { /* B a number: cross(V, B) = (-B V2, B V1) */
+        double c1[2];
+        vcross_real(B[0], u, c1);
+        vcross_real(B[0], c1, t1);
+        BB = B[0] * B[0];
+        cc = c1[0] * c1[0] + c1[1] * c1[1];
+      } else {
+        const double c1 = vcross2(u, B);
+        vcross_real(c1, B, t1);
+        BB = B[0] * B[0] + B[1] * B[1];
+        cc = c1 * c1;
+      }
+      const double pre = -I->params[0] / jpow(disp, 4);
+      for (int k = 0; k < d; k++) f21[k] = pre * (2 * t1[k] - 2 * BB * u[k] + 5 * u[k] * cc);
+      break;
+    }
+    default: { /* LBM_INTERACTION_LENNARD_JONES, :179-197 */
+      if (!(disp < I->params[2])) return;
+      for (int k = 0; k < d; k++) u[k] = disp21[k] / disp;
+      const double sd = I->params[1] / disp;
+      for (int k = 0; k < d; k++)
+        f21[k] = -4 * I->params[0] / disp * (12 * jpow(sd, 12) - 6 * jpow(sd, 6)) * u[k];
+      break;
+    }
+  }
+  add_force(m, I->ids[0], f21, 1);
+  add_force(m, I->ids[1], f21, -1);
+}
+
 /* ----------------------------------------------------------------- tick */
 
 /* tick! — gpu/fluids.jl:132-226 (gpu semantics), cpu/fluids.jl:377-461 */
@@ -610,6 +762,8 @@ static void tick(lbmo_model *m) {
   m->time += m->cfg.dt;
   m->tick += 1;
   hydro_update(m);
+  if (m->ladd && gpu)
+    for (int t = 0; t < m->ninter; t++) apply_interaction(m, &m->inter[t]);
   if (m->ladd)
     for (int p = 0; p < m->np; p++) {
       if (gpu)
@@ -808,6 +962,16 @@ int lbmo_run(lbmo_model *m, int64_t nsteps) {
 int lbmo_get_tick(lbmo_model *m, int64_t *t, double *time) {
   *t = m->tick;
   *time = m->time;
+  return LBM_OK;
+}
+
+int lbmo_set_interactions(lbmo_model *m, const lbm_interaction *terms, int32_t n) {
+  if (n < 0 || n > LBM_MAX_INTERACTIONS) return fail(LBM_EINVAL, "interaction count out of range");
+  for (int t = 0; t < n; t++)
+    for (int k = 0; k < (terms[t].kind == LBM_INTERACTION_POLAR ? 3 : 2); k++)
+      if (terms[t].ids[k] < 0 || terms[t].ids[k] >= m->np) return fail(LBM_EINVAL, "bad particle id");
+  if (n > 0) memcpy(m->inter, terms, sizeof(lbm_interaction) * (size_t)n);
+  m->ninter = n;
   return LBM_OK;
 }
 
