@@ -197,6 +197,20 @@ int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int6
                         const double *const coords[3], const double *rho, const double *u, int32_t q,
                         const double *f);
 
+/* writeTensor (gpu/aux.jl:364-400) as text: <name>.csv with columns tick,
+ * time, id_*, coordinate_*, then component_ab for the dims x dims tensor t
+ * in column-major order (component_xx, component_yx, ...); t is
+ * (N..., dims, dims), column major, FP64 host.  Host code only. */
+int lbm_write_tensor_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                         const double *const coords[3], const double *t);
+
+/* viscousStressTensor (cpu/fluids.jl:507-520) of the current populations:
+ * sigma_ab = -(1 - dt / (2 tau)) sum_i c_ia c_ib (f_i - feq_i(rho, u)), with
+ * rho and u of hydroVariablesUpdate! and the integer velocities c_i;
+ * evaluated on the device in FP64 whatever the storage.  sigma is
+ * (N..., dims, dims), column major, FP64 host. */
+int lbm_viscous_stress(lbm_handle *h, double *sigma);
+
 /* Boundary links of particle id in the reference's findall order over its
  * streamingInvasionRegions (gpu/aux.jl:187-198, gpu/fluids.jl:181): link
  * (node, j) means node is fluid and node + c_j is solid.  Writes at most cap

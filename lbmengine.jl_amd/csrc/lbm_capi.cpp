@@ -348,6 +348,7 @@ int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
     case 1: e = launch_set_populations<SK>(h->Q, a, l); break;          \
     case 2: e = launch_get_populations<SK>(h->Q, a, l); break;          \
     case 3: e = launch_hydro<SK>(h->Q, a, l); break;                    \
+    case 4: e = launch_stress<SK>(h->Q, a, l); break;                   \
   }
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: LBM_AUX(kF64); break;
@@ -588,6 +589,25 @@ int lbm_get_hydro(lbm_handle *h, double *rho, double *u) {
   return LBM_OK;
 }
 
+int lbm_viscous_stress(lbm_handle *h, double *sigma) {
+  if (!h || !sigma) return fail(LBM_EINVAL, "null argument");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  const size_t bytes = (size_t)h->N * h->dims * h->dims * sizeof(double);
+  double *tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes), "hipMalloc staging");
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.out64 = tmp;
+  int rc = aux_launch(h, 4, a);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (!rc && e == hipSuccess) e = hipMemcpy(sigma, tmp, bytes, hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "viscous stress");
+  return LBM_OK;
+}
+
 int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
   if (!h || !p) return fail(LBM_EINVAL, "null argument");
   if ((int)h->parts_h.size() >= LBM_MAX_PARTICLES) return fail(LBM_EINVAL, "too many particles");
@@ -800,36 +820,34 @@ int julia_double(double x, char *out) {
 
 }  // namespace
 
-int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
-                        const double *const coords[3], const double *rho, const double *u, int32_t q,
-                        const double *f) {
-  if (!path || !n || !coords || !rho || !u || !f) return fail(LBM_EINVAL, "null argument");
-  if (dims != 2 && dims != 3) return fail(LBM_EINVAL, "dims must be 2 or 3");
+namespace {
+
+// One row per node in column-major order: tick, time, the 1-based indices,
+// the coordinates, then the given columns (one FP64 value per node each),
+// as the reference's DataFrame + CSV.write produce them
+// (gpu/aux.jl:271-317,364-400).
+int write_node_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                   const double *const coords[3], const std::vector<std::string> &names,
+                   const std::vector<const double *> &cols) {
   int64_t N = 1;
-  for (int k = 0; k < dims; k++) {
-    if (n[k] < 1 || !coords[k]) return fail(LBM_EINVAL, "bad lattice size or coordinates");
-    N *= n[k];
-  }
-  if (q < 1) return fail(LBM_EINVAL, "bad population count");
+  for (int k = 0; k < dims; k++) N *= n[k];
   FILE *fp = std::fopen(path, "wb");
   if (!fp) return fail(LBM_EINVAL, std::string("cannot open ") + path);
   static const char *ax[3] = {"x", "y", "z"};
   std::string line = "tick,time";
   for (int k = 0; k < dims; k++) line += std::string(",id_") + ax[k];
   for (int k = 0; k < dims; k++) line += std::string(",coordinate_") + ax[k];
-  line += ",massDensity";
-  for (int k = 0; k < dims; k++) line += std::string(",fluidVelocity_") + ax[k];
-  for (int i = 1; i <= q; i++) line += ",f" + std::to_string(i);
+  for (const std::string &c : names) line += "," + c;
   line += "\n";
-  std::fwrite(line.data(), 1, line.size(), fp);
+  bool ok = std::fwrite(line.data(), 1, line.size(), fp) == line.size();
   // tick and time open every row
   char head[96];
   int hl = std::sprintf(head, "%lld,", (long long)tick);
   hl += julia_double(time, head + hl);
   std::vector<char> buf;
-  buf.reserve(1 << 20);
+  buf.reserve(1 << 21);
   char tmp[64];
-  for (int64_t x = 0; x < N; x++) {
+  for (int64_t x = 0; x < N && ok; x++) {
     buf.insert(buf.end(), head, head + hl);
     int64_t idx[3], r = x;
     for (int k = 0; k < dims; k++) {
@@ -846,21 +864,67 @@ int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int6
       buf.insert(buf.end(), tmp, tmp + l + 1);
     };
     for (int k = 0; k < dims; k++) put(coords[k][idx[k]]);
-    put(rho[x]);
-    for (int k = 0; k < dims; k++) put(u[(int64_t)k * N + x]);
-    for (int i = 0; i < q; i++) put(f[(int64_t)i * N + x]);
+    for (const double *c : cols) put(c[x]);
     buf.push_back('\n');
     if (buf.size() > (1u << 20)) {
-      if (std::fwrite(buf.data(), 1, buf.size(), fp) != buf.size()) {
-        std::fclose(fp);
-        return fail(LBM_EINVAL, std::string("write failed: ") + path);
-      }
+      ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
       buf.clear();
     }
   }
-  const bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+  if (ok) ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
   if (std::fclose(fp) != 0 || !ok) return fail(LBM_EINVAL, std::string("write failed: ") + path);
   return LBM_OK;
+}
+
+int check_csv_args(const char *path, int32_t dims, const int64_t n[3], const double *const coords[3],
+                   int64_t *N) {
+  if (!path || !n || !coords) return fail(LBM_EINVAL, "null argument");
+  if (dims != 2 && dims != 3) return fail(LBM_EINVAL, "dims must be 2 or 3");
+  *N = 1;
+  for (int k = 0; k < dims; k++) {
+    if (n[k] < 1 || !coords[k]) return fail(LBM_EINVAL, "bad lattice size or coordinates");
+    *N *= n[k];
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_write_fluid_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                        const double *const coords[3], const double *rho, const double *u, int32_t q,
+                        const double *f) {
+  int64_t N;
+  if (int rc = check_csv_args(path, dims, n, coords, &N)) return rc;
+  if (!rho || !u || !f) return fail(LBM_EINVAL, "null argument");
+  if (q < 1) return fail(LBM_EINVAL, "bad population count");
+  static const char *ax[3] = {"x", "y", "z"};
+  std::vector<std::string> names{"massDensity"};
+  std::vector<const double *> cols{rho};
+  for (int k = 0; k < dims; k++) {
+    names.push_back(std::string("fluidVelocity_") + ax[k]);
+    cols.push_back(u + (int64_t)k * N);
+  }
+  for (int i = 0; i < q; i++) {
+    names.push_back("f" + std::to_string(i + 1));
+    cols.push_back(f + (int64_t)i * N);
+  }
+  return write_node_csv(path, dims, n, tick, time, coords, names, cols);
+}
+
+int lbm_write_tensor_csv(const char *path, int32_t dims, const int64_t n[3], int64_t tick, double time,
+                         const double *const coords[3], const double *t) {
+  int64_t N;
+  if (int rc = check_csv_args(path, dims, n, coords, &N)) return rc;
+  if (!t) return fail(LBM_EINVAL, "null argument");
+  static const char *ax[3] = {"x", "y", "z"};
+  std::vector<std::string> names;
+  std::vector<const double *> cols;
+  for (int b = 0; b < dims; b++)
+    for (int a = 0; a < dims; a++) {
+      names.push_back(std::string("component_") + ax[a] + ax[b]);
+      cols.push_back(t + (int64_t)(a + dims * b) * N);
+    }
+  return write_node_csv(path, dims, n, tick, time, coords, names, cols);
 }
 
 int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_t cap, int64_t *count) {

@@ -25,6 +25,8 @@ template <int S>
 hipError_t launch_get_populations(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
+template <int S>
+hipError_t launch_stress(int Q, const AuxArgs &a, const Launch &l);
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);

@@ -99,6 +99,12 @@ hipError_t launch_hydro<S>(int Q, const AuxArgs &a, const Launch &l) {
   return hipGetLastError();
 }
 
+template <>
+hipError_t launch_stress<S>(int Q, const AuxArgs &a, const Launch &l) {
+  LBM_DISPATCH_Q(stress_kernel, a);
+  return hipGetLastError();
+}
+
 #endif
 
 }  // namespace lbm

@@ -1476,26 +1476,23 @@ __global__ void get_populations_kernel(AuxArgs a) {
     a.out64[q * a.g.N + n] = wall ? 0.0 : to_f64<S, Q>(f[pop_index(a.g, Q, q, x, row)], q, a.c.rho0);
 }
 
-// hydroVariablesUpdate! (gpu/fluids.jl:7-37) on the current populations,
-// evaluated in FP64 whatever the storage.
+// hydroVariablesUpdate! (gpu/fluids.jl:7-37, cpu/fluids.jl:7-38) at node n
+// from its populations f (FP64, read from storage): rho and u.
 template <int Q, int S>
-__global__ void hydro_kernel(AuxArgs a) {
+__device__ inline void node_hydro(const AuxArgs &a, int64_t n, int x, int y, int z, double (&f)[Q], double &rho,
+                                  double (&u)[3]) {
 #pragma clang fp contract(off)
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
   constexpr int D = L::D;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.g.N) return;
   const T *fp = static_cast<const T *>(a.f);
-  double f[Q];
-  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
 #pragma unroll
   for (int q = 0; q < Q; q++) f[q] = to_f64<S, Q>(fp[pop_index(a.g, Q, q, x, y + a.g.ny * z)], q, a.c.rho0);
   if (S != kF64 && is_wall_node(a.g, x, y, z)) {
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = 0.0;
   }
-  double rho = f[0];
+  rho = f[0];
 #pragma unroll
   for (int q = 1; q < Q; q++) rho = rho + f[q];
   double mom[3];
@@ -1525,11 +1522,70 @@ __global__ void hydro_kernel(AuxArgs a) {
       mom[k] = mom[k] + half_dt * Fk;
     }
   }
+  u[0] = u[1] = u[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) u[k] = rho != 0.0 ? mom[k] / rho : 0.0;
+}
+
+// hydroVariablesUpdate! (gpu/fluids.jl:7-37) on the current populations,
+// evaluated in FP64 whatever the storage.
+template <int Q, int S>
+__global__ void hydro_kernel(AuxArgs a) {
+  constexpr int D = Lattice<Q>::D;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+  double f[Q], rho, u[3];
+  node_hydro<Q, S>(a, n, x, y, z, f, rho, u);
   if (a.out64) a.out64[n] = rho;
   if (a.uout) {
 #pragma unroll
-    for (int k = 0; k < D; k++) a.uout[k * a.g.N + n] = rho != 0.0 ? mom[k] / rho : 0.0;
+    for (int k = 0; k < D; k++) a.uout[k * a.g.N + n] = u[k];
   }
+}
+
+// viscousStressTensor (cpu/fluids.jl:507-520) on the current populations, in
+// FP64: sigma_ab = -(1 - dt/(2 tau)) sum_i c_ia c_ib (f_i - feq_i), with
+// feq from rho and u as equilibriumDistribution builds it
+// (cpu/fluids.jl:46-63) and the integer c_i in the sum.  a.out64 receives
+// sigma as (N, D, D), column major: sigma_ab of node n at n + N (a + D b).
+template <int Q, int S>
+__global__ void stress_kernel(AuxArgs a) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.g.N) return;
+  const int x = n % a.g.nx, y = (n / a.g.nx) % a.g.ny, z = n / ((int64_t)a.g.nx * a.g.ny);
+  double f[Q], rho, u[3];
+  node_hydro<Q, S>(a, n, x, y, z, f, rho, u);
+  double uu = u[0] * u[0];
+#pragma unroll
+  for (int k = 1; k < D; k++) uu = uu + u[k] * u[k];
+  const double coeff = 1.0 - 0.5 * a.c.dt / a.c.tau;
+  double sig[D][D];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int c[3] = {L::cx[q], L::cy[q], L::cz[q]};
+    double udc = u[0] * (static_cast<double>(c[0]) * a.c.ls);
+#pragma unroll
+    for (int k = 1; k < D; k++) udc = udc + u[k] * (static_cast<double>(c[k]) * a.c.ls);
+    const double first = udc * a.c.invC2s + udc * udc * (0.5 * a.c.invC4s);
+    const double second = first - uu * (0.5 * a.c.invC2s);
+    const double feq = L::w(q) * ((second + 1.0) * rho);
+    const double neq = f[q] - feq;
+#pragma unroll
+    for (int al = 0; al < D; al++)
+#pragma unroll
+      for (int be = 0; be < D; be++) {
+        const double t = static_cast<double>(c[al] * c[be]) * neq;
+        sig[al][be] = q == 0 ? t : sig[al][be] + t;
+      }
+  }
+#pragma unroll
+  for (int al = 0; al < D; al++)
+#pragma unroll
+    for (int be = 0; be < D; be++) a.out64[n + a.g.N * (al + D * be)] = -coeff * sig[al][be];
 }
 
 // Link bits of one particle over its box (for lbm_get_links):

@@ -10,10 +10,12 @@ from .model import (DEFAULT, LBMbatch, LBMmodel, ResolvedModel, addBead, addSqui
                     resolve_particle, sphere_count)
 from .interactions import (addBistableBond, addDipoles, addLennardJones, addLinearBond, addPolarBond,
                            addWCA)
+from .rheology import viscousStressTensor
+from .savedata import writeTensor
 
 __all__ = [
     "DEFAULT", "LBMError", "LBMbatch", "LBMmodel", "LIB_PATH", "ResolvedModel", "addBead", "addSquirmer",
     "addBistableBond", "addDipoles", "addLennardJones", "addLinearBond", "addPolarBond", "addWCA",
     "hydroVariablesUpdate", "julia_range", "LBMpropagate", "modelInit", "propagate_ticks",
-    "resolve_model", "resolve_particle", "sphere_count",
+    "resolve_model", "resolve_particle", "sphere_count", "viscousStressTensor", "writeTensor",
 ]

@@ -91,6 +91,9 @@ SIGNATURES = {
     "lbm_get_trajectory": [_vp, C.c_int32, C.c_int64, C.c_int64, _dp],
     "lbm_write_fluid_csv": [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.c_int64, C.c_double,
                             C.POINTER(_dp), _dp, _dp, C.c_int32, _dp],
+    "lbm_write_tensor_csv": [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.c_int64, C.c_double,
+                             C.POINTER(_dp), _dp],
+    "lbm_viscous_stress": [_vp, _dp],
     "lbm_run": [_vp, C.c_int64],
     "lbm_sync": [_vp],
     "lbm_get_tick": [_vp, C.POINTER(C.c_int64), _dp],

@@ -88,6 +88,26 @@ def write_fluid(model) -> None:
                                                  _capi.as_dptr(u), rm.Q, _capi.as_dptr(f)))
 
 
+def writeTensor(model, T, name: str) -> None:
+    """writeTensor (gpu/aux.jl:364-400): output.lbm/<name>.csv of a dims x
+    dims tensor field T (T[a][b] an array over the lattice), with the node
+    indices and coordinates."""
+    rm = model.resolved
+    d = rm.dims
+    os.makedirs(OUTPUT_DIR, exist_ok=True)
+    shp = tuple(rm.n[:d])
+    t = np.zeros(shp + (d, d), order="F")
+    for a in range(d):
+        for b in range(d):
+            assert np.shape(T[a][b]) == shp, f"tensor components must have the lattice's shape {shp}"
+            t[..., a, b] = T[a][b]
+    n = (C.c_int64 * 3)(*[int(rm.n[k]) if k < d else 1 for k in range(3)])
+    coords = [np.ascontiguousarray(rm.coords[k], dtype=np.float64) for k in range(d)]
+    cptr = (_capi._dp * 3)(*[c.ctypes.data_as(_capi._dp) for c in coords], *([None] * (3 - d)))
+    _capi.check(model._fn["lbm_write_tensor_csv"](_out(f"{name}.csv").encode(), d, n, model.tick, model.time, cptr,
+                                                  _capi.as_dptr(t)))
+
+
 def _particle_header(d: int) -> str:
     cols = ["tick", "time", "particleId"]
     cols += [f"position_{a}" for a in _AX[:d]] + [f"velocity_{a}" for a in _AX[:d]]

@@ -196,6 +196,13 @@ class OracleModel:
     def fluidVelocity(self):
         return self._hydro()[1]
 
+    def viscous_stress(self):
+        """sigma as (N..., d, d), column major."""
+        rm = self.resolved
+        out = np.zeros(tuple(rm.n[: rm.dims]) + (rm.dims, rm.dims), order="F")
+        _check(self._fn["lbm_viscous_stress"](self._h, _capi.as_dptr(out)))
+        return out
+
 
 def oracle_model(**kw) -> OracleModel:
     return OracleModel(resolve_model(**kw))

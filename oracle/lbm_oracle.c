@@ -20,6 +20,7 @@
  *   bulkVelocity          cpu/particles.jl:20-43
  *   moveParticles!        gpu/particles.jl:7-92     cpu/particles.jl:45-100
  *   molecularDynamicsTick!, applyInteraction!  gpu/particles.jl:100-197
+ *   viscousStressTensor   cpu/fluids.jl:507-520 (nonEquilibriumDistribution :65-67)
  *   findInitialConditions gpu/initMethods.jl:417-438
  *   wallNodes             gpu/misc.jl:7-31
  *   D2Q9, D3Q19           gpu/misc.jl:195-210,230-248
@@ -972,6 +973,41 @@ int lbmo_set_interactions(lbmo_model *m, const lbm_interaction *terms, int32_t n
       if (terms[t].ids[k] < 0 || terms[t].ids[k] >= m->np) return fail(LBM_EINVAL, "bad particle id");
   if (n > 0) memcpy(m->inter, terms, sizeof(lbm_interaction) * (size_t)n);
   m->ninter = n;
+  return LBM_OK;
+}
+
+/* viscousStressTensor (cpu/fluids.jl:507-520) of the current populations:
+ * the non-equilibrium populations f_q - feq_q(rho, u) of every q
+ * (nonEquilibriumDistribution, :65-67, with equilibriumDistribution's
+ * compressible form, :46-63), then for every (a, b) the sum over q of
+ * (c_qa c_qb) f_neq_q, in q order, times -(1 - 0.5 dt / tau).  out is
+ * (N, d, d), column major. */
+int lbmo_viscous_stress(lbmo_model *m, double *out) {
+  const int d = m->d, Q = m->Q;
+  const int64_t N = m->N;
+  hydro_update(m);
+  double *neq = (double *)xcalloc((size_t)(Q * N), sizeof(double));
+  if (!neq) return fail(LBM_ENOMEM, "out of memory");
+  for (int q = 0; q < Q; q++)
+    for (int64_t x = 0; x < N; x++) {
+      double u[3];
+      for (int k = 0; k < d; k++) u[k] = m->u[(int64_t)k * N + x];
+      neq[(int64_t)q * N + x] = m->f[(int64_t)q * N + x] - feq_one(m, q, m->rho[x], u);
+    }
+  const double coeff = 1 - 0.5 * m->cfg.dt / m->cfg.tau;
+  for (int a = 0; a < d; a++)
+    for (int b = 0; b < d; b++) {
+      double *o = out + N * (a + d * b);
+      for (int64_t x = 0; x < N; x++) {
+        double s = 0.0;
+        for (int q = 0; q < Q; q++) {
+          const double t = (double)(m->c[q][a] * m->c[q][b]) * neq[(int64_t)q * N + x];
+          s = q == 0 ? t : s + t;
+        }
+        o[x] = -coeff * s;
+      }
+    }
+  free(neq);
   return LBM_OK;
 }
 
