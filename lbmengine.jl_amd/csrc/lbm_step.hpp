@@ -795,9 +795,6 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const StepParticles &sp = a.sp;
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  // the box blocks' chain of dependent loads is the kernel's critical path:
-  // let their waves issue ahead of the tiles' on a shared CU
-  __builtin_amdgcn_s_setprio(3);
 
   // The particle records of this tick, in LDS (read throughout), with the
   // regions this kernel's box blocks own (from the records as read); then
