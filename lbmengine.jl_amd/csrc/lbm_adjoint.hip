@@ -5,15 +5,25 @@
 
 namespace lbm {
 
-// Fixed-order sum of the blocks' partials: out[p][k] = sum_b partial[b][p][k].
+// Fixed-order sum of the blocks' partials: out[p][k] = sum_b partial[b][p][k],
+// one block per (p, k): each thread sums a strided share in double-double,
+// then the waves and the block combine in a fixed order.
 __global__ void __launch_bounds__(256) adjoint_reduce_kernel(const double *partial, int64_t nblocks, int np,
                                                             double *out) {
+  const int pk = blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
-  for (int pk = wave; pk < np * kAdjSums; pk += (int)(blockDim.x >> 6)) {
-    DD s{0.0, 0.0};
-    for (int64_t b = lane; b < nblocks; b += 64) s = dd_merge(s, DD{partial[b * np * kAdjSums + pk], 0.0});
-    s = wave_sum(s);
-    if (lane == kWaveSumLane) out[pk] = s.hi + s.lo;
+  const int64_t stride = (int64_t)np * kAdjSums;
+  DD s{0.0, 0.0};
+#pragma unroll 4
+  for (int64_t b = tid; b < nblocks; b += 256) s = dd_merge(s, DD{partial[b * stride + pk], 0.0});
+  s = wave_sum(s);
+  __shared__ DD part[4];
+  if (lane == kWaveSumLane) part[wave] = s;
+  __syncthreads();
+  if (tid == 0) {
+    DD t = part[0];
+    for (int w = 1; w < 4; w++) t = dd_merge(t, part[w]);
+    out[pk] = t.hi + t.lo;
   }
 }
 
@@ -43,7 +53,8 @@ hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const
 }
 
 hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream) {
-  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(1), dim3(256), 0, stream, partial, nblocks, np, out);
+  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(np * kAdjSums)), dim3(256), 0, stream, partial, nblocks, np,
+                     out);
   return hipGetLastError();
 }
 

@@ -1770,13 +1770,13 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   char *ck = nullptr, *seg = nullptr, *fin = nullptr;
   void *lam[2] = {nullptr, nullptr};
   double *gF = nullptr, *partial = nullptr, *sums_d = nullptr;
-  DevParticle *recs_d = nullptr;
+  DevParticle *recs_seg = nullptr;  // the records of every tick of a segment
   if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess || buf.alloc(&seg, (size_t)seg_len * pop_bytes) != hipSuccess ||
       buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
       buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
       buf.alloc(&partial, (size_t)nblocks * np * kAdjSums * sizeof(double)) != hipSuccess ||
       buf.alloc(&sums_d, (size_t)np * kAdjSums * sizeof(double)) != hipSuccess ||
-      buf.alloc(&recs_d, LBM_MAX_PARTICLES * sizeof(DevParticle)) != hipSuccess)
+      buf.alloc(&recs_seg, (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
   std::vector<std::vector<DevParticle>> ck_recs(nck, std::vector<DevParticle>(np));
   std::vector<std::vector<DevParticle>> seg_recs(seg_len + 1, std::vector<DevParticle>(np));
@@ -1797,24 +1797,44 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   const int64_t ws = T - Wn;  // V_t for t in (ws, T]
   std::vector<double> speeds;
   std::vector<std::array<double, 3>> vel;
+  // the window's velocities come from a per-tick log of the moves (the
+  // handle's own log, if any, is set aside meanwhile)
+  double *ring = nullptr;
+  if (buf.alloc(&ring, sizeof(double) * 9 * LBM_MAX_PARTICLES * (size_t)Wn) != hipSuccess)
+    return fail(LBM_ENOMEM, "hipMalloc adjoint speed log");
+  struct LogSwap {
+    lbm_handle *h;
+    double *d;
+    int64_t cap, from;
+    ~LogSwap() {
+      h->traj_d = d;
+      h->traj_cap = cap;
+      h->traj_from = from;
+      drop_graph(h);
+    }
+  } swap{h, h->traj_d, h->traj_cap, h->traj_from};
+  h->traj_d = ring;
+  h->traj_cap = Wn;
+  h->traj_from = h->tick;
+  drop_graph(h);
+  const int64_t tick0 = h->tick;
   for (int64_t t = 0; t < T;) {
     if (t % K == 0) {
       HIP_TRY(hipMemcpyAsync(ck + (size_t)(t / K) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice, h->stream),
               "save checkpoint");
       HIP_TRY(save_recs(ck_recs[t / K]), "save checkpoint records");
     }
-    if (t >= ws) {
-      if (const int rc = run(1)) return rc;
-      std::vector<DevParticle> r(np);
-      HIP_TRY(save_recs(r), "read particles");
-      std::array<double, 3> v{r[sw].V[0], r[sw].V[1], r[sw].V[2]};
+    const int64_t n = std::min((t / K + 1) * K, T) - t;
+    if (const int rc = run(n)) return rc;
+    t += n;
+  }
+  {
+    std::vector<double> lg((size_t)Wn * 9);
+    if (const int rc = lbm_get_trajectory(h, sw, tick0 + ws + 1, Wn, lg.data())) return rc;
+    for (int64_t i = 0; i < Wn; i++) {
+      std::array<double, 3> v{lg[i * 9 + 3], lg[i * 9 + 4], lg[i * 9 + 5]};
       vel.push_back(v);
       speeds.push_back(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-      t++;
-    } else {
-      const int64_t n = std::min((t / K + 1) * K, ws) - t;
-      if (const int rc = run(n)) return rc;
-      t += n;
     }
   }
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
@@ -1867,7 +1887,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.Ffield = h->F;
   aa.g = geom(h);
   aa.c = coll(h);
-  aa.parts = recs_d;
   for (int k = 0; k < 3; k++) aa.coords[k] = h->coords_d[k];
   aa.partial = partial;
   aa.np = np;
@@ -1879,19 +1898,39 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.shell = 1.5 * h->cfg.dx;
   std::vector<double> sums((size_t)np * kAdjSums);
   const Launch l{dim3((unsigned)nblocks), dim3(threads), h->stream};
+  std::vector<DevParticle> seg_host((size_t)(seg_len + 1) * np);
   for (int64_t c = nck - 1; c >= 0; c--) {
-    const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
-    HIP_TRY(hipMemcpyAsync(h->f[h->cur], ck + (size_t)c * pop_bytes, pop_bytes, hipMemcpyDeviceToDevice, h->stream),
-            "restore checkpoint");
+    const int64_t t0 = c * K, t1 = std::min(t0 + K, T), len = t1 - t0;
+    // S_k of the segment's ticks: the checkpoint itself, then slots 1.. of
+    // seg; slot 0 takes the last tick's output, which no tick reads
+    auto Sk = [&](int64_t j) -> void * {
+      return j == 0 ? static_cast<void *>(ck + (size_t)c * pop_bytes)
+                    : static_cast<void *>(seg + (size_t)(j % seg_len) * pop_bytes);
+    };
+    // recompute the segment tick by tick (collide, gather, move), each
+    // tick's output in its own slot and each tick's records on the device
     HIP_TRY(load_recs(ck_recs[c]), "restore checkpoint records");
-    for (int64_t k = t0; k < t1; k++) {
-      HIP_TRY(hipMemcpyAsync(seg + (size_t)(k - t0) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice,
-                             h->stream),
-              "keep segment state");
-      HIP_TRY(save_recs(seg_recs[k - t0]), "keep segment records");
-      if (const int rc = run(1)) return rc;
+    HIP_TRY(hipMemcpyAsync(recs_seg, parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToDevice,
+                           h->stream),
+            "keep segment records");
+    for (int64_t j = 0; j < len; j++) {
+      hipError_t e = launch_mode(h, kCollide, Sk(j), h->f[h->cur ^ 1], h->pcur);
+      if (e == hipSuccess) e = launch_mode(h, kGather, h->f[h->cur ^ 1], Sk(j + 1), h->pcur);
+      if (e != hipSuccess) return hip_fail(e, "segment step launch");
+      h->pcur ^= 1;
+      StepArgs fa = step_args(h, nullptr, nullptr, h->pcur);
+      fa.sp.parts_out = parts_of(h, h->pcur);  // in place
+      e = launch_finalize(fa.sp, h->stream);
+      if (e != hipSuccess) return hip_fail(e, "finalize launch");
+      HIP_TRY(hipMemcpyAsync(recs_seg + (size_t)(j + 1) * np, parts_of(h, h->pcur), np * sizeof(DevParticle),
+                             hipMemcpyDeviceToDevice, h->stream),
+              "keep segment records");
     }
-    HIP_TRY(save_recs(seg_recs[t1 - t0]), "keep segment records");
+    HIP_TRY(copy_sync(h, seg_host.data(), recs_seg, (size_t)(len + 1) * np * sizeof(DevParticle),
+                      hipMemcpyDeviceToHost),
+            "read segment records");
+    for (int64_t j = 0; j <= len; j++)
+      for (int p = 0; p < np; p++) seg_recs[j][p] = seg_host[(size_t)j * np + p];
     for (int64_t k = t1 - 1; k >= t0; k--) {
       const std::vector<DevParticle> &Pk = seg_recs[k - t0], &Pk1 = seg_recs[k - t0 + 1];
       std::vector<std::array<double, kPState>> mu_in(np);
@@ -1905,9 +1944,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
           aa.tau[p][k2] = tau[k2];
         }
       }
-      HIP_TRY(hipMemcpyAsync(recs_d, Pk.data(), np * sizeof(DevParticle), hipMemcpyHostToDevice, h->stream),
-              "upload records");
-      aa.S = seg + (size_t)(k - t0) * pop_bytes;
+      aa.parts = recs_seg + (size_t)(k - t0) * np;
+      aa.S = Sk(k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
       hipError_t e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
