@@ -1,6 +1,7 @@
 // lbm_adjoint.hip — launchers of the adjoint tick kernel (D3Q19; every
 // storage, collision and force kind) and of its fixed-order reduction.
 #include "lbm_adjoint.hpp"
+#include "lbm_adjoint_move.hpp"
 #include "lbm_dispatch.hpp"
 
 namespace lbm {
@@ -27,6 +28,57 @@ __global__ void __launch_bounds__(256) adjoint_reduce_kernel(const double *parti
   }
 }
 
+__global__ void __launch_bounds__(64) adjoint_move_kernel(AdjMoveArgs a) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x, j = threadIdx.x;
+  if (j >= kDualN) return;
+  const DevParticle P = a.recs_k[p];
+  const DevParticle &P1 = a.recs_k1[p];
+  const double F[3] = {P1.force[0], P1.force[1], P1.force[2]};
+  const double Tq[3] = {P1.torque[0], P1.torque[1], P1.torque[2]};
+  DualStateT<1> s = move_dual_t<1>(P, F, Tq, a.dt, j);
+  double g = 0.0;
+  for (int o = 0; o < kPState; o++) {
+    const double m = a.mu[p * kPState + o];
+    if (m != 0.0) g += m * s.slot(o)->d[0];
+  }
+  if (j < kPState)
+    a.mu_in[p * kPState + j] = g;
+  else
+    a.phitau[p * 6 + (j - kPState)] = g;
+}
+
+__global__ void __launch_bounds__(256) adjoint_finish_kernel(AdjFinishArgs a) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  if (t >= a.np * kPState) return;
+  const int p = t / kPState, i = t % kPState;
+  const double *sm = a.sums + p * kAdjSums;
+  double v = a.mu_in[t];
+  int si = -1;
+  if (i < kStR + 3)
+    si = kAdjR + i;
+  else if (i >= kStNR && i < kStNR + 3)
+    si = kAdjNR + (i - kStNR);
+  else if (i >= kStNV && i < kStNV + 3)
+    si = kAdjNV + (i - kStNV);
+  else if (i >= kStNW && i < kStNW + 3)
+    si = kAdjNW + (i - kStNW);
+  else if (i >= kStNE && i < kStNE + 3)
+    si = kAdjNE + (i - kStNE);
+  if (si >= 0) v = v + sm[si];
+  if (a.direct && p == a.sw && i >= kStV && i < kStV + 3) {
+    const double *V = a.recs_k[p].V;
+    const double sp = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    if (sp > 0.0) v = v + (0.0 + a.gscale * V[i - kStV] / sp);
+  }
+  a.mu[t] = v;
+  if (i == 0) {
+    a.gB[p * 2] += sm[kAdjB1];
+    a.gB[p * 2 + 1] += sm[kAdjB2];
+  }
+}
+
 template <int S, int COLL>
 static hipError_t launch_adjoint_sc(int fm, const AdjArgs &a, const Launch &l) {
   switch (fm) {
@@ -50,6 +102,16 @@ hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const
     case LBM_STORAGE_F32: return launch_adjoint_s<kF32>(coll, fm, a, l);
     default: return launch_adjoint_s<kF16>(coll, fm, a, l);
   }
+}
+
+hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream) {
+  hipLaunchKernelGGL(adjoint_move_kernel, dim3((unsigned)np), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_finish(const AdjFinishArgs &a, hipStream_t stream) {
+  hipLaunchKernelGGL(adjoint_finish_kernel, dim3(1), dim3(256), 0, stream, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream) {

@@ -138,7 +138,28 @@ struct AdjArgs {
   int32_t np;
   double coeff, ls;     // momentum-exchange coefficient and lattice speed
   double shell;         // 1.5 dx: link nodes lie within r + shell of a centre
-  double phi[LBM_MAX_PARTICLES][3], tau[LBM_MAX_PARTICLES][3];  // dL/dF_p, dL/dT_p
+  const double *phitau;  // [particle][6]: dL/dF_p (0..2) and dL/dT_p (3..5) of the tick's move
+};
+
+// The reverse of one tick's particle moves, on the device (block p, lane j <
+// kDualN carries the tangent of input j), and the update of the particles'
+// adjoint state once the tick's adjoint sums are in.
+struct AdjMoveArgs {
+  const DevParticle *recs_k;   // records at the start of tick k
+  const DevParticle *recs_k1;  // after its move (force, torque consumed)
+  const double *mu;            // [np][kPState]: dL/d(state after the move)
+  double *mu_in;               // [np][kPState]: dL/d(state before it), from the move alone
+  double *phitau;              // [np][6]
+  double dt;
+};
+struct AdjFinishArgs {
+  const double *sums;          // [np][kAdjSums] of tick k's adjoint kernel
+  const double *mu_in;
+  double *mu;                  // [np][kPState] <- dL/d(state at the start of tick k)
+  double *gB;                  // [np][2] += dL/dB1, dL/dB2
+  const DevParticle *recs_k;
+  int32_t np, sw, direct;      // direct: V_k is one of the averaged speeds
+  double gscale;               // 2 (mean - target) / window
 };
 
 // Squared distance of node (x, y, z) from Rg in getSphere's order, and the
@@ -268,8 +289,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
         const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
                               arm[0] * cq[1] - arm[1] * cq[0]};
         const double sh = -a.coeff * a.ls *
-                          (a.phi[p][0] * cq[0] + a.phi[p][1] * cq[1] + a.phi[p][2] * cq[2] + a.tau[p][0] * cr[0] +
-                           a.tau[p][1] * cr[1] + a.tau[p][2] * cr[2]);
+                          (a.phitau[p * 6 + 0] * cq[0] + a.phitau[p * 6 + 1] * cq[1] + a.phitau[p * 6 + 2] * cq[2] + a.phitau[p * 6 + 3] * cr[0] +
+                           a.phitau[p * 6 + 4] * cr[1] + a.phitau[p * 6 + 5] * cr[2]);
         shat_sum += sh;
       }
       // dL/dfs = lam + 2 sum shat; dL/dv_j = lam + sum_{i >= j} shat_i
@@ -287,8 +308,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
         const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
                               arm[0] * cq[1] - arm[1] * cq[0]};
         pre += -a.coeff * a.ls *
-               (a.phi[p][0] * cq[0] + a.phi[p][1] * cq[1] + a.phi[p][2] * cq[2] + a.tau[p][0] * cr[0] +
-                a.tau[p][1] * cr[1] + a.tau[p][2] * cr[2]);
+               (a.phitau[p * 6 + 0] * cq[0] + a.phitau[p * 6 + 1] * cq[1] + a.phitau[p * 6 + 2] * cq[2] + a.phitau[p * 6 + 3] * cr[0] +
+                a.phitau[p * 6 + 4] * cr[1] + a.phitau[p * 6 + 5] * cr[2]);
       }
     }
   }
@@ -358,8 +379,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
           const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
                                 arm[0] * cq[1] - arm[1] * cq[0]};
           const double sh = -a.coeff * a.ls *
-                            (a.phi[i][0] * cq[0] + a.phi[i][1] * cq[1] + a.phi[i][2] * cq[2] + a.tau[i][0] * cr[0] +
-                             a.tau[i][1] * cr[1] + a.tau[i][2] * cr[2]);
+                            (a.phitau[i * 6 + 0] * cq[0] + a.phitau[i * 6 + 1] * cq[1] + a.phitau[i * 6 + 2] * cq[2] + a.phitau[i * 6 + 3] * cr[0] +
+                             a.phitau[i * 6 + 4] * cr[1] + a.phitau[i * 6 + 5] * cr[2]);
           if (i < p) pre += sh;
           if (i == p) s_p = fs + v + shift;
           shat_sum += sh;
@@ -370,7 +391,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
                                 aj * lc * (double)rho * cq[2] * a.ls};
         node_velocity_vjp(xs, Pp, ubar, acc);
         // torque arm: d/dR of -coeff ls tau . ((X - R) x c) s = coeff ls s (c x tau)
-        const double *tp = a.tau[p];
+        const double *tp = a.phitau + p * 6 + 3;
         const double cxt[3] = {cq[1] * tp[2] - cq[2] * tp[1], cq[2] * tp[0] - cq[0] * tp[2],
                                cq[0] * tp[1] - cq[1] * tp[0]};
         for (int k = 0; k < 3; k++) acc[kAdjR + k] += a.coeff * a.ls * s_p * cxt[k];

@@ -1738,9 +1738,6 @@ hipError_t copy_sync(lbm_handle *h, void *dst, const void *src, size_t bytes, hi
   return e;
 }
 
-// Indices of the particle state in the adjoint's vectors (lbm_adjoint_move.hpp).
-enum { kStR = 0, kStV = 3, kStW = 6, kStE = 9, kStNR = 12, kStNV = 15, kStNW = 18, kStNE = 21 };
-
 }  // namespace
 
 extern "C" {
@@ -1779,7 +1776,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       buf.alloc(&recs_seg, (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
   std::vector<std::vector<DevParticle>> ck_recs(nck, std::vector<DevParticle>(np));
-  std::vector<std::vector<DevParticle>> seg_recs(seg_len + 1, std::vector<DevParticle>(np));
   std::vector<DevParticle> fin_recs(np);
   auto save_recs = [&](std::vector<DevParticle> &dst) {
     return copy_sync(h, dst.data(), parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToHost);
@@ -1877,10 +1873,24 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     direct(fin_recs[sw], m);
     for (int i = 0; i < kPState; i++) mu[sw][i] = m[i];
   }
-  std::vector<double> gB1(np, 0.0), gB2(np, 0.0);
+  // the particles' adjoint state and the B1, B2 gradients live on the device:
+  // the reverse sweep never waits for the host
+  double *mu_d = nullptr, *muin_d = nullptr, *pt_d = nullptr, *gB_d = nullptr;
+  if (buf.alloc(&mu_d, (size_t)np * kPState * sizeof(double)) != hipSuccess ||
+      buf.alloc(&muin_d, (size_t)np * kPState * sizeof(double)) != hipSuccess ||
+      buf.alloc(&pt_d, (size_t)np * 6 * sizeof(double)) != hipSuccess ||
+      buf.alloc(&gB_d, (size_t)np * 2 * sizeof(double)) != hipSuccess)
+    return fail(LBM_ENOMEM, "hipMalloc adjoint particle state");
+  {
+    std::vector<double> mu0((size_t)np * kPState);
+    for (int p = 0; p < np; p++)
+      for (int i = 0; i < kPState; i++) mu0[(size_t)p * kPState + i] = mu[p][i];
+    HIP_TRY(copy_sync(h, mu_d, mu0.data(), mu0.size() * sizeof(double), hipMemcpyHostToDevice), "upload adjoint state");
+  }
   int a = 0;
   HIP_TRY(hipMemsetAsync(lam[a], 0, N * h->Q * rsize, h->stream), "clear adjoint");
   HIP_TRY(hipMemsetAsync(gF, 0, N * 3 * sizeof(double), h->stream), "clear force gradient");
+  HIP_TRY(hipMemsetAsync(gB_d, 0, (size_t)np * 2 * sizeof(double), h->stream), "clear B gradients");
   AdjArgs aa;
   std::memset(&aa, 0, sizeof aa);
   aa.gF = gF;
@@ -1896,9 +1906,21 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   }
   aa.ls = h->cfg.dx / h->cfg.dt;
   aa.shell = 1.5 * h->cfg.dx;
-  std::vector<double> sums((size_t)np * kAdjSums);
+  aa.phitau = pt_d;
+  AdjMoveArgs ma{};
+  ma.mu = mu_d;
+  ma.mu_in = muin_d;
+  ma.phitau = pt_d;
+  ma.dt = h->cfg.dt;
+  AdjFinishArgs fa2{};
+  fa2.sums = sums_d;
+  fa2.mu_in = muin_d;
+  fa2.mu = mu_d;
+  fa2.gB = gB_d;
+  fa2.np = np;
+  fa2.sw = sw;
+  fa2.gscale = gscale;
   const Launch l{dim3((unsigned)nblocks), dim3(threads), h->stream};
-  std::vector<DevParticle> seg_host((size_t)(seg_len + 1) * np);
   for (int64_t c = nck - 1; c >= 0; c--) {
     const int64_t t0 = c * K, t1 = std::min(t0 + K, T), len = t1 - t0;
     // S_k of the segment's ticks: the checkpoint itself, then slots 1.. of
@@ -1926,53 +1948,31 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
                              hipMemcpyDeviceToDevice, h->stream),
               "keep segment records");
     }
-    HIP_TRY(copy_sync(h, seg_host.data(), recs_seg, (size_t)(len + 1) * np * sizeof(DevParticle),
-                      hipMemcpyDeviceToHost),
-            "read segment records");
-    for (int64_t j = 0; j <= len; j++)
-      for (int p = 0; p < np; p++) seg_recs[j][p] = seg_host[(size_t)j * np + p];
     for (int64_t k = t1 - 1; k >= t0; k--) {
-      const std::vector<DevParticle> &Pk = seg_recs[k - t0], &Pk1 = seg_recs[k - t0 + 1];
-      std::vector<std::array<double, kPState>> mu_in(np);
-      for (int p = 0; p < np; p++) {
-        double mo[kPState], mi[kPState], phi[3], tau[3];
-        for (int i = 0; i < kPState; i++) mo[i] = mu[p][i];
-        move_vjp(Pk[p], Pk1[p].force, Pk1[p].torque, h->cfg.dt, mo, mi, phi, tau);
-        for (int i = 0; i < kPState; i++) mu_in[p][i] = mi[i];
-        for (int k2 = 0; k2 < 3; k2++) {
-          aa.phi[p][k2] = phi[k2];
-          aa.tau[p][k2] = tau[k2];
-        }
-      }
+      // the move's reverse (dL/dF, dL/dT of tick k), the adjoint tick, then
+      // the particles' adjoint state at the start of tick k
+      ma.recs_k = recs_seg + (size_t)(k - t0) * np;
+      ma.recs_k1 = recs_seg + (size_t)(k - t0 + 1) * np;
+      hipError_t e = launch_adjoint_move(ma, np, h->stream);
       aa.parts = recs_seg + (size_t)(k - t0) * np;
       aa.S = Sk(k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
-      hipError_t e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
       if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks, np, sums_d, h->stream);
+      fa2.recs_k = ma.recs_k;
+      fa2.direct = k > ws;  // V_k is one of the averaged speeds
+      if (e == hipSuccess) e = launch_adjoint_finish(fa2, h->stream);
       if (e != hipSuccess) return hip_fail(e, "adjoint launch");
-      HIP_TRY(copy_sync(h, sums.data(), sums_d, sums.size() * sizeof(double), hipMemcpyDeviceToHost), "adjoint sums");
       a ^= 1;
-      for (int p = 0; p < np; p++) {
-        const double *s = &sums[(size_t)p * kAdjSums];
-        gB1[p] += s[kAdjB1];
-        gB2[p] += s[kAdjB2];
-        for (int k2 = 0; k2 < 3; k2++) {
-          mu_in[p][kStNR + k2] += s[kAdjNR + k2];
-          mu_in[p][kStNV + k2] += s[kAdjNV + k2];
-          mu_in[p][kStNW + k2] += s[kAdjNW + k2];
-          mu_in[p][kStNE + k2] += s[kAdjNE + k2];
-          mu_in[p][kStR + k2] += s[kAdjR + k2];
-        }
-        mu[p] = mu_in[p];
-      }
-      if (k > ws) {  // V_k is one of the averaged speeds
-        double m[kPState];
-        for (int i = 0; i < kPState; i++) m[i] = 0.0;
-        direct(Pk[sw], m);
-        for (int i = 0; i < kPState; i++) mu[sw][i] += m[i];
-      }
     }
+  }
+  std::vector<double> gB((size_t)np * 2);
+  HIP_TRY(copy_sync(h, gB.data(), gB_d, gB.size() * sizeof(double), hipMemcpyDeviceToHost), "read B gradients");
+  std::vector<double> gB1(np), gB2(np);
+  for (int p = 0; p < np; p++) {
+    gB1[p] = gB[(size_t)p * 2];
+    gB2[p] = gB[(size_t)p * 2 + 1];
   }
   // the state the forward run reached
   HIP_TRY(hipMemcpyAsync(h->f[h->cur], fin, pop_bytes, hipMemcpyDeviceToDevice, h->stream), "restore final state");

@@ -36,5 +36,9 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
 struct AdjArgs;
 hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l);
 hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream);
+struct AdjMoveArgs;
+struct AdjFinishArgs;
+hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream);
+hipError_t launch_adjoint_finish(const AdjFinishArgs &a, hipStream_t stream);
 
 }  // namespace lbm
