@@ -196,3 +196,26 @@ def test_fp32_batch_against_fp64_boxes(gpu):
         eu = float(np.linalg.norm(a.fluidVelocity - b.fluidVelocity) / np.linalg.norm(b.fluidVelocity))
         va, vb = np.linalg.norm(a.particles[0].velocity), np.linalg.norm(b.particles[0].velocity)
         assert eu < 1e-3 and abs(va - vb) / vb < 1e-3, (eu, va, vb)
+
+
+def test_batch_with_interactions_matches_boxes_run_alone(gpu):
+    """Interaction terms ride in each box's arguments: a batch of bonded
+    boxes ends bit-identical to the boxes advanced alone."""
+    from lbmengine_hip import interactions as it
+    kw = dict(**KW3, xlims=(-16, 15))
+    bead = dict(radius=3.0, coupleForces=True, coupleTorques=True)
+    pairs = []
+    for i, k in enumerate((0.05, 0.02)):
+        a, b = twins(gpu, kw, [("bead", dict(bead, position=[-4.2, 0.3 * i, 0.1])),
+                               ("bead", dict(bead, position=[5.1, -1.2, 0.7]))], seed=80 + i)
+        for m in (a, b):
+            it.addLinearBond(m, 1, 2, equilibriumDisplacement=7.0, stiffness=k)
+            it.addDipoles(m, magneticField=[0.0, 0.3, 0.4], permeability=1.0)
+        pairs.append((a, b))
+    batch = gpu.LBMbatch([a for a, _ in pairs])
+    for steps in (3, 40):
+        gpu.LBMpropagate(batch, ticks=steps)
+        for _, b in pairs:
+            gpu.LBMpropagate(b, ticks=steps)
+        for a, b in pairs:
+            same(a, b)
