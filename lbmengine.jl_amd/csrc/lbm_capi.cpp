@@ -101,6 +101,8 @@ BoxGeom geom(const lbm_handle *h) {
   g.nz = h->n[2];
   g.walled = h->cfg.walled;
   g.N = h->N;
+  g.off32 = (uint64_t)h->N * (uint64_t)h->Q * elem_size(h->cfg.storage) <= 0xFFFFFFFFull;
+  g.reserved = 0;
   return g;
 }
 
@@ -115,6 +117,7 @@ CollParams coll(const lbm_handle *h) {
   c.rho0 = h->cfg.rho0;
   for (int k = 0; k < 3; k++) c.F[k] = h->cfg.force == LBM_FORCE_UNIFORM ? h->cfg.force_uniform[k] : 0.0;
   c.force_mode = h->cfg.force;
+  set_float_params(c);
   return c;
 }
 
@@ -127,6 +130,8 @@ Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
   t.ty = (h->n[1] + by - 1) / by;
   t.tz = (h->n[2] + bz - 1) / bz;
   t.ntiles = t.tx * t.ty * t.tz;
+  t.mtx = div_magic((uint32_t)t.tx, (uint64_t)t.ntiles);
+  t.mty = div_magic((uint32_t)t.ty, (uint64_t)t.ntiles / (uint64_t)t.tx + 1);
   t.remap = 1;
   return t;
 }

@@ -76,6 +76,8 @@ struct BoxGeom {
   int32_t nx, ny, nz;   // nodes per dimension (nz = 1 in 2D)
   int32_t walled;       // bit k: dimension k walled
   int64_t N;            // nx*ny*nz
+  int32_t off32;        // 1: every population's byte offset in a buffer fits in 32 bits
+  int32_t reserved;
 };
 
 struct CollParams {
@@ -84,7 +86,21 @@ struct CollParams {
   double rho0;                 // deviation shift
   double F[3];                 // uniform Guo force
   int32_t force_mode;
+  // the FP32 collisions' constants, each rounded once from the above
+  // (set_float_params), so that no thread repeats the conversions
+  float f_ls, f_invC2s, f_invC4s, f_dt, f_half_dt, f_wp, f_wm, f_rho0;
 };
+
+__host__ __device__ inline void set_float_params(CollParams &c) {
+  c.f_ls = static_cast<float>(c.ls);
+  c.f_invC2s = static_cast<float>(c.invC2s);
+  c.f_invC4s = static_cast<float>(c.invC4s);
+  c.f_dt = static_cast<float>(c.dt);
+  c.f_half_dt = static_cast<float>(0.5 * c.dt);
+  c.f_wp = static_cast<float>(c.dt / c.tau);
+  c.f_wm = static_cast<float>(c.dt / c.tau_minus);
+  c.f_rho0 = static_cast<float>(c.rho0);
+}
 
 // Tile of the step kernel: block of bx*by*bz threads, one node each.
 struct Tiling {
@@ -92,7 +108,25 @@ struct Tiling {
   int32_t tx, ty, tz;   // tiles per dimension
   int32_t ntiles;
   int32_t remap;        // 1: xcd_remap the tile order
+  uint32_t mtx, mty;    // tile_div multipliers for tx and ty (0: divide)
 };
+
+// Multiplier m with n / d == umulhi(n, m) for every n < nmax, or 0 when
+// there is none of 32 bits (then tile_div divides).  m = ceil(2^32 / d)
+// gives n m / 2^32 = n / d + n e / (d 2^32) with e = m d - 2^32 < d, whose
+// fractional excess stays below 1 / d while n d < 2^32.
+inline uint32_t div_magic(uint32_t d, uint64_t nmax) {
+  if (d <= 1) return 0;
+  const uint64_t m = ((uint64_t(1) << 32) + d - 1) / d;
+  if (m > 0xFFFFFFFFull || nmax * d >= (uint64_t(1) << 32)) return 0;
+  return (uint32_t)m;
+}
+
+// n / d for a tile index (uniform over the block): a scalar multiply-high
+// instead of an integer division
+__device__ __forceinline__ uint32_t tile_div(uint32_t n, int32_t d, uint32_t m) {
+  return m ? __umulhi(n, m) : (d == 1 ? n : n / (uint32_t)d);
+}
 
 __host__ __device__ inline int32_t xcd_remap(int32_t b, int32_t nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs (each with its own

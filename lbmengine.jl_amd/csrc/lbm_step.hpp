@@ -241,9 +241,9 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
                                             float &rho_out) {
   using L = Lattice<Q>;
   constexpr int D = L::D;
-  const float ls = static_cast<float>(c.ls);
-  const float invC2s = static_cast<float>(c.invC2s);
-  const float invC4s = static_cast<float>(c.invC4s);
+  const float ls = c.f_ls;
+  const float invC2s = c.f_invC2s;
+  const float invC4s = c.f_invC4s;
   float drho = 0.0f;
 #pragma unroll
   for (int q = 0; q < Q; q++) drho += d[q];
@@ -258,11 +258,11 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
   if constexpr (FM != LBM_FORCE_NONE) {
 #pragma unroll
     for (int k = 0; k < D; k++) F[k] = Fn[k];
-    const float half_dt = static_cast<float>(0.5 * c.dt);
+    const float half_dt = c.f_half_dt;
 #pragma unroll
     for (int k = 0; k < D; k++) j[k] += half_dt * F[k];
   }
-  const float rho = static_cast<float>(c.rho0) + drho;
+  const float rho = c.f_rho0 + drho;
   const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
   float u[3];
 #pragma unroll
@@ -271,9 +271,9 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
   float uu = 0.0f;
 #pragma unroll
   for (int k = 0; k < D; k++) uu += u[k] * u[k];
-  const float dt = static_cast<float>(c.dt);
-  const float wp = static_cast<float>(c.dt / c.tau);
-  const float wm = COLL == LBM_COLLISION_TRT ? static_cast<float>(c.dt / c.tau_minus) : wp;
+  const float dt = c.f_dt;
+  const float wp = c.f_wp;
+  const float wm = COLL == LBM_COLLISION_TRT ? c.f_wm : wp;
   const float gp = 1.0f - 0.5f * wp, gm = 1.0f - 0.5f * wm;
   float uF = 0.0f;
   if constexpr (FM != LBM_FORCE_NONE) {
@@ -332,9 +332,9 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
   constexpr float is0 = static_cast<float>(kF16Scale * L::invw(0));
   constexpr float is1 = static_cast<float>(kF16Scale * L::invw(1));
   constexpr float is2 = static_cast<float>(kF16Scale * L::invw(Q - 1));
-  const float ls = static_cast<float>(c.ls);
-  const float invC2s = static_cast<float>(c.invC2s);
-  const float invC4s = static_cast<float>(c.invC4s);
+  const float ls = c.f_ls;
+  const float invC2s = c.f_invC2s;
+  const float invC4s = c.f_invC4s;
   // class sums of the stored values and of their first moments
   float S1 = 0.0f, S2 = 0.0f, A1[3] = {0.0f, 0.0f, 0.0f}, A2[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -360,14 +360,14 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
   for (int k = 0; k < D; k++) j[k] = fmaf(s2, A2[k], s1 * A1[k]);
   float F[3] = {0.0f, 0.0f, 0.0f};
   if constexpr (FM != LBM_FORCE_NONE) {
-    const float half_dt = static_cast<float>(0.5 * c.dt);
+    const float half_dt = c.f_half_dt;
 #pragma unroll
     for (int k = 0; k < D; k++) {
       F[k] = Fn[k];
       j[k] = fmaf(half_dt, F[k], j[k]);
     }
   }
-  const float rho = static_cast<float>(c.rho0) + drho;
+  const float rho = c.f_rho0 + drho;
   const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
   float u[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -376,9 +376,9 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
   float uu = 0.0f;
 #pragma unroll
   for (int k = 0; k < D; k++) uu = fmaf(u[k], u[k], uu);
-  const float dt = static_cast<float>(c.dt);
-  const float wp = static_cast<float>(c.dt / c.tau);
-  const float wm = COLL == LBM_COLLISION_TRT ? static_cast<float>(c.dt / c.tau_minus) : wp;
+  const float dt = c.f_dt;
+  const float wp = c.f_wp;
+  const float wm = COLL == LBM_COLLISION_TRT ? c.f_wm : wp;
   const float gp = 1.0f - 0.5f * wp, gm = 1.0f - 0.5f * wm;
   float uF = 0.0f;
   if constexpr (FM != LBM_FORCE_NONE) {
@@ -1236,9 +1236,11 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
     b -= a.sp.nbox_pad;
   }
   const int tile = a.t.remap ? xcd_remap(b, a.t.ntiles) : b;
-  const int tpx = tile % a.t.tx;
-  const int tpy = (tile / a.t.tx) % a.t.ty;
-  const int tpz = tile / (a.t.tx * a.t.ty);
+  const uint32_t tq = tile_div((uint32_t)tile, a.t.tx, a.t.mtx);  // tile / tx
+  const uint32_t tqq = tile_div(tq, a.t.ty, a.t.mty);              // tile / (tx ty)
+  const int tpx = tile - (int)tq * a.t.tx;
+  const int tpy = (int)tq - (int)tqq * a.t.ty;
+  const int tpz = (int)tqq;
   const int x = tpx * a.t.bx + threadIdx.x;
   const int y = tpy * a.t.by + threadIdx.y;
   const int z = tpz * a.t.bz + threadIdx.z;
@@ -1271,27 +1273,41 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
       // first and the particles' regions (scalar, one load each) are
       // checked while they are in flight.  A tile that touches none of
       // them finishes here; one that does takes the general path below.
-      const int row = y + g.ny * z;
-      const int64_t own = (int64_t)row * Q * g.nx + x;
-      const T *__restrict__ fin = static_cast<const T *>(a.fin);
-      T *__restrict__ fout = static_cast<T *>(a.fout);
-      const GatherRows gr = gather_rows(g, Q, y, z);
-      const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
-      T raw[Q];
+      // Offsets in I: 32 bits when every population's byte offset fits
+      // (loads and stores then take the buffer's address from scalar
+      // registers and one 32-bit offset), 64 otherwise.
+      auto fast = [&](auto zero) -> bool {
+        using I = decltype(zero);
+        // byte offsets: row (y', z') of population q at x' is
+        // y' Qb + z' ny Qb + q nxb + x' Tb (nxb, Qb uniform)
+        const I Tb = (I)sizeof(T), nxb = (I)g.nx * Tb, Qb = (I)Q * nxb, Zs = (I)g.ny * Qb;
+        const char *__restrict__ fin = static_cast<const char *>(a.fin);
+        char *__restrict__ fout = static_cast<char *>(a.fout);
+        const I ya[3] = {(I)(y == g.ny - 1 ? 0 : y + 1) * Qb, (I)y * Qb, (I)(y == 0 ? g.ny - 1 : y - 1) * Qb};
+        const I zb[3] = {(I)(z == g.nz - 1 ? 0 : z + 1) * Zs, (I)z * Zs, (I)(z == 0 ? g.nz - 1 : z - 1) * Zs};
+        const I xb[3] = {(I)(x == g.nx - 1 ? 0 : x + 1) * Tb, (I)x * Tb, (I)(x == 0 ? g.nx - 1 : x - 1) * Tb};
+        I rb[3][3];
 #pragma unroll
-      for (int q = 0; q < Q; q++)
-        raw[q] = fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]];
-      bool near = false;
-      for (int p = 0; p < a.sp.np; p++) near = near | tile_touches(a.sp.parts_in[p]);
-      if (!near) {
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) rb[i][j] = ya[i] + zb[j];
+        T raw[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+          raw[q] = *reinterpret_cast<const T *>(fin + (rb[L::cy[q] + 1][L::cz[q] + 1] + xb[L::cx[q] + 1] + (I)q * nxb));
+        bool near = false;
+        for (int p = 0; p < a.sp.np; p++) near = near | tile_touches(a.sp.parts_in[p]);
+        if (near) return false;
         R Fn[3], rho;
-        node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)row, Fn);
+        node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)(y + g.ny * z), Fn);
         T out[Q];
         collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
+        const I own = rb[1][1] + xb[1];
 #pragma unroll
-        for (int q = 0; q < Q; q++) fout[own + (int64_t)q * g.nx] = out[q];
-        return;
-      }
+        for (int q = 0; q < Q; q++) *reinterpret_cast<T *>(fout + (own + (I)q * nxb)) = out[q];
+        return true;
+      };
+      if (g.off32 ? fast(uint32_t(0)) : fast(int64_t(0))) return;
     }
   }
   bool boxed = false;  // owned by the box blocks

@@ -39,6 +39,7 @@ static bool check_collision() {
   c.invC2s = 3.0;
   c.invC4s = 9.0;
   c.rho0 = 1.0;
+  set_float_params(c);
   for (int trial = 0; trial < 20; trial++) {
     double f[Q], Fn[3], lam[Q];
     for (int q = 0; q < Q; q++) f[q] = L::w(q) * (1.0 + U(-0.05, 0.05)), lam[q] = U(-1, 1);
