@@ -1772,13 +1772,16 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   char *ck = nullptr, *seg = nullptr, *fin = nullptr;
   void *lam[2] = {nullptr, nullptr};
   double *gF = nullptr, *partial = nullptr, *sums_d = nullptr;
-  DevParticle *recs_seg = nullptr;  // the records of every tick of a segment
-  if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess || buf.alloc(&seg, (size_t)seg_len * pop_bytes) != hipSuccess ||
+  DevParticle *recs_seg = nullptr;  // the records of every tick of a segment (two sets)
+  DevParticle *ck_recs_d = nullptr;  // the checkpoints' records
+  if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess ||
+      buf.alloc(&seg, 2 * (size_t)seg_len * pop_bytes) != hipSuccess ||
       buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
       buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
       buf.alloc(&partial, (size_t)nblocks * np * kAdjSums * sizeof(double)) != hipSuccess ||
       buf.alloc(&sums_d, (size_t)np * kAdjSums * sizeof(double)) != hipSuccess ||
-      buf.alloc(&recs_seg, (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess)
+      buf.alloc(&recs_seg, 2 * (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess ||
+      buf.alloc(&ck_recs_d, (size_t)nck * np * sizeof(DevParticle)) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
   std::vector<std::vector<DevParticle>> ck_recs(nck, std::vector<DevParticle>(np));
   std::vector<DevParticle> fin_recs(np);
@@ -1824,6 +1827,9 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       HIP_TRY(hipMemcpyAsync(ck + (size_t)(t / K) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice, h->stream),
               "save checkpoint");
       HIP_TRY(save_recs(ck_recs[t / K]), "save checkpoint records");
+      HIP_TRY(hipMemcpyAsync(ck_recs_d + (size_t)(t / K) * np, parts_of(h, h->pcur), np * sizeof(DevParticle),
+                             hipMemcpyDeviceToDevice, h->stream),
+              "keep checkpoint records");
     }
     const int64_t n = std::min((t / K + 1) * K, T) - t;
     if (const int rc = run(n)) return rc;
@@ -1925,52 +1931,99 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   fa2.np = np;
   fa2.sw = sw;
   fa2.gscale = gscale;
-  const Launch l{dim3((unsigned)nblocks), dim3(threads), h->stream};
-  for (int64_t c = nck - 1; c >= 0; c--) {
-    const int64_t t0 = c * K, t1 = std::min(t0 + K, T), len = t1 - t0;
-    // S_k of the segment's ticks: the checkpoint itself, then slots 1.. of
-    // seg; slot 0 takes the last tick's output, which no tick reads
-    auto Sk = [&](int64_t j) -> void * {
-      return j == 0 ? static_cast<void *>(ck + (size_t)c * pop_bytes)
-                    : static_cast<void *>(seg + (size_t)(j % seg_len) * pop_bytes);
-    };
-    // recompute the segment tick by tick (collide, gather, move), each
-    // tick's output in its own slot and each tick's records on the device
-    HIP_TRY(load_recs(ck_recs[c]), "restore checkpoint records");
-    HIP_TRY(hipMemcpyAsync(recs_seg, parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToDevice,
-                           h->stream),
+  // Segment c's recomputation (on a second stream, into slot set c % 2)
+  // overlaps the reverse sweep of segment c + 1 (on the handle's stream):
+  // the two share only the read-only checkpoints.  The recomputation uses
+  // the lattice's own working state (populations' spare buffer, records,
+  // partials, rho), which the reverse sweep does not touch.
+  hipStream_t s_rev = h->stream, s_rec = nullptr;
+  hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+  struct Streams {
+    lbm_handle *h;
+    hipStream_t main, *extra;
+    hipEvent_t *evs;
+    ~Streams() {
+      h->stream = main;
+      if (*extra) {
+        hipStreamSynchronize(*extra);
+        hipStreamDestroy(*extra);
+      }
+      for (int i = 0; i < 4; i++)
+        if (evs[i]) hipEventDestroy(evs[i]);
+    }
+  };
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
+  Streams guard{h, s_rev, &s_rec, evs};
+  HIP_TRY(hipStreamCreate(&s_rec), "create recomputation stream");
+  for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming), "create event");
+  ev_rec[0] = evs[0];
+  ev_rec[1] = evs[1];
+  ev_rev[0] = evs[2];
+  ev_rev[1] = evs[3];
+  HIP_TRY(hipEventRecord(ev_rev[0], s_rev), "event");  // (both sets free)
+  HIP_TRY(hipEventRecord(ev_rev[1], s_rev), "event");
+  // S_k of segment c's ticks: the checkpoint itself, then slots 1.. of its
+  // set; slot 0 takes the last tick's output, which no tick reads
+  auto Sk = [&](int64_t c, int64_t j) -> void * {
+    return j == 0 ? static_cast<void *>(ck + (size_t)c * pop_bytes)
+                  : static_cast<void *>(seg + ((size_t)(c & 1) * seg_len + (size_t)(j % seg_len)) * pop_bytes);
+  };
+  auto recs_of = [&](int64_t c, int64_t j) { return recs_seg + ((size_t)(c & 1) * (seg_len + 1) + (size_t)j) * np; };
+  // recompute segment c tick by tick (collide, gather, move), each tick's
+  // output in its own slot and each tick's records on the device
+  auto recompute = [&](int64_t c) -> int {
+    const int64_t t0 = c * K, len = std::min(t0 + K, T) - t0;
+    h->stream = s_rec;
+    HIP_TRY(hipStreamWaitEvent(s_rec, ev_rev[c & 1], 0), "wait for the set");
+    HIP_TRY(hipMemcpyAsync(parts_of(h, h->pcur), ck_recs_d + (size_t)c * np, np * sizeof(DevParticle),
+                           hipMemcpyDeviceToDevice, s_rec),
+            "restore checkpoint records");
+    HIP_TRY(hipMemcpyAsync(recs_of(c, 0), parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToDevice,
+                           s_rec),
             "keep segment records");
     for (int64_t j = 0; j < len; j++) {
-      hipError_t e = launch_mode(h, kCollide, Sk(j), h->f[h->cur ^ 1], h->pcur);
-      if (e == hipSuccess) e = launch_mode(h, kGather, h->f[h->cur ^ 1], Sk(j + 1), h->pcur);
+      hipError_t e = launch_mode(h, kCollide, Sk(c, j), h->f[h->cur ^ 1], h->pcur);
+      if (e == hipSuccess) e = launch_mode(h, kGather, h->f[h->cur ^ 1], Sk(c, j + 1), h->pcur);
       if (e != hipSuccess) return hip_fail(e, "segment step launch");
       h->pcur ^= 1;
       StepArgs fa = step_args(h, nullptr, nullptr, h->pcur);
       fa.sp.parts_out = parts_of(h, h->pcur);  // in place
-      e = launch_finalize(fa.sp, h->stream);
+      e = launch_finalize(fa.sp, s_rec);
       if (e != hipSuccess) return hip_fail(e, "finalize launch");
-      HIP_TRY(hipMemcpyAsync(recs_seg + (size_t)(j + 1) * np, parts_of(h, h->pcur), np * sizeof(DevParticle),
-                             hipMemcpyDeviceToDevice, h->stream),
+      HIP_TRY(hipMemcpyAsync(recs_of(c, j + 1), parts_of(h, h->pcur), np * sizeof(DevParticle),
+                             hipMemcpyDeviceToDevice, s_rec),
               "keep segment records");
     }
+    HIP_TRY(hipEventRecord(ev_rec[c & 1], s_rec), "event");
+    h->stream = s_rev;
+    return LBM_OK;
+  };
+  const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
+  if (const int rc = recompute(nck - 1)) return rc;
+  for (int64_t c = nck - 1; c >= 0; c--) {
+    const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
+    HIP_TRY(hipStreamWaitEvent(s_rev, ev_rec[c & 1], 0), "wait for the recomputation");
+    if (c > 0)
+      if (const int rc = recompute(c - 1)) return rc;
     for (int64_t k = t1 - 1; k >= t0; k--) {
       // the move's reverse (dL/dF, dL/dT of tick k), the adjoint tick, then
       // the particles' adjoint state at the start of tick k
-      ma.recs_k = recs_seg + (size_t)(k - t0) * np;
-      ma.recs_k1 = recs_seg + (size_t)(k - t0 + 1) * np;
-      hipError_t e = launch_adjoint_move(ma, np, h->stream);
-      aa.parts = recs_seg + (size_t)(k - t0) * np;
-      aa.S = Sk(k - t0);
+      ma.recs_k = recs_of(c, k - t0);
+      ma.recs_k1 = recs_of(c, k - t0 + 1);
+      hipError_t e = launch_adjoint_move(ma, np, s_rev);
+      aa.parts = ma.recs_k;
+      aa.S = Sk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
       if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
-      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks, np, sums_d, h->stream);
+      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks, np, sums_d, s_rev);
       fa2.recs_k = ma.recs_k;
       fa2.direct = k > ws;  // V_k is one of the averaged speeds
-      if (e == hipSuccess) e = launch_adjoint_finish(fa2, h->stream);
+      if (e == hipSuccess) e = launch_adjoint_finish(fa2, s_rev);
       if (e != hipSuccess) return hip_fail(e, "adjoint launch");
       a ^= 1;
     }
+    HIP_TRY(hipEventRecord(ev_rev[c & 1], s_rev), "event");
   }
   std::vector<double> gB((size_t)np * 2);
   HIP_TRY(copy_sync(h, gB.data(), gB_d, gB.size() * sizeof(double), hipMemcpyDeviceToHost), "read B gradients");
