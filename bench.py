@@ -45,7 +45,7 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
 HBM_PEAK_GBS = 8000.0
 # the short run the profiles are taken on (no variants, batch, adjoint or CPU leg)
-PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --block 192,1,2"
+PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --block 192,2,1"
 BYTES = {"f64": 8, "f32": 4, "f16": 2}
 
 
