@@ -1379,21 +1379,26 @@ step_kernel(StepArgs a) {
 }
 
 // One step kernel over a batch of independent lattices (the same lattice,
-// storage, collision and force kinds; any sizes, parameters and particles):
-// lattice i owns blocks [start[i], start[i + 1]) of the grid, its box blocks
-// (gathering modes) and tiles followed by idle blocks up to a multiple of 8.
+// storage, collision and force kinds; any sizes, parameters and particles).
+// The grid is a list of segments, each a multiple of 8 blocks: segment s
+// covers blocks [start[s], start[s + 1]) and holds either the box blocks
+// (gathering modes) or the tiles, followed by idle blocks, of lattice
+// seg[s] >> 1 (seg[s] & 1: tiles).  A lattice's box segment may come well
+// before its tiles, so that the box blocks' chain of dependent steps starts
+// early in the launch rather than behind the last tiles.
 constexpr int kMaxBatch = 256;
 struct BatchArgs {
   const StepArgs *boxes;  // device array of the lattices' arguments
-  int32_t n;
-  int32_t start[kMaxBatch + 1];
+  int32_t n;              // segments
+  int32_t start[2 * kMaxBatch + 1];
+  uint16_t seg[2 * kMaxBatch];
 };
 
 template <int Q, int S, int COLL, int FM, int MODE>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
 batch_step_kernel(BatchArgs ba) {
   const int b = blockIdx.x;
-  int lo = 0, hi = ba.n - 1;  // the lattice whose range holds b
+  int lo = 0, hi = ba.n - 1;  // the last segment starting at or before b
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (ba.start[mid] <= b)
@@ -1401,9 +1406,10 @@ batch_step_kernel(BatchArgs ba) {
     else
       hi = mid - 1;
   }
-  const StepArgs &a = ba.boxes[lo];
-  const int local = b - ba.start[lo];
-  if (local >= (MODE != kCollide ? a.sp.nbox_pad : 0) + a.t.ntiles) return;
+  const StepArgs &a = ba.boxes[ba.seg[lo] >> 1];
+  const int nbox = MODE != kCollide ? a.sp.nbox_pad : 0;
+  const int local = b - ba.start[lo] + ((ba.seg[lo] & 1) ? nbox : 0);
+  if (local >= nbox + a.t.ntiles) return;
   step_block<Q, S, COLL, FM, MODE>(a, local);
 }
 
