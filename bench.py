@@ -105,7 +105,11 @@ def batch_bench(lh, args):
     boxes = batch_boxes(lh, "f32", args.seed)
     nodes = sum(int(np.prod(m.resolved.n[:3])) for m in boxes)
     b = lh.LBMbatch(boxes)
-    block, tuned_ms = b.autotune(5)
+    if args.batch_block:
+        block = tuple(int(v) for v in args.batch_block.split(","))
+        b.set_block(*block)
+    else:
+        block, tuned_ms = b.autotune(5)
     b.run(args.batch_warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -305,6 +309,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
+    ap.add_argument("--batch-block", default="", help="bx,by,bz of the batched kernel instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-ticks", type=int, default=2000)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
