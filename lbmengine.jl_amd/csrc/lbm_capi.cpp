@@ -1149,9 +1149,18 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
         cands.push_back({bx, by, bz, 0});
       }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits this lattice");
+  // The trial kernels step a copy of the populations back and forth between
+  // two scratch buffers, as a run does: stepping the same input again and
+  // again would leave it in the last-level cache (the streamed stores do not
+  // evict it) and time a rate no run reaches.
   const size_t bytes = (size_t)h->N * h->Q * elem_size(h->cfg.storage);
-  void *scratch = nullptr;
-  HIP_TRY(hipMalloc(&scratch, bytes), "hipMalloc autotune scratch");
+  char *scratch = nullptr;
+  HIP_TRY(hipMalloc(&scratch, 2 * bytes), "hipMalloc autotune scratch");
+  if (const hipError_t ce = hipMemcpyAsync(scratch, h->f[h->cur], bytes, hipMemcpyDeviceToDevice, h->stream);
+      ce != hipSuccess) {
+    hipFree(scratch);
+    return hip_fail(ce, "autotune copy");
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -1174,11 +1183,17 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
       return rc;
     }
     h->tiling = t;
-    // fused kernels reading the current populations into the scratch buffer
-    for (int r = 0; r < 2 && e == hipSuccess; r++)  // warm-up
-      e = launch_mode(h, kFused, h->f[h->cur], scratch, h->pcur);
+    // fused kernels between the two scratch buffers (an even count of
+    // warm-up steps, so the timed ones start from the copy's buffer)
+    int side = 0;
+    auto trial = [&]() {
+      char *from = scratch + side * bytes;
+      side ^= 1;
+      return launch_mode(h, kFused, from, scratch + side * bytes, h->pcur);
+    };
+    for (int r = 0; r < 2 && e == hipSuccess; r++) e = trial();  // warm-up
     if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_mode(h, kFused, h->f[h->cur], scratch, h->pcur);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = trial();
     if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
@@ -1615,11 +1630,11 @@ int lbm_batch_last_run_times(lbm_batch *b, double *kernel_ms, double *step_ms) {
   return LBM_OK;
 }
 
-// Autotuner of a batch: times the batched fused kernel (reading every box's
-// current populations, writing one scratch buffer; the records and partials
-// it also writes are the ones every run rewrites before reading) over the
-// block shapes that fit every box and both tile orders, and keeps the
-// fastest.
+// Autotuner of a batch: times the batched fused kernel (stepping a copy of
+// every box's populations back and forth between two scratch buffers, as a
+// run does; the records and partials it also writes are the ones every run
+// rewrites before reading) over the block shapes that fit every box and both
+// tile orders, and keeps the fastest.
 int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best_ms) {
   if (!b) return fail(LBM_EINVAL, "null batch");
   if (reps < 1) reps = 1;
@@ -1674,11 +1689,22 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits the boxes");
   char *scratch = nullptr;
-  StepArgs *trial_d = nullptr;
-  HIP_TRY(hipMalloc(&scratch, scratch_bytes), "hipMalloc autotune scratch");
-  if (hipMalloc(&trial_d, (size_t)n * sizeof(StepArgs)) != hipSuccess) {
+  StepArgs *trial_d = nullptr;  // two tables: buffer 0 -> 1, then 1 -> 0
+  HIP_TRY(hipMalloc(&scratch, 2 * scratch_bytes), "hipMalloc autotune scratch");
+  if (hipMalloc(&trial_d, 2 * (size_t)n * sizeof(StepArgs)) != hipSuccess) {
     hipFree(scratch);
     return fail(LBM_ENOMEM, "hipMalloc autotune table");
+  }
+  for (int i = 0; i < n; i++) {
+    const lbm_handle *h = b->boxes[i];
+    const hipError_t ce = hipMemcpyAsync(scratch + off[i], h->f[h->cur],
+                                         (size_t)h->N * h->Q * elem_size(b->storage), hipMemcpyDeviceToDevice,
+                                         b->stream);
+    if (ce != hipSuccess) {
+      hipFree(trial_d);
+      hipFree(scratch);
+      return hip_fail(ce, "batch autotune copy");
+    }
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipEventCreate(&e0);
@@ -1688,7 +1714,7 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   Tiling best_c = cands[0];
   int rc = LBM_OK;
   hipError_t e = hipSuccess;
-  std::vector<StepArgs> trial(n);
+  std::vector<StepArgs> trial(2 * (size_t)n);
   for (const Tiling &c : cands) {
     for (lbm_handle *h : b->boxes)
       if ((rc = adopt_shape(h, c))) break;
@@ -1696,12 +1722,21 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
     b->shape = c;
     for (int i = 0; i < n; i++) {
       lbm_handle *h = b->boxes[i];
-      trial[i] = step_args(h, h->f[h->cur], scratch + off[i], h->pcur);
+      char *s0 = scratch + off[i], *s1 = s0 + scratch_bytes;
+      trial[i] = step_args(h, s0, s1, h->pcur);
+      trial[n + i] = step_args(h, s1, s0, h->pcur);
     }
-    e = hipMemcpyAsync(trial_d, trial.data(), (size_t)n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream);
-    for (int r = 0; r < 2 && e == hipSuccess; r++) e = launch_batch_mode(b, kFused, trial.data(), trial_d);
+    e = hipMemcpyAsync(trial_d, trial.data(), 2 * (size_t)n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream);
+    // (an even count of warm-up steps, so the timed ones start from buffer 0)
+    int side = 0;
+    auto step = [&]() {
+      const hipError_t le = launch_batch_mode(b, kFused, trial.data() + side * n, trial_d + side * n);
+      side ^= 1;
+      return le;
+    };
+    for (int r = 0; r < 2 && e == hipSuccess; r++) e = step();
     if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_batch_mode(b, kFused, trial.data(), trial_d);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = step();
     if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
