@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
   if (live) {
     if (wall) {
 #pragma unroll
-      for (int q = 0; q < Q; q++) lout[own_idx + (int64_t)q * g.nx] = (R)0;
+      for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], (R)0);
     } else {
       R u[3] = {0, 0, 0};
       {
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       R ls_[Q], gFn[3] = {0, 0, 0};
       collide_vjp<Q, COLL, FM, R>(rho, u, Fn, a.c, lp, ls_, gFn);
 #pragma unroll
-      for (int q = 0; q < Q; q++) lout[own_idx + (int64_t)q * g.nx] = ls_[q] + grho;
+      for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], ls_[q] + grho);
       if constexpr (FM == LBM_FORCE_FIELD)
         for (int k = 0; k < 3; k++) a.gF[k * g.N + n] += (double)gFn[k];
     }
