@@ -7,11 +7,14 @@ namespace lbm {
 
 // Plain device copy of n16 16-byte words, one per thread: the measured HBM
 // ceiling, and the kernel of known traffic (n16*16 bytes read and written)
-// that scales the byte counters of the profiler.
+// that scales the byte counters of the profiler.  Its stores are
+// non-temporal, like the step kernels' population stores.
 __global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                    int64_t n16) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n16) dst[i] = src[i];
+  if (i < n16)
+    __builtin_nontemporal_store(reinterpret_cast<const u32x4 *>(src)[i], reinterpret_cast<u32x4 *>(dst) + i);
 }
 
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream) {
