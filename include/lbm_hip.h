@@ -225,6 +225,19 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir,
  * populations before and after are the reference's distributions. */
 int lbm_run(lbm_handle *h, int64_t nsteps);
 
+/* lbm_run without any allocation, for callers that cannot allocate device
+ * memory at call time (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The same kernels as lbm_run, launched directly (no graph is captured, no
+ * timing event recorded), with the particles' momentum-exchange partials and
+ * link-slot values in a fixed 8 MiB device array that the library holds on
+ * each device it runs on until the process ends.  Returns LBM_EINVAL, before
+ * launching anything, when the particles of this block shape need more.
+ * Every call on one device shares that array, so calls that overlap on one
+ * device (several host threads, or handles on different streams) race.
+ * The result is bit-identical to lbm_run's. */
+int lbm_run_noalloc(lbm_handle *h, int64_t nsteps);
+
 /* Wait for the handle's stream; reports a particle that left the supported
  * region (LBM_ERANGE) or a HIP error. */
 int lbm_sync(lbm_handle *h);

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,7 @@ struct lbm_handle {
   int32_t first_block[LBM_MAX_PARTICLES + 1] = {0};
   Tiling tiling{};
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at lbm_create (the stream's)
   int64_t tick = 0;
   double time = 0.0;
   bool populated = false;
@@ -451,7 +453,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   h->Q = cfg->q;
   for (int k = 0; k < 3; k++) h->n[k] = k < cfg->dims ? (int32_t)cfg->n[k] : 1;
   h->N = N;
-  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete h;
     return hip_fail(e, "hipStreamCreate");
@@ -980,14 +983,17 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_
 
 // A run of n ticks is one collide-only kernel, n - 1 fused kernels (replayed
 // from captured graphs of graph_k where possible) and one gather-only kernel,
-// so the populations between runs are the reference's distributions.
-int lbm_run(lbm_handle *h, int64_t nsteps) {
-  if (!h) return fail(LBM_EINVAL, "null handle");
-  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
-  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
-  if (nsteps == 0) return LBM_OK;
+// so the populations between runs are the reference's distributions.  With
+// `direct`, every kernel is launched directly (no graph is captured) and no
+// timing event is recorded, so the run allocates nothing.
+}  // extern "C"
+
+namespace {
+
+int run_ticks(lbm_handle *h, int64_t nsteps, bool direct) {
   const int64_t nfused = nsteps - 1;
-  if (h->timing && nfused > 0) {
+  const bool timing = h->timing && !direct;
+  if (timing && nfused > 0) {
     if ((int64_t)h->ev.size() < 2 * nfused) {
       const size_t old = h->ev.size();
       h->ev.resize(2 * nfused);
@@ -995,13 +1001,13 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
         HIP_TRY(hipEventCreate(&h->ev[i]), "hipEventCreate");
     }
   }
-  HIP_TRY(hipEventRecord(h->run_start, h->stream), "event record");
+  if (!direct) HIP_TRY(hipEventRecord(h->run_start, h->stream), "event record");
   int a = h->cur;
   hipError_t e = launch_mode(h, kCollide, h->f[a], h->f[a ^ 1], h->pcur);
   if (e != hipSuccess) return hip_fail(e, "step launch");
   a ^= 1;
   int64_t s = 0;
-  if (!h->timing && h->graph_k > 0) {
+  if (!timing && !direct && h->graph_k > 0) {
     // replay captured blocks of graph_k fused kernels (graph_k is even, so a
     // block leaves both parities where it found them)
     while (nfused - s >= h->graph_k) {
@@ -1015,8 +1021,8 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
     }
   }
   for (int64_t i = 0; s < nfused; s++, i++) {
-    hipEvent_t e0 = h->timing ? h->ev[2 * i] : nullptr;
-    hipEvent_t e1 = h->timing ? h->ev[2 * i + 1] : nullptr;
+    hipEvent_t e0 = timing ? h->ev[2 * i] : nullptr;
+    hipEvent_t e1 = timing ? h->ev[2 * i + 1] : nullptr;
     e = fused_once(h, a, h->pcur, e0, e1);
     if (e != hipSuccess) return hip_fail(e, "step launch");
     a ^= 1;
@@ -1036,8 +1042,9 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
   }
   h->tick += nsteps;
   for (int64_t k = 0; k < nsteps; k++) h->time += h->cfg.dt;
+  if (direct) return LBM_OK;
   HIP_TRY(hipEventRecord(h->run_stop, h->stream), "event record");
-  if (h->timing) {
+  if (timing) {
     HIP_TRY(hipEventSynchronize(h->run_stop), "sync");
     double sum = 0.0;
     for (int64_t k = 0; k < nfused; k++) {
@@ -1051,6 +1058,89 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
     h->step_ms = tot / (double)nsteps;
   }
   return LBM_OK;
+}
+
+// ------------------------------------------------- allocation-free run
+// lbm_run_noalloc's scratch: the momentum-exchange partials (two parities)
+// and the link-slot values of the box blocks, in one device array of fixed
+// size.  Host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.
+constexpr size_t kStaticScratchBytes = size_t(8) << 20;
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+int static_scratch(int dev, void **out);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+// Address of this device's copy of `scratch` (device `dev` must be current),
+// looked up once per device.
+int static_scratch(int dev, void **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(LBM_EINVAL, "device index beyond the scratch table");
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_run(lbm_handle *h, int64_t nsteps) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
+  if (nsteps == 0) return LBM_OK;
+  return run_ticks(h, nsteps, false);
+}
+
+int lbm_run_noalloc(lbm_handle *h, int64_t nsteps) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
+  if (nsteps == 0) return LBM_OK;
+  // the partials (both parities) and link-slot values the box blocks of this
+  // block shape use, as plan_boxes sizes them
+  const int64_t blocks = std::max<int64_t>(h->first_block[LBM_MAX_PARTICLES], 1);
+  const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+  const size_t part_doubles = (size_t)blocks * LBM_MAX_PARTICLES * 6 * 2;
+  const size_t part_bytes = (2 * part_doubles * sizeof(double) + 255) / 256 * 256;
+  const size_t lv_bytes = (size_t)blocks * (size_t)tile_threads(h->tiling) * (h->Q - 1) * rsize;
+  if (part_bytes + lv_bytes > kStaticScratchBytes)
+    return fail(LBM_EINVAL, "lbm_run_noalloc: the particles' scratch (" + std::to_string(part_bytes + lv_bytes) +
+                                " bytes for this block shape and these particles) exceeds the fixed " +
+                                std::to_string(kStaticScratchBytes) + " bytes; use lbm_run");
+  int prev = 0;
+  HIP_TRY(hipGetDevice(&prev), "hipGetDevice");
+  if (prev != h->device) HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+  void *base = nullptr;
+  int rc = static_scratch(h->device, &base);
+  if (rc == LBM_OK) {
+    // the run's kernels take the static scratch in place of the handle's
+    // buffers (they take every pointer by value at launch)
+    double *const own_partial = h->partial;
+    const size_t own_cap = h->partial_cap;
+    void *const own_linkval = h->linkval;
+    h->partial = static_cast<double *>(base);
+    h->partial_cap = part_doubles;
+    h->linkval = static_cast<unsigned char *>(base) + part_bytes;
+    rc = run_ticks(h, nsteps, true);
+    h->partial = own_partial;
+    h->partial_cap = own_cap;
+    h->linkval = own_linkval;
+  }
+  if (prev != h->device) hipSetDevice(prev);
+  return rc;
 }
 
 int lbm_sync(lbm_handle *h) {

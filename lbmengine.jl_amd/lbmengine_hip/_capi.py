@@ -95,6 +95,7 @@ SIGNATURES = {
                              C.POINTER(_dp), _dp],
     "lbm_viscous_stress": [_vp, _dp],
     "lbm_run": [_vp, C.c_int64],
+    "lbm_run_noalloc": [_vp, C.c_int64],
     "lbm_sync": [_vp],
     "lbm_get_tick": [_vp, C.POINTER(C.c_int64), _dp],
     "lbm_set_block": [_vp, C.c_int32, C.c_int32, C.c_int32],
