@@ -40,6 +40,9 @@ extern "C" {
 #define LBM_STORAGE_F64 0   /* populations f_i stored in FP64                  */
 #define LBM_STORAGE_F32 1   /* deviations f_i - w_i rho0 stored in FP32        */
 #define LBM_STORAGE_F16 2   /* deviations scaled by 2^14/w_i stored in FP16    */
+#define LBM_STORAGE_F16C 3  /* deviations over w_i, (f_i - w_i rho0) / w_i, in a
+                               16-bit float of 1 sign, 4 exponent and 11
+                               mantissa bits, |value| < 1/8 (clamped beyond) */
 
 #define LBM_SEMANTICS_GPU 0 /* follow src/gpu/ (Rodrigues rotation of e, ...)  */
 #define LBM_SEMANTICS_CPU 1 /* follow src/cpu/ (swimming direction not rotated) */

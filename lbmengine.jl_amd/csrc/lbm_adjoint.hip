@@ -100,6 +100,7 @@ hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const
   switch (storage) {
     case LBM_STORAGE_F64: return launch_adjoint_s<kF64>(coll, fm, a, l);
     case LBM_STORAGE_F32: return launch_adjoint_s<kF32>(coll, fm, a, l);
+    case LBM_STORAGE_F16C: return launch_adjoint_s<kF16C>(coll, fm, a, l);
     default: return launch_adjoint_s<kF16>(coll, fm, a, l);
   }
 }

@@ -42,7 +42,7 @@ int hip_fail(hipError_t e, const char *what) {
   } while (0)
 
 size_t elem_size(int storage) {
-  return storage == LBM_STORAGE_F64 ? 8 : (storage == LBM_STORAGE_F32 ? 4 : 2);
+  return storage == LBM_STORAGE_F64 ? 8 : (storage == LBM_STORAGE_F32 ? 4 : 2);  // F16, F16C: 2
 }
 
 }  // namespace
@@ -298,6 +298,7 @@ hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: return launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
     case LBM_STORAGE_F32: return launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
+    case LBM_STORAGE_F16C: return launch_step<kF16C>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
     default: return launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
   }
 }
@@ -360,6 +361,7 @@ int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: LBM_AUX(kF64); break;
     case LBM_STORAGE_F32: LBM_AUX(kF32); break;
+    case LBM_STORAGE_F16C: LBM_AUX(kF16C); break;
     default: LBM_AUX(kF16); break;
   }
 #undef LBM_AUX
@@ -426,8 +428,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     return fail(LBM_EINVAL, "lattice must be D2Q9 (dims 2) or D3Q19 (dims 3)");
   if (cfg->collision != LBM_COLLISION_BGK && cfg->collision != LBM_COLLISION_TRT)
     return fail(LBM_EINVAL, "collision must be BGK or TRT");
-  if (cfg->storage < LBM_STORAGE_F64 || cfg->storage > LBM_STORAGE_F16)
-    return fail(LBM_EINVAL, "storage must be F64, F32 or F16");
+  if (cfg->storage < LBM_STORAGE_F64 || cfg->storage > LBM_STORAGE_F16C)
+    return fail(LBM_EINVAL, "storage must be F64, F32, F16 or F16C");
   if (cfg->semantics != LBM_SEMANTICS_GPU && cfg->semantics != LBM_SEMANTICS_CPU)
     return fail(LBM_EINVAL, "semantics must be GPU or CPU");
   if (cfg->force < LBM_FORCE_NONE || cfg->force > LBM_FORCE_FIELD)
@@ -1493,6 +1495,7 @@ hipError_t launch_batch_mode(lbm_batch *b, int mode, const StepArgs *host, const
   switch (b->storage) {
     case LBM_STORAGE_F64: return launch_batch_step<kF64>(b->Q, b->coll, b->force, mode, ba, l);
     case LBM_STORAGE_F32: return launch_batch_step<kF32>(b->Q, b->coll, b->force, mode, ba, l);
+    case LBM_STORAGE_F16C: return launch_batch_step<kF16C>(b->Q, b->coll, b->force, mode, ba, l);
     default: return launch_batch_step<kF16>(b->Q, b->coll, b->force, mode, ba, l);
   }
 }

@@ -53,6 +53,16 @@ struct Lattice<19> {
 // FP16 deviation storage: h_i = half((f_i - w_i rho0) * 2^14 / w_i).
 constexpr float kF16Scale = 16384.0f;
 
+// Custom 16-bit deviation storage (1 sign, 4 exponent, 11 mantissa bits; the
+// FP16C format of Lehmann et al., 2022, with its range placed for deviation
+// storage): h_i = (f_i - w_i rho0) / w_i with |h| < 1/8 is kept as bits 26..12
+// of the FP32 pattern of h * 2^-108, whose exponent field then runs 0..15:
+// 12 significant bits for 2^-18 <= |h| < 2^-3 and FP32's gradual underflow
+// (absolute step 2^-29) below.  Larger magnitudes are clamped.  Wall nodes,
+// whose stored values nothing reads, hold the clamped -rho0.
+constexpr double kF16CDown = 0x1p-108;
+constexpr double kF16CUp = 0x1p108;
+
 // ------------------------------------------------------- device particle
 // One rigid sphere.  The geometry (solid region) is the sphere at Rg; the
 // node velocities are those of the state (nR, nV, nW, ne) at the last

@@ -1,11 +1,11 @@
 // lbm_kernels.hip — launchers of the step and auxiliary kernels for
 // one storage kind.  Compiled once per storage, -DLBM_STORE=0 (FP64),
-// 1 (FP32) or 2 (FP16), and per part: -DLBM_PART=0 builds the one-lattice
+// 1 (FP32), 2 (FP16) or 3 (custom 16-bit), and per part: -DLBM_PART=0 builds the one-lattice
 // step and the auxiliary kernels, -DLBM_PART=1 the batched step.
 #include "lbm_dispatch.hpp"
 
 #ifndef LBM_STORE
-#error "LBM_STORE must be 0 (FP64), 1 (FP32) or 2 (FP16)"
+#error "LBM_STORE must be 0 (FP64), 1 (FP32), 2 (FP16) or 3 (custom 16-bit)"
 #endif
 #ifndef LBM_PART
 #error "LBM_PART must be 0 (one lattice and auxiliary kernels) or 1 (batched step)"

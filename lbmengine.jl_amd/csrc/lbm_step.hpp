@@ -15,7 +15,8 @@
 //
 // Populations are stored row-blocked: the Q planes of each x-row are
 // contiguous, f[(row Q + q) nx + x] with row = y + ny z.  FP64 storage holds
-// f; FP32 holds f - w_i rho0; FP16 holds (f - w_i rho0) * 2^14 / w_i.
+// f; FP32 holds f - w_i rho0; FP16 holds (f - w_i rho0) * 2^14 / w_i; the
+// custom 16-bit format (kF16C) holds (f - w_i rho0) / w_i.
 #pragma once
 
 #include "lbm_common.hpp"
@@ -23,7 +24,7 @@
 
 namespace lbm {
 
-enum StoreKind { kF64 = 0, kF32 = 1, kF16 = 2 };
+enum StoreKind { kF64 = 0, kF32 = 1, kF16 = 2, kF16C = 3 };
 
 template <int S>
 struct Store;
@@ -42,6 +43,27 @@ struct Store<kF16> {
   using T = _Float16;
   using R = float;
 };
+template <>
+struct Store<kF16C> {
+  using T = int16_t;  // loaded sign-extended (see f16c_scaled)
+  using R = float;
+};
+
+// kF16C element -> h * 2^-108 as an FP32 value (exact): with the element
+// sign-extended, shifting it left by 12 puts the exponent and mantissa bits
+// in place and copies of the sign in bits 31..27; the mask keeps bit 31.
+__device__ __forceinline__ float f16c_scaled(int16_t v) {
+  return __builtin_bit_cast(float, (int32_t)((uint32_t)(int32_t)v << 12) & (int32_t)0x87FFF000);
+}
+
+// h * 2^-108 (FP32, gradual underflow kept) -> kF16C element, rounded to the
+// nearest (ties away from zero) and clamped to the largest magnitude.
+__device__ __forceinline__ int16_t f16c_from_scaled(float s) {
+  const float lim = __builtin_bit_cast(float, 0x07FFF7FFu);  // rounds down to 0x07FFF000
+  s = __builtin_amdgcn_fmed3f(s, -lim, lim);
+  const uint32_t b = __builtin_bit_cast(uint32_t, s) + 0x800u;  // round the 12 dropped bits
+  return (int16_t)(uint16_t)(((b >> 12) & 0x7FFFu) | ((b >> 16) & 0x8000u));
+}
 
 // Every population a step kernel writes is read next by the following
 // kernel, long after the caches would have dropped it, and nothing reads it
@@ -57,6 +79,8 @@ template <int S, int Q>
 __device__ __forceinline__ typename Store<S>::R decode(typename Store<S>::T v, int i) {
   if constexpr (S == kF16)
     return static_cast<float>(v) * static_cast<float>(Lattice<Q>::w(i) / kF16Scale);
+  else if constexpr (S == kF16C)
+    return f16c_scaled(v) * static_cast<float>(Lattice<Q>::w(i) * kF16CUp);
   else
     return v;
 }
@@ -65,6 +89,8 @@ template <int S, int Q>
 __device__ __forceinline__ typename Store<S>::T encode(typename Store<S>::R v, int i) {
   if constexpr (S == kF16)
     return static_cast<_Float16>(v * static_cast<float>(kF16Scale * Lattice<Q>::invw(i)));
+  else if constexpr (S == kF16C)
+    return f16c_from_scaled(v * static_cast<float>(Lattice<Q>::invw(i) * kF16CDown));
   else
     return v;
 }
@@ -75,6 +101,9 @@ __device__ __forceinline__ typename Store<S>::T encode_class(typename Store<S>::
   if constexpr (S == kF16)
     return static_cast<_Float16>(
         v * static_cast<float>(kF16Scale * (cls == 1 ? Lattice<Q>::invw(1) : Lattice<Q>::invw(Q - 1))));
+  else if constexpr (S == kF16C)
+    return f16c_from_scaled(
+        v * static_cast<float>(kF16CDown * (cls == 1 ? Lattice<Q>::invw(1) : Lattice<Q>::invw(Q - 1))));
   else
     return v;
 }
@@ -288,24 +317,42 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
   }
 }
 
-// FP16 storage: the same collision on the stored values h_q = d_q / s_q
-// (s_q = w_q / kF16Scale) directly: the moments are sums of the stored
-// halves (mixed-precision FMAs take them as they are), and each population
-// is updated as h_q + delta_q / s_q and rounded once to FP16.  The FMAs are
-// written out and contraction is off, so the result is the same bits
-// wherever the routine is inlined.
-template <int Q, int COLL, int FM>
-__device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float *Fn, const CollParams &c,
-                                            _Float16 (&hs)[Q], float &rho_out) {
+// 16-bit storage (FP16 or the custom format): the same collision on the
+// stored values h_q = d_q / s_q directly (FP16: s_q = w_q / kF16Scale, read
+// by mixed-precision FMAs as they are; custom format: h_q * 2^-108 as FP32
+// from the bit pattern, s_q = w_q 2^108): the moments are sums of the stored
+// values, and each population is updated as h_q + delta_q / s_q and rounded
+// once to the storage format.  The FMAs are written out and contraction is
+// off, so the result is the same bits wherever the routine is inlined.
+template <int S>
+__device__ __forceinline__ float h16_value(typename Store<S>::T v) {
+  if constexpr (S == kF16C)
+    return f16c_scaled(v);
+  else
+    return static_cast<float>(v);
+}
+template <int S>
+__device__ __forceinline__ typename Store<S>::T h16_store(float v) {
+  if constexpr (S == kF16C)
+    return f16c_from_scaled(v);
+  else
+    return static_cast<_Float16>(v);
+}
+
+template <int Q, int COLL, int FM, int SK = kF16>
+__device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q], const float *Fn, const CollParams &c,
+                                            typename Store<SK>::T (&hs)[Q], float &rho_out) {
 #pragma clang fp contract(off)
   using L = Lattice<Q>;
   constexpr int D = L::D;
-  constexpr float s0 = static_cast<float>(L::w(0) / kF16Scale);
-  constexpr float s1 = static_cast<float>(L::w(1) / kF16Scale);
-  constexpr float s2 = static_cast<float>(L::w(Q - 1) / kF16Scale);
-  constexpr float is0 = static_cast<float>(kF16Scale * L::invw(0));
-  constexpr float is1 = static_cast<float>(kF16Scale * L::invw(1));
-  constexpr float is2 = static_cast<float>(kF16Scale * L::invw(Q - 1));
+  // stored unit -> deviation (s) and back (is)
+  constexpr double kUnit = SK == kF16C ? kF16CUp : 1.0 / kF16Scale;
+  constexpr float s0 = static_cast<float>(L::w(0) * kUnit);
+  constexpr float s1 = static_cast<float>(L::w(1) * kUnit);
+  constexpr float s2 = static_cast<float>(L::w(Q - 1) * kUnit);
+  constexpr float is0 = static_cast<float>(L::invw(0) / kUnit);
+  constexpr float is1 = static_cast<float>(L::invw(1) / kUnit);
+  constexpr float is2 = static_cast<float>(L::invw(Q - 1) / kUnit);
   const float ls = c.f_ls;
   const float invC2s = c.f_invC2s;
   const float invC4s = c.f_invC4s;
@@ -313,7 +360,7 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
   float S1 = 0.0f, S2 = 0.0f, A1[3] = {0.0f, 0.0f, 0.0f}, A2[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int q = 1; q < Q; q++) {
-    const float v = static_cast<float>(h[q]);
+    const float v = h16_value<SK>(h[q]);
     const int cq[3] = {L::cx[q], L::cy[q], L::cz[q]};
     if (L::wclass[q] == 1) {
       S1 = fmaf(v, 1.0f, S1);
@@ -327,7 +374,7 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
         if (cq[k]) A2[k] = fmaf(v, cq[k] > 0 ? 1.0f : -1.0f, A2[k]);
     }
   }
-  const float h0 = static_cast<float>(h[0]);
+  const float h0 = h16_value<SK>(h[0]);
   const float drho = fmaf(s2, S2, fmaf(s1, S1, s0 * h0));
   float j[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -367,7 +414,7 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
     float S = 0.0f;
     if constexpr (FM != LBM_FORCE_NONE) S = (w0 * dt * gp) * (-uF * invC2s);
     const float delta = fmaf(-wp, fmaf(s0, h0, -dfe), S);
-    hs[0] = static_cast<_Float16>(fmaf(delta, is0, h0));
+    hs[0] = h16_store<SK>(fmaf(delta, is0, h0));
   }
   // conjugate pairs (q, q̄), q < q̄: symmetric and antisymmetric parts
 #pragma unroll
@@ -394,13 +441,13 @@ __device__ __forceinline__ void collide_h16(const _Float16 (&h)[Q], const float 
       Ssym = (w * dt * gp) * fmaf(cu * cF, invC4s, -uF * invC2s);
       Santi = (w * dt * gm) * (cF * invC2s);
     }
-    const float hq = static_cast<float>(h[q]), ho = static_cast<float>(h[o]);
+    const float hq = h16_value<SK>(h[q]), ho = h16_value<SK>(h[o]);
     const float psym = fmaf(0.5f * sq, hq + ho, -esym);
     const float panti = fmaf(0.5f * sq, hq - ho, -eanti);
     const float dsym = fmaf(-wp, psym, Ssym);
     const float danti = fmaf(-wm, panti, Santi);
-    hs[q] = static_cast<_Float16>(fmaf(dsym + danti, isq, hq));
-    hs[o] = static_cast<_Float16>(fmaf(dsym - danti, isq, ho));
+    hs[q] = h16_store<SK>(fmaf(dsym + danti, isq, hq));
+    hs[o] = h16_store<SK>(fmaf(dsym - danti, isq, ho));
   }
 }
 
@@ -410,8 +457,8 @@ __device__ __forceinline__ void collide_stored(const typename Store<S>::T (&raw)
                                                const CollParams &c, typename Store<S>::T (&out)[Q],
                                                typename Store<S>::R &rho) {
   using R = typename Store<S>::R;
-  if constexpr (S == kF16) {
-    collide_h16<Q, COLL, FM>(raw, Fn, c, out, rho);
+  if constexpr (S == kF16 || S == kF16C) {
+    collide_h16<Q, COLL, FM, S>(raw, Fn, c, out, rho);
   } else {
     R f[Q], fs[Q];
 #pragma unroll

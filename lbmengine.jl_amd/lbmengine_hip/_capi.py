@@ -16,7 +16,7 @@ LBM_OK = 0
 LBM_EINVAL, LBM_ENOMEM, LBM_EDEVICE, LBM_ESTATE, LBM_ERANGE = -1, -2, -3, -4, -5
 
 COLLISION = {"bgk": 0, "trt": 1}
-STORAGE = {"f64": 0, "f32": 1, "f16": 2}
+STORAGE = {"f64": 0, "f32": 1, "f16": 2, "f16c": 3}
 SEMANTICS = {"gpu": 0, "cpu": 1}
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 BEAD, SQUIRMER = 0, 1

@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     OBJ_DIR.mkdir(exist_ok=True)
     jobs = []
-    for store, tag in ((0, "f64"), (1, "f32"), (2, "f16")):
+    for store, tag in ((0, "f64"), (1, "f32"), (2, "f16"), (3, "f16c")):
         for part, ptag in ((0, ""), (1, "_batch")):
             jobs.append([HIPCC, *COMMON, "-c", f"-DLBM_STORE={store}", f"-DLBM_PART={part}",
                          str(CSRC / "lbm_kernels.hip"), "-o", str(OBJ_DIR / f"lbm_kernels_{tag}{ptag}.o")])

@@ -7,8 +7,9 @@ Names and keyword arguments follow LBMengine.jl's GPU backend
 times, B1/B2, masses) happens here on the host exactly as in the
 reference; the time step runs in the HIP library.
 
-Backend-specific keywords of `modelInit`: `storage` ("f64", "f32" or
-"f16": how populations are stored on the device), `semantics` ("gpu" to
+Backend-specific keywords of `modelInit`: `storage` ("f64", "f32", "f16"
+or "f16c", a 16-bit float of 1 sign, 4 exponent and 11 mantissa bits: how
+populations are stored on the device), `semantics` ("gpu" to
 follow src/gpu, "cpu" to follow src/cpu) and `rho0` (shift of the
 deviation storage).
 """

@@ -82,7 +82,7 @@ def run_squirmer(storage, steps, n=48, r=6.0, beta=-5.0, seed=3):
     return m, np.array(speeds)
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "f16c"])
 @pytest.mark.parametrize("beta", [-5.0, 5.0])
 def test_reduced_precision_against_fp64(gpu, storage, beta):
     ref, s_ref = run_squirmer("f64", 1000, beta=beta)
