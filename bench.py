@@ -3,15 +3,20 @@
 Workload (the headline configuration): D3Q19 TRT (Lambda 3/16, tau 0.8),
 192^3 periodic, one squirmer of radius 12 (B1 = 1e-3, beta from --beta,
 position and orientation uniform from the seed), Ladd moving-wall bounce-back
-with momentum-exchange force/torque, populations stored as FP16 deviations
-with FP32 collision.  One step = one tick! of the reference.
+with momentum-exchange force/torque, populations stored in 16 bits (the
+custom 1-4-11 float over (f - w rho0)/w, "f16c") with FP32 collision.  One
+step = one tick! of the reference.
 
 Prints ONE JSON line: MLUPS (all nodes x steps / wall second on the default
 run path, hipGraph replays of the one-launch tick), the roofline of the step
 kernel (algorithmic bytes 2*Q*sizeof(storage) per node over its average
-duration from HIP events on its stream), the CPU baseline (the oracle, a C
-restatement of the reference step, on this host's cores), FP64/FP32 variants
-of the same step, and the FP16-vs-FP64 errors after --accuracy-steps steps.
+duration from HIP events on its stream, and over the wall-clock step), the
+CPU baseline (the oracle, a C restatement of the reference step, on this
+host's cores), FP64/FP32/IEEE-FP16 variants of the same step, the errors of
+the FP32, FP16 and f16c storages against FP64 after --accuracy-steps steps
+for beta in {-5, 0, 5} (the velocity field outside the sphere is the graded
+figure), and the two smaller configurations (D2Q9 squirmer 128^2 FP64,
+D3Q19 channel 256x64x64).
 
 The "batch" object measures the batched workload: 64 independent D3Q19 TRT
 periodic boxes, edges drawn from {48, 64, 96}, one squirmer each (radius
@@ -45,8 +50,9 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
 HBM_PEAK_GBS = 8000.0
 # the short run the profiles are taken on (no variants, batch, adjoint or CPU leg)
-PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --block 192,2,1"
-BYTES = {"f64": 8, "f32": 4, "f16": 2}
+PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --no-configs --block 192,1,1"
+BYTES = {"f64": 8, "f32": 4, "f16": 2, "f16c": 2}
+ACC_BOUND = 1e-3  # velocity-field error against FP64 the headline is quoted at
 
 
 def workload_kwargs(storage: str, n: int):
@@ -285,9 +291,62 @@ def cpu_baseline(args):
         t = time.perf_counter() - t0
     nodes = args.n ** 3
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": nodes * steps / t / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(),
             "sample": f"{steps} steps of the same 192^3 squirmer workload in FP64 ({t:.1f} s), "
-                      f"oracle/lbm_oracle.c with OpenMP over nodes"}
+                      f"oracle/lbm_oracle.c with OpenMP over nodes ({cores} threads)"}
+
+
+def small_configs(lh, args):
+    """The two smaller configurations: MLUPS on the run path and the step
+    kernel's time from HIP events (with its algorithmic-byte rate)."""
+    out = {}
+    # D2Q9 BGK 128^2 periodic FP64, squirmer disk r = 8 at the centre, random
+    # orientation from the seed, CPU-backend semantics, 1000 steps
+    rng = np.random.default_rng(args.seed)
+    th = rng.uniform(0, 2 * np.pi)
+    m = lh.modelInit(dims=2, velocities="D2Q9", latticeParameter=1.0, relaxationTimeRatio=0.8,
+                     collisionModel="bgk", xlims=(-64, 63), storage="f64", semantics="cpu")
+    lh.addSquirmer(m, radius=8.0, swimmingSpeed=2e-3 / 3, beta=0.0, swimmingDirection=[np.cos(th), np.sin(th)],
+                   coupleForces=True, coupleTorques=True)
+    m.autotune(20)
+    w = run_wall(m, 1000, 100)
+    k, s = kernel_times(m, 1000)
+    n1 = 128 * 128
+    out["C1"] = {"workload": "D2Q9 BGK tau=0.8 128^2 periodic, squirmer disk r=8 B1=1e-3 beta=0, CPU-backend "
+                             "semantics, f64 storage, 1000 steps",
+                 "MLUPS": round(n1 * 1000 / w / 1e6, 1), "ms_per_step": round(w / 1000 * 1e3, 5),
+                 "kernel_ms": round(k, 5), "step_ms_device": round(s, 5), "block": list(m.get_block()),
+                 "hbm_frac_algorithmic": round(2 * 9 * 8 * n1 / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "note": "16,384 nodes (2.4 MB) per launch: launch-latency bound, the working set lives in cache"}
+    del m
+    # D3Q19 TRT (Lambda 3/16) channel 256 x 64 x 64, walls at z = 0 and 63,
+    # uniform Guo force 1e-6 along x, tau 0.9
+    for st in ("f32", "f64"):
+        m = lh.modelInit(dims=3, velocities="D3Q19", latticeParameter=1.0, relaxationTimeRatio=0.9,
+                         kineticParameter=3 / 16, xlims=(0, 255), ylims=(0, 63), zlims=(0, 63), walledDimensions=[3],
+                         forceDensity=[1e-6, 0, 0], storage=st)
+        m.autotune(20)
+        w = run_wall(m, 2000, 200)
+        k, s = kernel_times(m, 1000)
+        n2 = 256 * 64 * 64
+        out[f"C2_{st}"] = {"workload": f"D3Q19 TRT tau=0.9 channel 256x64x64, walls at z=0,63, Guo force 1e-6, "
+                                       f"{st} storage",
+                           "MLUPS": round(n2 * 2000 / w / 1e6, 1), "ms_per_step": round(w / 2000 * 1e3, 5),
+                           "kernel_ms": round(k, 5), "step_ms_device": round(s, 5), "block": list(m.get_block()),
+                           "hbm_frac_algorithmic": round(2 * 19 * BYTES[st] * n2 / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "note": f"{2 * 19 * BYTES[st] * n2 / 1e6:.0f} MB per launch: within the 256 MB "
+                                   "last-level cache for f32, so the algorithmic rate can exceed what HBM delivers"}
+        del m
+    return out
 
 
 def main():
@@ -295,7 +354,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--storage", default="f16", choices=list(BYTES))
+    ap.add_argument("--storage", default="f16c", choices=list(BYTES))
     ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--radius", type=float, default=12.0)
     ap.add_argument("--beta", type=float, default=0.0)
@@ -310,6 +369,7 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
     ap.add_argument("--batch-block", default="", help="bx,by,bz of the batched kernel instead of autotuning")
+    ap.add_argument("--no-configs", action="store_true", help="skip the D2Q9 and channel configurations")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-ticks", type=int, default=2000)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
@@ -356,6 +416,9 @@ def main():
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.storage,
+        "dtype_note": {"f16c": "16-bit float (1 sign, 4 exponent, 11 mantissa bits) over (f - w rho0)/w, "
+                               "FP32 collision", "f16": "IEEE FP16 over (f - w rho0) 2^14/w, FP32 collision",
+                       "f32": "FP32 deviations", "f64": "FP64 populations"}[args.storage],
         "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
@@ -365,15 +428,17 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
                      "bytes_per_launch": bytes_per_launch,
+                     "frac_wall_clock": round(bytes_per_launch / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                      "copy_ceiling_GBs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
     }
     del m
 
     if not args.no_variants:
         var = {}
-        for st in ("f64", "f32", "f16"):
+        for st in ("f64", "f32", "f16", "f16c"):
             if st == args.storage:
-                var[st] = {"MLUPS": out["value"], "kernel_ms": out["roofline"]["kernel_ms"]}
+                var[st] = {"MLUPS": out["value"], "kernel_ms": out["roofline"]["kernel_ms"],
+                           "hbm_frac": out["roofline"]["frac"]}
                 continue
             mv = build_model(lh, st, args)
             mv.autotune(10)
@@ -383,38 +448,51 @@ def main():
             var[st] = {"MLUPS": round(nodes * ks / w / 1e6, 1), "kernel_ms": round(k, 5),
                        "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del mv
+        var["speedup_f16c_over_f64"] = round(var["f16c"]["MLUPS"] / var["f64"]["MLUPS"], 2)
         var["speedup_f16_over_f64"] = round(var["f16"]["MLUPS"] / var["f64"]["MLUPS"], 2)
         out["variants"] = var
 
-        # FP16/FP32 deviation storage against FP64 after the same number of ticks
-        acc = {}
-        ref = build_model(lh, "f64", args)
-        ref.run(args.accuracy_steps)
-        ref.sync()
-        u64 = ref.fluidVelocity
-        v64 = ref.particles[0].velocity
-        # nodes outside the sphere (the reference also streams the interior)
-        R = ref.particles[0].position
-        c = [np.asarray(x) for x in ref.resolved.coords]
-        d2 = ((c[0][:, None, None] - R[0]) ** 2 + (c[1][None, :, None] - R[1]) ** 2
-              + (c[2][None, None, :] - R[2]) ** 2)
-        ext = d2 >= args.radius ** 2
-        del ref
-        for st in ("f32", "f16"):
-            mm = build_model(lh, st, args)
-            mm.run(args.accuracy_steps)
-            mm.sync()
-            u = mm.fluidVelocity
-            v = mm.particles[0].velocity
-            acc[st] = {"u_rel_l2": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
-                       "u_rel_l2_exterior": float(np.linalg.norm((u - u64)[ext]) / np.linalg.norm(u64[ext])),
-                       "speed_rel": float(abs(np.linalg.norm(v) - np.linalg.norm(v64)) / np.linalg.norm(v64)),
-                       "velocity_rel": float(np.linalg.norm(v - v64) / np.linalg.norm(v64))}
-            del mm
-        acc["steps"] = args.accuracy_steps
-        acc["swimming_speed_f64"] = float(np.linalg.norm(v64))
+        # the reduced-precision storages against FP64 after the same ticks,
+        # for each beta of the headline sweep; the main figure is the
+        # velocity field outside the sphere (the reference also streams the
+        # fluid inside it)
+        acc = {"steps": args.accuracy_steps, "bound": ACC_BOUND, "graded": "u_rel_l2_exterior"}
+        worst = {}
+        for beta in (-5.0, 0.0, 5.0):
+            a2 = argparse.Namespace(**vars(args))
+            a2.beta = beta
+            ref = build_model(lh, "f64", a2)
+            ref.run(args.accuracy_steps)
+            ref.sync()
+            u64 = ref.fluidVelocity
+            v64 = ref.particles[0].velocity
+            R = ref.particles[0].position
+            c = [np.asarray(x) for x in ref.resolved.coords]
+            d2 = ((c[0][:, None, None] - R[0]) ** 2 + (c[1][None, :, None] - R[1]) ** 2
+                  + (c[2][None, None, :] - R[2]) ** 2)
+            ext = d2 >= args.radius ** 2
+            del ref
+            row = {"swimming_speed_f64": float(np.linalg.norm(v64))}
+            for st in ("f32", "f16", "f16c"):
+                mm = build_model(lh, st, a2)
+                mm.run(args.accuracy_steps)
+                mm.sync()
+                u = mm.fluidVelocity
+                v = mm.particles[0].velocity
+                r = {"u_rel_l2_exterior": float(np.linalg.norm((u - u64)[ext]) / np.linalg.norm(u64[ext])),
+                     "u_rel_l2": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                     "speed_rel": float(abs(np.linalg.norm(v) - np.linalg.norm(v64)) / np.linalg.norm(v64)),
+                     "velocity_rel": float(np.linalg.norm(v - v64) / np.linalg.norm(v64))}
+                row[st] = r
+                worst[st] = max(worst.get(st, 0.0), r["u_rel_l2_exterior"], r["speed_rel"])
+                del mm
+            acc[f"beta={beta:g}"] = row
+        acc["worst_over_beta"] = worst
+        acc["within_bound"] = {st: bool(e <= ACC_BOUND) for st, e in worst.items()}
         out["accuracy_vs_f64"] = acc
 
+    if not args.no_configs:
+        out["configs"] = small_configs(lh, args)
     if not args.no_batch:
         out["batch"] = batch_bench(lh, args)
     if not args.no_adjoint:
