@@ -50,7 +50,8 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
 HBM_PEAK_GBS = 8000.0
 # the short run the profiles are taken on (no variants, batch, adjoint or CPU leg)
-PROFILE_ARGS = "--steps 20 --warmup 5 --no-cpu --no-variants --no-batch --no-adjoint --no-configs --block 192,1,1"
+PROFILE_ARGS = ("--steps 1000 --warmup 1000 --no-cpu --no-variants --no-configs --block 192,1,1 --batch-steps 200 "
+                "--batch-warmup 100 --batch-accuracy 0 --batch-block 96,2,1 --adjoint-ticks 400 --adjoint-dirs 0")
 BYTES = {"f64": 8, "f32": 4, "f16": 2, "f16c": 2}
 ACC_BOUND = 1e-3  # velocity-field error against FP64 the headline is quoted at
 
@@ -86,10 +87,10 @@ def run_wall(m, steps, warmup):
     return time.perf_counter() - t0
 
 
-def batch_boxes(lh, storage, seed, count=64):
-    """The batched workload's boxes (same seed, same boxes for any storage)."""
+def batch_specs(seed, count=64):
+    """The batched workload's boxes: (edge, radius, beta, position, axis) each."""
     rng = np.random.default_rng(seed)
-    models = []
+    specs = []
     for _ in range(count):
         n = int(rng.choice([48, 64, 96]))
         r = float(rng.uniform(6.0, 10.0))
@@ -97,9 +98,17 @@ def batch_boxes(lh, storage, seed, count=64):
         lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
         pos = rng.uniform(lo, hi, 3)
         e = rng.standard_normal(3)
+        specs.append((n, r, beta, pos, e / np.linalg.norm(e)))
+    return specs
+
+
+def batch_boxes(lh, storage, seed, count=64):
+    """The batched workload's boxes (same seed, same boxes for any storage)."""
+    models = []
+    for n, r, beta, pos, e in batch_specs(seed, count):
         m = lh.modelInit(**workload_kwargs(storage, n))
-        lh.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2e-3 / 3, beta=beta,
-                       swimmingDirection=e / np.linalg.norm(e), coupleForces=True, coupleTorques=True)
+        lh.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2e-3 / 3, beta=beta, swimmingDirection=e,
+                       coupleForces=True, coupleTorques=True)
         models.append(m)
     return models
 
@@ -112,8 +121,11 @@ def batch_bench(lh, args):
     nodes = sum(int(np.prod(m.resolved.n[:3])) for m in boxes)
     b = lh.LBMbatch(boxes)
     if args.batch_block:
-        block = tuple(int(v) for v in args.batch_block.split(","))
+        v = [int(x) for x in args.batch_block.split(",")]
+        block = tuple(v[:3])
         b.set_block(*block)
+        if len(v) > 3:
+            b.set_tile_order(v[3])
     else:
         block, tuned_ms = b.autotune(5)
     b.run(args.batch_warmup)
@@ -133,6 +145,7 @@ def batch_bench(lh, args):
            "boxes": len(boxes), "nodes": nodes, "steps": args.batch_steps, "warmup": args.batch_warmup,
            "MLUPS": round(nodes * args.batch_steps / wall / 1e6, 1),
            "ms_per_step": round(wall / args.batch_steps * 1e3, 5), "block": list(block),
+           "tile_order": b.get_tile_order(),
            "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
            "hbm_frac": round(bytes_per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if args.batch_accuracy:
@@ -226,6 +239,32 @@ def kernel_times(m, steps):
     return m.last_run_times()
 
 
+# Bytes one launch of each kernel moves by the algorithm (read + write), for
+# the workloads this script runs: the step kernel reads and writes every
+# population once; the batched kernel does so for every box; the adjoint
+# kernel reads the tick's populations, reads and writes the adjoint
+# populations (compute type), reads the force field and updates dL/dF (FP64):
+# 19*4*3 + 12 + 48 = 288 B per node in FP32.
+ADJ_N, ADJ_BYTES_PER_NODE = 96, 288
+
+
+def _kernel_family(name):
+    """Short key of a traced kernel: the fused step kernel of a storage, the
+    fused batched kernel, the adjoint kernel or the copy kernel."""
+    n = name.replace(" ", "")
+    if n.startswith("void"):
+        n = n[4:]
+    if "copy_kernel" in n:
+        return "copy_kernel"
+    if n.startswith("lbm::step_kernel<") and n.endswith(",1>(lbm::StepArgs)"):
+        return "step_kernel/" + {"0": "f64", "1": "f32", "2": "f16", "3": "f16c"}[n.split("<")[1].split(",")[1]]
+    if n.startswith("lbm::batch_step_kernel<") and n.endswith(",1>(lbm::BatchArgs)"):
+        return "batch_step_kernel"
+    if n.startswith("lbm::adjoint_kernel<"):
+        return "adjoint_kernel"
+    return None
+
+
 def _pmc_means(run_dir):
     """Mean counter value per kernel family in a rocprofv3 --pmc output folder."""
     import csv
@@ -236,42 +275,46 @@ def _pmc_means(run_dir):
     acc = defaultdict(list)
     for f in files:
         for r in csv.DictReader(open(f)):
-            name = r["Kernel_Name"]
-            # the fused gather+collide kernel (last template argument 1) and the copy
-            fused = "step_kernel<" in name and name.replace(" ", "").endswith(",1>(lbm::StepArgs)")
-            fam = "step_kernel" if fused else ("copy_kernel" if "copy_kernel" in name else None)
+            fam = _kernel_family(r["Kernel_Name"])
             if fam:
                 acc[(fam, r["Counter_Name"])].append(float(r["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
 
 
-def traffic_summary(fetch_dir, write_dir, storage, n):
-    """HBM bytes per step-kernel launch from two rocprofv3 --pmc runs of this
-    script (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by
-    what the same counter reads for the copy kernel, whose traffic is known
-    (it reads and writes half the step's algorithmic bytes)."""
+def traffic_summary(fetch_dir, write_dir, args, argv):
+    """HBM bytes per launch of every traced kernel from two rocprofv3 --pmc
+    runs of this script (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch),
+    each scaled by what the same counter reads for the copy kernel, whose
+    traffic is known (it reads and writes half the headline step's bytes)."""
     f, fc = _pmc_means(fetch_dir)
     w, wc = _pmc_means(write_dir)
-    bytes_per_launch = 2 * 19 * BYTES[storage] * n ** 3
-    copy_bytes = bytes_per_launch // 2 // 16 * 16
+    copy_bytes = 2 * 19 * BYTES[args.storage] * args.n ** 3 // 2 // 16 * 16
     kf = copy_bytes / (f[("copy_kernel", "FETCH_SIZE")] * 1024)
     kw = copy_bytes / (w[("copy_kernel", "WRITE_SIZE")] * 1024)
-    rd = f[("step_kernel", "FETCH_SIZE")] * 1024 * kf
-    wr = w[("step_kernel", "WRITE_SIZE")] * 1024 * kw
-    return {"kernel": "step_kernel", "storage": storage, "n": n,
-            "hbm_bytes_per_launch": round(rd + wr), "read_bytes": round(rd), "write_bytes": round(wr),
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "traffic_over_algorithmic": round((rd + wr) / bytes_per_launch, 4),
-            "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
-            "raw_kib": {"step_fetch": f[("step_kernel", "FETCH_SIZE")], "step_write": w[("step_kernel", "WRITE_SIZE")],
-                        "copy_fetch": f[("copy_kernel", "FETCH_SIZE")], "copy_write": w[("copy_kernel", "WRITE_SIZE")]},
-            "dispatches": {"step_fetch": fc[("step_kernel", "FETCH_SIZE")], "step_write": wc[("step_kernel", "WRITE_SIZE")]},
-            "made_by": [
-                f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {fetch_dir} -o step "
-                f"--output-format csv -- python3 bench.py {PROFILE_ARGS}",
-                f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex 'step_kernel|copy_kernel' -d {write_dir} -o step "
-                f"--output-format csv -- python3 bench.py {PROFILE_ARGS}",
-                f"python3 bench.py --traffic-from {fetch_dir} {write_dir}"]}
+    algo = {"batch_step_kernel": 2 * 19 * 4 * sum(s[0] ** 3 for s in batch_specs(args.seed)),
+            "adjoint_kernel": ADJ_BYTES_PER_NODE * ADJ_N ** 3}
+    # (only the headline storage's step kernel: the others in a run belong
+    # to other workloads, such as the adjoint's 96^3 forward steps)
+    algo[f"step_kernel/{args.storage}"] = 2 * 19 * BYTES[args.storage] * args.n ** 3
+    made_by = [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o pmc --output-format csv -- python3 bench.py {argv}",
+               f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o pmc --output-format csv -- python3 bench.py {argv}",
+               f"python3 bench.py --traffic-from {fetch_dir} {write_dir} {argv}"]
+    out = {}
+    for fam, a in algo.items():
+        if (fam, "FETCH_SIZE") not in f or (fam, "WRITE_SIZE") not in w:
+            continue
+        rd = f[(fam, "FETCH_SIZE")] * 1024 * kf
+        wr = w[(fam, "WRITE_SIZE")] * 1024 * kw
+        out[fam] = {"kernel": fam, "hbm_bytes_per_launch": round(rd + wr), "read_bytes": round(rd),
+                    "write_bytes": round(wr), "algorithmic_bytes_per_launch": a,
+                    "traffic_over_algorithmic": round((rd + wr) / a, 4),
+                    "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
+                    "raw_kib": {"fetch": f[(fam, "FETCH_SIZE")], "write": w[(fam, "WRITE_SIZE")],
+                                "copy_fetch": f[("copy_kernel", "FETCH_SIZE")],
+                                "copy_write": w[("copy_kernel", "WRITE_SIZE")]},
+                    "dispatches": {"fetch": fc[(fam, "FETCH_SIZE")], "write": wc[(fam, "WRITE_SIZE")]},
+                    "made_by": made_by}
+    return out
 
 
 def cpu_baseline(args):
@@ -368,20 +411,25 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
-    ap.add_argument("--batch-block", default="", help="bx,by,bz of the batched kernel instead of autotuning")
+    ap.add_argument("--batch-block", default="", help="bx,by,bz[,tile order] of the batched kernel instead of "
+                                                      "autotuning")
     ap.add_argument("--no-configs", action="store_true", help="skip the D2Q9 and channel configurations")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-ticks", type=int, default=2000)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
-    ap.add_argument("--block", default="", help="bx,by,bz of the step kernel instead of autotuning")
+    ap.add_argument("--block", default="", help="bx,by,bz[,tile order] of the step kernel instead of autotuning")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of this script into the traffic file and exit")
     args = ap.parse_args()
     if args.traffic_from:
-        out = traffic_summary(*args.traffic_from, args.storage, args.n)
+        # the remaining arguments are those of the profiled runs
+        argv = " ".join(a for a in sys.argv[1:] if a not in ("--traffic-from", *args.traffic_from))
+        out = traffic_summary(*args.traffic_from, args, argv)
         (ROOT / "profiles").mkdir(exist_ok=True)
-        dst = ROOT / "profiles" / f"traffic_step_{args.storage}_{args.n}.json"
-        dst.write_text(json.dumps(out, indent=1) + "\n")
+        names = {"batch_step_kernel": "traffic_batch_64_boxes.json", "adjoint_kernel": "traffic_adjoint_96.json"}
+        for fam, rec in out.items():
+            dst = ROOT / "profiles" / (names[fam] if fam in names else f"traffic_step_{fam.split('/')[1]}_{args.n}.json")
+            dst.write_text(json.dumps(rec, indent=1) + "\n")
         print(json.dumps(out))
         return
     if args.gpus != 1:
@@ -395,7 +443,10 @@ def main():
 
     m = build_model(lh, args.storage, args)
     if args.block:
-        m.set_block(*[int(v) for v in args.block.split(",")])
+        v = [int(x) for x in args.block.split(",")]
+        m.set_block(*v[:3])
+        if len(v) > 3:
+            m.set_tile_order(v[3])
         tuned_ms = float("nan")
     else:
         block, tuned_ms = m.autotune(20)
@@ -422,14 +473,15 @@ def main():
         "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
-                   "nodes": nodes, "block": list(m.get_block()), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
+                   "nodes": nodes, "block": list(m.get_block()), "tile_order": m.get_tile_order(), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
         "event_timed_MLUPS": round(nodes / (sms * 1e-3) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
                      "bytes_per_launch": bytes_per_launch,
                      "frac_wall_clock": round(bytes_per_launch / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
-                     "copy_ceiling_GBs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "copy_ceiling_GBs": round(copy_gbs, 1), "copy_ms": round(copy_ms, 5),
+                     "frac_of_copy": round(achieved / copy_gbs, 4)},
     }
     del m
 
@@ -446,7 +498,8 @@ def main():
             w = run_wall(mv, ks, args.warmup // 4)
             k, _ = kernel_times(mv, min(ks, 500))
             var[st] = {"MLUPS": round(nodes * ks / w / 1e6, 1), "kernel_ms": round(k, 5),
-                       "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                       "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "block": list(mv.get_block()), "tile_order": mv.get_tile_order()}
             del mv
         var["speedup_f16c_over_f64"] = round(var["f16c"]["MLUPS"] / var["f64"]["MLUPS"], 2)
         var["speedup_f16_over_f64"] = round(var["f16"]["MLUPS"] / var["f64"]["MLUPS"], 2)

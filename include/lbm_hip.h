@@ -256,6 +256,13 @@ int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz);
 /* Current block shape of the step kernel. */
 int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz);
 
+/* Order of the step kernel's tiles over its grid (normally chosen by the
+ * autotuner; lbm_set_block resets it to 1): 1 gives each of the 8 XCDs one
+ * contiguous range of tiles, 0 the plain order.  No counterpart in the
+ * reference. */
+int lbm_set_tile_order(lbm_handle *h, int32_t xcd_remap);
+int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap);
+
 /* Autotuner: times the fused step kernel over candidate block shapes and
  * tile orders (reading the current populations, writing a scratch buffer,
  * so the state is untouched), keeps the fastest and returns it with its time
@@ -314,9 +321,12 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps);
  * supported region, as lbm_sync does. */
 int lbm_batch_sync(lbm_batch *b);
 
-/* Block shape shared by the boxes (the rules of lbm_set_block). */
+/* Block shape shared by the boxes (the rules of lbm_set_block) and the
+ * order of the tiles (the rules of lbm_set_tile_order). */
 int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz);
 int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz);
+int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap);
+int lbm_batch_get_tile_order(lbm_batch *b, int32_t *xcd_remap);
 
 /* Autotuner of the batched fused kernel over the block shapes that fit every
  * box and both tile orders (the boxes' state is untouched). */

@@ -1357,6 +1357,22 @@ int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz) {
   return LBM_OK;
 }
 
+// Order of the step kernel's tiles over the grid: 1 gives each of the 8
+// XCDs one contiguous range of tiles, 0 the plain order.
+int lbm_set_tile_order(lbm_handle *h, int32_t xcd_remap) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  h->tiling.remap = xcd_remap != 0;
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap) {
+  if (!h || !xcd_remap) return fail(LBM_EINVAL, "null argument");
+  *xcd_remap = h->tiling.remap;
+  return LBM_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ batches
@@ -1693,6 +1709,22 @@ int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz) {
   return LBM_OK;
 }
 
+int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  b->shape.remap = xcd_remap != 0;
+  drop_batch_graphs(b);
+  for (lbm_handle *h : b->boxes)
+    if (const int rc = adopt_shape(h, b->shape)) return rc;
+  return LBM_OK;
+}
+
+int lbm_batch_get_tile_order(lbm_batch *b, int32_t *xcd_remap) {
+  if (!b || !xcd_remap) return fail(LBM_EINVAL, "null argument");
+  *xcd_remap = b->shape.remap;
+  return LBM_OK;
+}
+
 int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz) {
   if (!b) return fail(LBM_EINVAL, "null batch");
   if (bx) *bx = b->shape.bx;
@@ -1930,7 +1962,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       buf.alloc(&recs_seg, 2 * (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess ||
       buf.alloc(&ck_recs_d, (size_t)nck * np * sizeof(DevParticle)) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
-  std::vector<std::vector<DevParticle>> ck_recs(nck, std::vector<DevParticle>(np));
   std::vector<DevParticle> fin_recs(np);
   auto save_recs = [&](std::vector<DevParticle> &dst) {
     return copy_sync(h, dst.data(), parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToHost);
@@ -1953,6 +1984,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   double *ring = nullptr;
   if (buf.alloc(&ring, sizeof(double) * 9 * LBM_MAX_PARTICLES * (size_t)Wn) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint speed log");
+  // (the handle's own log was not written meanwhile: on restore it holds
+  // only ticks after the one the handle has reached)
   struct LogSwap {
     lbm_handle *h;
     double *d;
@@ -1960,7 +1993,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     ~LogSwap() {
       h->traj_d = d;
       h->traj_cap = cap;
-      h->traj_from = from;
+      h->traj_from = std::max(from, h->tick);
       drop_graph(h);
     }
   } swap{h, h->traj_d, h->traj_cap, h->traj_from};
@@ -1973,7 +2006,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     if (t % K == 0) {
       HIP_TRY(hipMemcpyAsync(ck + (size_t)(t / K) * pop_bytes, h->f[h->cur], pop_bytes, hipMemcpyDeviceToDevice, h->stream),
               "save checkpoint");
-      HIP_TRY(save_recs(ck_recs[t / K]), "save checkpoint records");
       HIP_TRY(hipMemcpyAsync(ck_recs_d + (size_t)(t / K) * np, parts_of(h, h->pcur), np * sizeof(DevParticle),
                              hipMemcpyDeviceToDevice, h->stream),
               "keep checkpoint records");

@@ -100,6 +100,10 @@ SIGNATURES = {
     "lbm_get_tick": [_vp, C.POINTER(C.c_int64), _dp],
     "lbm_set_block": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "lbm_get_block": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "lbm_set_tile_order": [_vp, C.c_int32],
+    "lbm_get_tile_order": [_vp, C.POINTER(C.c_int32)],
+    "lbm_batch_set_tile_order": [_vp, C.c_int32],
+    "lbm_batch_get_tile_order": [_vp, C.POINTER(C.c_int32)],
     "lbm_set_timing": [_vp, C.c_int],
     "lbm_last_run_ms": [_vp, _dp],
     "lbm_last_run_times": [_vp, _dp, _dp],

@@ -549,6 +549,14 @@ class LBMmodel:
         _capi.check(self._fn["lbm_get_block"](self._h, *[C.byref(x) for x in b]))
         return tuple(x.value for x in b)
 
+    def set_tile_order(self, xcd_remap: int) -> None:
+        _capi.check(self._fn["lbm_set_tile_order"](self._h, int(xcd_remap)))
+
+    def get_tile_order(self) -> int:
+        v = C.c_int32()
+        _capi.check(self._fn["lbm_get_tile_order"](self._h, C.byref(v)))
+        return v.value
+
     def autotune(self, reps: int = 20):
         """Pick the fastest block shape of the stream-collide kernel."""
         best = (C.c_int32 * 3)()
@@ -616,6 +624,14 @@ class LBMbatch:
         b = [C.c_int32() for _ in range(3)]
         _capi.check(self._fn["lbm_batch_get_block"](self._b, *[C.byref(x) for x in b]))
         return tuple(x.value for x in b)
+
+    def set_tile_order(self, xcd_remap: int) -> None:
+        _capi.check(self._fn["lbm_batch_set_tile_order"](self._b, int(xcd_remap)))
+
+    def get_tile_order(self) -> int:
+        v = C.c_int32()
+        _capi.check(self._fn["lbm_batch_get_tile_order"](self._b, C.byref(v)))
+        return v.value
 
     def autotune(self, reps: int = 10):
         """Pick the fastest block shape of the batched stream-collide kernel."""

@@ -150,3 +150,29 @@ def test_autograd_wrapper_matches_the_adjoint(gpu):
     assert L.item() == ref.loss
     assert B1.grad.item() == 3.0 * ref.dB1[0] and B2.grad.item() == 3.0 * ref.dB2[0]
     assert np.array_equal(Ft.grad.numpy(), 3.0 * ref.dF)
+
+
+def test_trajectory_log_after_adjoint(gpu):
+    """The adjoint's forward run logs into its own ring: afterwards the
+    handle's log refuses the ticks it did not see and serves later ones."""
+    import ctypes as C
+
+    from lbmengine_hip import _capi
+    n, r = 24, 4.0
+    F, pos, e = setup(n, r, 5)
+    m = make(gpu, "f32", n, r, 1e-3, 0.0, F, pos, e)
+    fn, h = m._fn, m._h
+    _capi.check(fn["lbm_set_trajectory_log"](h, 64))
+    m.run(10)
+    from lbmengine_hip import adjoint
+    adjoint.swimming_loss_gradient(m, ticks=20, checkpoint_every=10, window=5, target_speed=1e-4, force_penalty=0.0)
+    assert m.tick == 30
+    buf = np.zeros((5, 9))
+    rc = fn["lbm_get_trajectory"](h, 0, 25, 5, _capi.as_dptr(buf))  # ticks inside the adjoint's span
+    assert rc == _capi.LBM_EINVAL
+    m.run(4)
+    m.sync()
+    out = np.zeros((4, 9))
+    _capi.check(fn["lbm_get_trajectory"](h, 0, 31, 4, _capi.as_dptr(out)))
+    assert np.allclose(out[-1, :3], m.particles[0].position, rtol=0, atol=0)
+    assert np.allclose(out[-1, 3:6], m.particles[0].velocity, rtol=0, atol=0)
