@@ -56,11 +56,18 @@ extern "C" {
 
 #define LBM_MAX_PARTICLES 8
 
+/* lbm_config.flags */
+#define LBM_FLAG_INCOMPRESSIBLE 1 /* isFluidCompressible = false: the equilibrium
+                                     w rho + w rho0 (...) (gpu/fluids.jl:51), with
+                                     rho0 = lbm_config.rho0 the uniform initial
+                                     density                                   */
+
 /* Resolved model parameters: the output of modelInit's unit and scheme
  * resolution (gpu/initMethods.jl:440-702), done by the host front end. */
 typedef struct lbm_config {
   int32_t dims;          /* 2 or 3                                        */
-  int32_t q;             /* 9 (D2Q9, dims 2) or 19 (D3Q19, dims 3)        */
+  int32_t q;             /* 9 (D2Q9, dims 2); 15, 19 or 27 (D3Q15, D3Q19,
+                            D3Q27, dims 3; gpu/misc.jl:195-272)           */
   int64_t n[3];          /* nodes per dimension; n[2] = 1 when dims == 2  */
   int32_t collision;     /* LBM_COLLISION_*                               */
   int32_t storage;       /* LBM_STORAGE_*                                 */
@@ -68,7 +75,7 @@ typedef struct lbm_config {
   int32_t walled;        /* bit k: dimension k has a wall layer at index 0
                             and n[k]-1 (walledDimensions, misc.jl:7-31)   */
   int32_t force;         /* LBM_FORCE_*                                   */
-  int32_t reserved0;
+  int32_t flags;         /* LBM_FLAG_*                                    */
   double force_uniform[3]; /* used when force == LBM_FORCE_UNIFORM        */
   double dt;             /* timeStep                                      */
   double dx;             /* latticeParameter (the GPU backend forces dx == dt) */
@@ -121,6 +128,22 @@ int lbm_init_uniform(lbm_handle *h, double rho, const double *u);
 /* Per-node Guo force density, host FP64 (N..., dims); zeroed on walls as
  * modelInit does (gpu/initMethods.jl:643-647).  Requires LBM_FORCE_FIELD. */
 int lbm_set_force_field(lbm_handle *h, const double *force);
+
+/* The wall region of modelInit's `walls` function (gpu/initMethods.jl:
+ * 591-605): wall[n] != 0 marks node n (N..., host, Julia order) a wall
+ * node, in addition to the layers of the walled dimensions.  Wall nodes hold
+ * zero populations and zero rho and u; the fluid next to them bounces back
+ * halfway (bounceBackPrep, gpu/aux.jl:187-198).  Call before the populations
+ * are set and before particles are added. */
+int lbm_set_walls(lbm_handle *h, const uint8_t *wall);
+
+/* solidNodeVelocity of modelInit's `movingWalls` function (gpu/initMethods.jl:
+ * 607-626): u (N..., dims), host FP64.  Nodes with a non-zero velocity join
+ * the wall region; a population that bounces back off a moving wall node
+ * x - c_q takes 2 w_q / c_s^2 rho(x) (u_w(x - c_q) . c_q) more
+ * (gpu/fluids.jl:168-174).  Call before the populations are set and before
+ * particles are added. */
+int lbm_set_wall_velocity(lbm_handle *h, const double *u);
 
 /* model.distributions <- f / f <- model.distributions, host FP64 (N..., Q). */
 int lbm_set_populations(lbm_handle *h, const double *f);

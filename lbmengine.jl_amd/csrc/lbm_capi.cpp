@@ -62,6 +62,11 @@ struct lbm_handle {
   void *linkval = nullptr;   // link-slot values of the box blocks
   size_t linkval_cap = 0;    // bytes
   void *F = nullptr;  // force field (double or float)
+  // walls beyond the walled layers (lbm_set_walls, lbm_set_wall_velocity)
+  std::vector<uint8_t> wall_h;     // the `walls` region (N), empty if none
+  std::vector<double> uwall_h;     // solidNodeVelocity (N, d), empty if none
+  uint32_t *wmask_d = nullptr;     // wall bits per node (BoxGeom::wmask)
+  void *uwall_d = nullptr;         // wall velocities, compute type
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
   double *partial = nullptr;     // 2 parities x [box block][particle][6][hi, lo]
@@ -105,6 +110,10 @@ BoxGeom geom(const lbm_handle *h) {
   g.N = h->N;
   g.off32 = (uint64_t)h->N * (uint64_t)h->Q * elem_size(h->cfg.storage) <= 0xFFFFFFFFull;
   g.reserved = 0;
+  g.wmask = h->wmask_d;
+  g.uwall = h->uwall_d;
+  if (h->wmask_d) g.walled |= kWallMask;
+  if (h->uwall_d) g.walled |= kMovingWalls;
   return g;
 }
 
@@ -119,6 +128,7 @@ CollParams coll(const lbm_handle *h) {
   c.rho0 = h->cfg.rho0;
   for (int k = 0; k < 3; k++) c.F[k] = h->cfg.force == LBM_FORCE_UNIFORM ? h->cfg.force_uniform[k] : 0.0;
   c.force_mode = h->cfg.force;
+  c.incompressible = (h->cfg.flags & LBM_FLAG_INCOMPRESSIBLE) != 0;
   set_float_params(c);
   return c;
 }
@@ -255,7 +265,9 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   sp.coeff = coeff;
   sp.ls = h->cfg.dx / h->cfg.dt;
   sp.cutoff = 2.0 * h->cfg.dx;
-  sp.shell = 1.5 * h->cfg.dx;
+  // a link's fluid end lies within |c| of the sphere: sqrt(2) dx on D2Q9 and
+  // D3Q19, sqrt(3) dx on the sets with body diagonals
+  sp.shell = (h->Q == 15 || h->Q == 27 ? 1.75 : 1.5) * h->cfg.dx;
   sp.dt = h->cfg.dt;
   sp.dx = h->cfg.dx;
   for (int k = 0; k < 3; k++) {
@@ -369,6 +381,83 @@ int aux_launch(lbm_handle *h, int which, AuxArgs &a) {
   return LBM_OK;
 }
 
+// Whether node n is a wall node: a layer of a walled dimension, a node of
+// the `walls` region or a moving wall node (non-zero solidNodeVelocity).
+bool host_wall(const lbm_handle *h, int64_t n) {
+  const int64_t id[3] = {n % h->n[0], (n / h->n[0]) % h->n[1], n / ((int64_t)h->n[0] * h->n[1])};
+  for (int k = 0; k < h->dims; k++)
+    if (((h->cfg.walled >> k) & 1) && (id[k] == 0 || id[k] == h->n[k] - 1)) return true;
+  if (!h->wall_h.empty() && h->wall_h[n]) return true;
+  if (!h->uwall_h.empty())
+    for (int k = 0; k < h->dims; k++)
+      if (h->uwall_h[(size_t)k * h->N + n] != 0.0) return true;
+  return false;
+}
+
+// BoxGeom::wmask of the handle's wall region: bounceBackPrep's links
+// (gpu/aux.jl:187-198: fluid node x and wall node x - c_q) and the moving
+// walls among them.
+template <int Q>
+void wall_mask_of(const lbm_handle *h, std::vector<uint32_t> &m) {
+  using L = Lattice<Q>;
+  const int64_t N = h->N;
+  std::vector<uint8_t> wall(N), moving(N, 0);
+  for (int64_t n = 0; n < N; n++) {
+    wall[n] = host_wall(h, n);
+    if (!h->uwall_h.empty())
+      for (int k = 0; k < h->dims; k++)
+        if (h->uwall_h[(size_t)k * N + n] != 0.0) moving[n] = 1;
+  }
+  m.assign(N, 0u);
+  const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
+  for (int64_t n = 0; n < N; n++) {
+    if (wall[n]) {
+      m[n] = 1u;
+      continue;
+    }
+    const int x = n % nx, y = (n / nx) % ny, z = n / ((int64_t)nx * ny);
+    uint32_t b = 0;
+    for (int q = 1; q < Q; q++) {
+      const int xs = (x - L::cx[q] + nx) % nx, ys = (y - L::cy[q] + ny) % ny, zs = (z - L::cz[q] + nz) % nz;
+      const int64_t s = xs + (int64_t)nx * (ys + (int64_t)ny * zs);
+      if (wall[s]) {
+        b |= 1u << q;
+        if (moving[s]) b |= kMovingLinkBit;
+      }
+    }
+    m[n] = b;
+  }
+}
+
+// Rebuild the device wall bits and wall velocities from wall_h / uwall_h.
+int upload_walls(lbm_handle *h) {
+  std::vector<uint32_t> m;
+  switch (h->Q) {
+    case 9: wall_mask_of<9>(h, m); break;
+    case 15: wall_mask_of<15>(h, m); break;
+    case 19: wall_mask_of<19>(h, m); break;
+    default: wall_mask_of<27>(h, m); break;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (!h->wmask_d) HIP_TRY(hipMalloc(&h->wmask_d, (size_t)h->N * sizeof(uint32_t)), "hipMalloc wall mask");
+  HIP_TRY(hipMemcpy(h->wmask_d, m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "copy wall mask");
+  if (!h->uwall_h.empty()) {
+    const size_t cnt = (size_t)h->N * h->dims;
+    if (h->cfg.storage == LBM_STORAGE_F64) {
+      if (!h->uwall_d) HIP_TRY(hipMalloc(&h->uwall_d, cnt * sizeof(double)), "hipMalloc wall velocity");
+      HIP_TRY(hipMemcpy(h->uwall_d, h->uwall_h.data(), cnt * sizeof(double), hipMemcpyHostToDevice), "copy wall velocity");
+    } else {
+      std::vector<float> uf(h->uwall_h.begin(), h->uwall_h.end());
+      if (!h->uwall_d) HIP_TRY(hipMalloc(&h->uwall_d, cnt * sizeof(float)), "hipMalloc wall velocity");
+      HIP_TRY(hipMemcpy(h->uwall_d, uf.data(), cnt * sizeof(float), hipMemcpyHostToDevice), "copy wall velocity");
+    }
+    const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
+    if (!h->rho_side) HIP_TRY(hipMalloc(&h->rho_side, (size_t)h->N * rsize), "hipMalloc rho");
+  }
+  drop_graph(h);
+  return LBM_OK;
+}
+
 int check_particles(lbm_handle *h) {
   if (h->parts_h.empty()) return LBM_OK;
   std::vector<DevParticle> tmp(h->parts_h.size());
@@ -407,6 +496,8 @@ int lbm_destroy(lbm_handle *h) {
   for (int b = 0; b < 2; b++)
     if (h->f[b]) hipFree(h->f[b]);
   if (h->F) hipFree(h->F);
+  if (h->wmask_d) hipFree(h->wmask_d);
+  if (h->uwall_d) hipFree(h->uwall_d);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
   if (h->rho_side) hipFree(h->rho_side);
@@ -424,8 +515,11 @@ int lbm_destroy(lbm_handle *h) {
 int lbm_create(const lbm_config *cfg, lbm_handle **out) {
   if (!cfg || !out) return fail(LBM_EINVAL, "null argument");
   *out = nullptr;
-  if (!((cfg->dims == 2 && cfg->q == 9) || (cfg->dims == 3 && cfg->q == 19)))
-    return fail(LBM_EINVAL, "lattice must be D2Q9 (dims 2) or D3Q19 (dims 3)");
+  if (!((cfg->dims == 2 && cfg->q == 9) || (cfg->dims == 3 && (cfg->q == 15 || cfg->q == 19 || cfg->q == 27))))
+    return fail(LBM_EINVAL, "lattice must be D2Q9 (dims 2) or D3Q15, D3Q19, D3Q27 (dims 3)");
+  if (cfg->flags & ~LBM_FLAG_INCOMPRESSIBLE) return fail(LBM_EINVAL, "unknown configuration flags");
+  if ((cfg->flags & LBM_FLAG_INCOMPRESSIBLE) && !(cfg->rho0 > 0))
+    return fail(LBM_EINVAL, "the incompressible equilibrium needs the initial density rho0 > 0");
   if (cfg->collision != LBM_COLLISION_BGK && cfg->collision != LBM_COLLISION_TRT)
     return fail(LBM_EINVAL, "collision must be BGK or TRT");
   if (cfg->storage < LBM_STORAGE_F64 || cfg->storage > LBM_STORAGE_F16C)
@@ -519,15 +613,9 @@ int lbm_set_force_field(lbm_handle *h, const double *force) {
   const int64_t N = h->N;
   const int d = h->dims;
   std::vector<double> F(force, force + (size_t)N * d);
-  for (int64_t n = 0; n < N; n++) {
-    const int x = n % h->n[0], y = (n / h->n[0]) % h->n[1], z = n / ((int64_t)h->n[0] * h->n[1]);
-    const int id[3] = {x, y, z};
-    bool wall = false;
-    for (int k = 0; k < d; k++)
-      if (((h->cfg.walled >> k) & 1) && (id[k] == 0 || id[k] == h->n[k] - 1)) wall = true;
-    if (wall)
+  for (int64_t n = 0; n < N; n++)
+    if (host_wall(h, n))
       for (int k = 0; k < d; k++) F[(size_t)k * N + n] = 0.0;
-  }
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   if (h->cfg.storage == LBM_STORAGE_F64) {
     HIP_TRY(hipMemcpy(h->F, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice), "copy force");
@@ -536,6 +624,22 @@ int lbm_set_force_field(lbm_handle *h, const double *force) {
     HIP_TRY(hipMemcpy(h->F, Ff.data(), Ff.size() * sizeof(float), hipMemcpyHostToDevice), "copy force");
   }
   return LBM_OK;
+}
+
+int lbm_set_walls(lbm_handle *h, const uint8_t *wall) {
+  if (!h || !wall) return fail(LBM_EINVAL, "null argument");
+  if (h->populated || !h->parts_h.empty())
+    return fail(LBM_ESTATE, "walls are set before the populations and the particles");
+  h->wall_h.assign(wall, wall + h->N);
+  return upload_walls(h);
+}
+
+int lbm_set_wall_velocity(lbm_handle *h, const double *u) {
+  if (!h || !u) return fail(LBM_EINVAL, "null argument");
+  if (h->populated || !h->parts_h.empty())
+    return fail(LBM_ESTATE, "walls are set before the populations and the particles");
+  h->uwall_h.assign(u, u + (size_t)h->N * h->dims);
+  return upload_walls(h);
 }
 
 int lbm_set_populations(lbm_handle *h, const double *f) {
@@ -1586,6 +1690,7 @@ int lbm_batch_create(lbm_handle *const *boxes, int32_t n, lbm_batch **out) {
     const lbm_config &c0 = boxes[0]->cfg, &c = boxes[i]->cfg;
     if (c.q != c0.q || c.storage != c0.storage || c.collision != c0.collision || c.force != c0.force)
       return fail(LBM_EINVAL, "the boxes of a batch share lattice, storage, collision and force kinds");
+  if (boxes[0]->Q != 9 && boxes[0]->Q != 19) return fail(LBM_EINVAL, "batches run D2Q9 and D3Q19 lattices");
   }
   lbm_batch *b = new (std::nothrow) lbm_batch;
   if (!b) return fail(LBM_ENOMEM, "out of host memory");
@@ -1930,6 +2035,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
                     double *dF) {
   if (!h || !cfg) return fail(LBM_EINVAL, "null argument");
   if (h->dims != 3 || h->Q != 19) return fail(LBM_EINVAL, "the adjoint is built for D3Q19");
+  if (h->wmask_d || (h->cfg.flags & LBM_FLAG_INCOMPRESSIBLE))
+    return fail(LBM_EINVAL, "the adjoint covers walled layers and the compressible equilibrium only");
   if (h->cfg.semantics != LBM_SEMANTICS_GPU) return fail(LBM_EINVAL, "the adjoint follows the GPU semantics");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
   if (!h->inter_h.empty()) return fail(LBM_EINVAL, "the adjoint does not differentiate particle interactions");

@@ -11,43 +11,79 @@
 namespace lbm {
 
 // ---------------------------------------------------------------- lattices
-// Velocity sets in the reference's order (gpu/misc.jl:195-210,230-248):
-// rest, the axis pairs (-,+) per axis, then the comprehensions
-// [[p,q] for p in [-1,1], q in [-1,1]] with p running fastest.
+// Velocity sets in the reference's order (gpu/misc.jl:195-272): rest, the
+// axis pairs (-,+) per axis, then the comprehensions [[p,q] for p in [-1,1],
+// q in [-1,1]] (and [[p,q,r] ...]) with p running fastest.  wclass groups the
+// velocities by speed (0 rest, 1 axis, 2 face diagonal, 3 body diagonal);
+// wc(k) is the weight of class k.
 template <int Q>
 struct Lattice;
 
 template <>
 struct Lattice<9> {
-  static constexpr int D = 2;
+  static constexpr int D = 2, NC = 3;
   static constexpr int cx[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
   static constexpr int cy[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
   static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   static constexpr int opp[9] = {0, 2, 1, 4, 3, 8, 7, 6, 5};
-  // inverse weights (exact in FP32/FP64): 9/4, 9, 36
   static constexpr int wclass[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
-  __host__ __device__ static constexpr double w(int i) {
-    return wclass[i] == 0 ? 4.0 / 9.0 : (wclass[i] == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
-  }
-  __host__ __device__ static constexpr double invw(int i) {
-    return wclass[i] == 0 ? 2.25 : (wclass[i] == 1 ? 9.0 : 36.0);
-  }
+  // weights 4/9, 1/9, 1/36; inverse weights exact in FP32/FP64
+  __host__ __device__ static constexpr double wc(int k) { return k == 0 ? 4.0 / 9.0 : (k == 1 ? 1.0 / 9.0 : 1.0 / 36.0); }
+  __host__ __device__ static constexpr double invwc(int k) { return k == 0 ? 2.25 : (k == 1 ? 9.0 : 36.0); }
+  __host__ __device__ static constexpr double w(int i) { return wc(wclass[i]); }
+  __host__ __device__ static constexpr double invw(int i) { return invwc(wclass[i]); }
+};
+
+template <>
+struct Lattice<15> {
+  static constexpr int D = 3, NC = 4;  // class 2 unused (no face diagonals)
+  static constexpr int cx[15] = {0, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+  static constexpr int cy[15] = {0, 0, 0, -1, 1, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1};
+  static constexpr int cz[15] = {0, 0, 0, 0, 0, -1, 1, -1, -1, -1, -1, 1, 1, 1, 1};
+  static constexpr int opp[15] = {0, 2, 1, 4, 3, 6, 5, 14, 13, 12, 11, 10, 9, 8, 7};
+  static constexpr int wclass[15] = {0, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 3, 3, 3, 3};
+  // weights 2/9, 1/9, 1/72
+  __host__ __device__ static constexpr double wc(int k) { return k == 0 ? 2.0 / 9.0 : (k == 1 ? 1.0 / 9.0 : 1.0 / 72.0); }
+  __host__ __device__ static constexpr double invwc(int k) { return k == 0 ? 4.5 : (k == 1 ? 9.0 : 72.0); }
+  __host__ __device__ static constexpr double w(int i) { return wc(wclass[i]); }
+  __host__ __device__ static constexpr double invw(int i) { return invwc(wclass[i]); }
 };
 
 template <>
 struct Lattice<19> {
-  static constexpr int D = 3;
+  static constexpr int D = 3, NC = 3;
   static constexpr int cx[19] = {0, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
   static constexpr int cy[19] = {0, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1};
   static constexpr int cz[19] = {0, 0, 0, 0, 0, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0};
   static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
   static constexpr int wclass[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-  __host__ __device__ static constexpr double w(int i) {
-    return wclass[i] == 0 ? 1.0 / 3.0 : (wclass[i] == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+  // weights 1/3, 1/18, 1/36
+  __host__ __device__ static constexpr double wc(int k) { return k == 0 ? 1.0 / 3.0 : (k == 1 ? 1.0 / 18.0 : 1.0 / 36.0); }
+  __host__ __device__ static constexpr double invwc(int k) { return k == 0 ? 3.0 : (k == 1 ? 18.0 : 36.0); }
+  __host__ __device__ static constexpr double w(int i) { return wc(wclass[i]); }
+  __host__ __device__ static constexpr double invw(int i) { return invwc(wclass[i]); }
+};
+
+template <>
+struct Lattice<27> {
+  static constexpr int D = 3, NC = 4;
+  static constexpr int cx[27] = {0, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1,
+                                 -1, 1, -1, 1, -1, 1, -1, 1};
+  static constexpr int cy[27] = {0, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1,
+                                 -1, -1, 1, 1, -1, -1, 1, 1};
+  static constexpr int cz[27] = {0, 0, 0, 0, 0, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0,
+                                 -1, -1, -1, -1, 1, 1, 1, 1};
+  static constexpr int opp[27] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15,
+                                  26, 25, 24, 23, 22, 21, 20, 19};
+  static constexpr int wclass[27] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+                                     3, 3, 3, 3, 3, 3, 3, 3};
+  // weights 8/27, 2/27, 1/54, 1/216
+  __host__ __device__ static constexpr double wc(int k) {
+    return k == 0 ? 8.0 / 27.0 : (k == 1 ? 2.0 / 27.0 : (k == 2 ? 1.0 / 54.0 : 1.0 / 216.0));
   }
-  __host__ __device__ static constexpr double invw(int i) {
-    return wclass[i] == 0 ? 3.0 : (wclass[i] == 1 ? 18.0 : 36.0);
-  }
+  __host__ __device__ static constexpr double invwc(int k) { return k == 0 ? 3.375 : (k == 1 ? 13.5 : (k == 2 ? 54.0 : 216.0)); }
+  __host__ __device__ static constexpr double w(int i) { return wc(wclass[i]); }
+  __host__ __device__ static constexpr double invw(int i) { return invwc(wclass[i]); }
 };
 
 // FP16 deviation storage: h_i = half((f_i - w_i rho0) * 2^14 / w_i).
@@ -81,13 +117,25 @@ struct DevParticle {
   double force[3], torque[3];          // loads consumed by the last move
 };
 
+// Walls beyond the layers of walled dimensions (the reference's `walls` and
+// `movingWalls` functions, gpu/initMethods.jl:580-620): BoxGeom::walled
+// then carries kWallMask, and wmask[n] (node n in Julia order) holds bit 0
+// when n is a wall node, bit q (1 <= q < Q) when n is fluid and n - c_q a
+// wall node (population q bounces back), and bit 31 when one of those walls
+// moves (kMovingWalls; uwall holds their velocities).
+constexpr int32_t kWallMask = 8;
+constexpr int32_t kMovingWalls = 16;
+constexpr uint32_t kMovingLinkBit = 1u << 31;
+
 // Everything the step kernels need about one box of the lattice.
 struct BoxGeom {
   int32_t nx, ny, nz;   // nodes per dimension (nz = 1 in 2D)
-  int32_t walled;       // bit k: dimension k walled
+  int32_t walled;       // bit k: dimension k walled; kWallMask, kMovingWalls
   int64_t N;            // nx*ny*nz
   int32_t off32;        // 1: every population's byte offset in a buffer fits in 32 bits
   int32_t reserved;
+  const uint32_t *wmask;  // wall bits per node (with kWallMask), else null
+  const void *uwall;      // wall velocities (N, d), compute type (with kMovingWalls), else null
 };
 
 struct CollParams {
@@ -96,6 +144,7 @@ struct CollParams {
   double rho0;                 // deviation shift
   double F[3];                 // uniform Guo force
   int32_t force_mode;
+  int32_t incompressible;      // 1: equilibrium w rho + w rho0 (...) (gpu/fluids.jl:51), rho0 below
   // the FP32 collisions' constants, each rounded once from the above
   // (set_float_params), so that no thread repeats the conversions
   float f_ls, f_invC2s, f_invC4s, f_dt, f_half_dt, f_wp, f_wm, f_rho0;

@@ -16,6 +16,8 @@ struct Launch {
 template <int S>
 hipError_t launch_step(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
+hipError_t launch_step_q15_q27(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
+template <int S>
 hipError_t launch_batch_step(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);

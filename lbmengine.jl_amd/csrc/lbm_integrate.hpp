@@ -86,7 +86,7 @@ struct StepParticles {
   double coeff;          // dx^d / dt (gpu semantics) or dx^d (cpu semantics)
   double ls;             // lattice speed dx / dt
   double cutoff;         // 2 dx (cpu-semantics node-velocity range)
-  double shell;          // 1.5 dx: link nodes lie within r + shell of a centre
+  double shell;          // link nodes lie within r + shell of a centre (1.5 dx; 1.75 dx with body diagonals)
   double dt, dx;
   double x0[3];
   int32_t n[3];

@@ -1,36 +1,39 @@
 // lbm_kernels.hip — launchers of the step and auxiliary kernels for
 // one storage kind.  Compiled once per storage, -DLBM_STORE=0 (FP64),
-// 1 (FP32), 2 (FP16) or 3 (custom 16-bit), and per part: -DLBM_PART=0 builds the one-lattice
-// step and the auxiliary kernels, -DLBM_PART=1 the batched step.
+// 1 (FP32), 2 (FP16) or 3 (custom 16-bit), and per part: -DLBM_PART=0 builds
+// the one-lattice step of D2Q9/D3Q19 and the auxiliary kernels of every
+// lattice, -DLBM_PART=1 the batched step (D2Q9/D3Q19), -DLBM_PART=2 the
+// one-lattice step of D3Q15/D3Q27 (separate objects compile in parallel).
 #include "lbm_dispatch.hpp"
 
 #ifndef LBM_STORE
 #error "LBM_STORE must be 0 (FP64), 1 (FP32), 2 (FP16) or 3 (custom 16-bit)"
 #endif
 #ifndef LBM_PART
-#error "LBM_PART must be 0 (one lattice and auxiliary kernels) or 1 (batched step)"
+#error "LBM_PART must be 0 (one lattice and auxiliary kernels), 1 (batched step) or 2 (D3Q15/D3Q27 step)"
 #endif
 
 namespace lbm {
 
 constexpr int S = LBM_STORE;
 
-// (lattice, collision, force) -> the launcher instantiated for them
-#define LBM_KEY_SWITCH(FN, ...)                          \
-  switch ((Q == 19 ? 100 : 0) + coll * 10 + fm) {        \
-    case 0: return FN<9, 0, 0>(__VA_ARGS__);             \
-    case 1: return FN<9, 0, 1>(__VA_ARGS__);             \
-    case 2: return FN<9, 0, 2>(__VA_ARGS__);             \
-    case 10: return FN<9, 1, 0>(__VA_ARGS__);            \
-    case 11: return FN<9, 1, 1>(__VA_ARGS__);            \
-    case 12: return FN<9, 1, 2>(__VA_ARGS__);            \
-    case 100: return FN<19, 0, 0>(__VA_ARGS__);          \
-    case 101: return FN<19, 0, 1>(__VA_ARGS__);          \
-    case 102: return FN<19, 0, 2>(__VA_ARGS__);          \
-    case 110: return FN<19, 1, 0>(__VA_ARGS__);          \
-    case 111: return FN<19, 1, 1>(__VA_ARGS__);          \
-    case 112: return FN<19, 1, 2>(__VA_ARGS__);          \
-    default: return hipErrorInvalidValue;                \
+// (lattice, collision, force) -> the launcher instantiated for them, for
+// the two lattices QA, QB of this part
+#define LBM_KEY_SWITCH(QA, QB, FN, ...)                   \
+  switch ((Q == QB ? 100 : (Q == QA ? 0 : 1000)) + coll * 10 + fm) { \
+    case 0: return FN<QA, 0, 0>(__VA_ARGS__);             \
+    case 1: return FN<QA, 0, 1>(__VA_ARGS__);             \
+    case 2: return FN<QA, 0, 2>(__VA_ARGS__);             \
+    case 10: return FN<QA, 1, 0>(__VA_ARGS__);            \
+    case 11: return FN<QA, 1, 1>(__VA_ARGS__);            \
+    case 12: return FN<QA, 1, 2>(__VA_ARGS__);            \
+    case 100: return FN<QB, 0, 0>(__VA_ARGS__);           \
+    case 101: return FN<QB, 0, 1>(__VA_ARGS__);           \
+    case 102: return FN<QB, 0, 2>(__VA_ARGS__);           \
+    case 110: return FN<QB, 1, 0>(__VA_ARGS__);           \
+    case 111: return FN<QB, 1, 1>(__VA_ARGS__);           \
+    case 112: return FN<QB, 1, 2>(__VA_ARGS__);           \
+    default: return hipErrorInvalidValue;                 \
   }
 
 #define LBM_MODE_SWITCH(KERNEL, ARGS)                                                                        \
@@ -52,7 +55,19 @@ static hipError_t launch_batch_qcf(int mode, const BatchArgs &a, const Launch &l
 
 template <>
 hipError_t launch_batch_step<S>(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l) {
-  LBM_KEY_SWITCH(launch_batch_qcf, mode, a, l)
+  LBM_KEY_SWITCH(9, 19, launch_batch_qcf, mode, a, l)
+}
+
+#elif LBM_PART == 2
+
+template <int Q, int COLL, int FM>
+static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) {
+  LBM_MODE_SWITCH(step_kernel, a)
+}
+
+template <>
+hipError_t launch_step_q15_q27<S>(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l) {
+  LBM_KEY_SWITCH(15, 27, launch_step_qcf, mode, a, l)
 }
 
 #else
@@ -64,15 +79,19 @@ static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) 
 
 template <>
 hipError_t launch_step<S>(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l) {
-  LBM_KEY_SWITCH(launch_step_qcf, mode, a, l)
+  if (Q == 15 || Q == 27) return launch_step_q15_q27<S>(Q, coll, fm, mode, a, l);
+  LBM_KEY_SWITCH(9, 19, launch_step_qcf, mode, a, l)
 }
 
-#define LBM_DISPATCH_Q(KERNEL, ARGS)                                                \
-  do {                                                                              \
-    if (Q == 9)                                                                     \
-      hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, l.shmem, l.stream, ARGS);       \
-    else                                                                            \
-      hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, l.shmem, l.stream, ARGS);      \
+#define LBM_DISPATCH_Q(KERNEL, ARGS)                                                         \
+  do {                                                                                       \
+    switch (Q) {                                                                             \
+      case 9: hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+      case 15: hipLaunchKernelGGL((KERNEL<15, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 19: hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 27: hipLaunchKernelGGL((KERNEL<27, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      default: return hipErrorInvalidValue;                                                   \
+    }                                                                                        \
   } while (0)
 
 template <>

@@ -57,10 +57,13 @@ hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t strea
 }
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l) {
-  if (Q == 9)
-    hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, l.shmem, l.stream, a);
-  else
-    hipLaunchKernelGGL(link_mask_kernel<19>, l.grid, l.block, l.shmem, l.stream, a);
+  switch (Q) {
+    case 9: hipLaunchKernelGGL(link_mask_kernel<9>, l.grid, l.block, l.shmem, l.stream, a); break;
+    case 15: hipLaunchKernelGGL(link_mask_kernel<15>, l.grid, l.block, l.shmem, l.stream, a); break;
+    case 19: hipLaunchKernelGGL(link_mask_kernel<19>, l.grid, l.block, l.shmem, l.stream, a); break;
+    case 27: hipLaunchKernelGGL(link_mask_kernel<27>, l.grid, l.block, l.shmem, l.stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -95,19 +95,6 @@ __device__ __forceinline__ typename Store<S>::T encode(typename Store<S>::R v, i
     return v;
 }
 
-// encode() for a moving population (weight class 1 or 2) known at run time.
-template <int S, int Q>
-__device__ __forceinline__ typename Store<S>::T encode_class(typename Store<S>::R v, uint32_t cls) {
-  if constexpr (S == kF16)
-    return static_cast<_Float16>(
-        v * static_cast<float>(kF16Scale * (cls == 1 ? Lattice<Q>::invw(1) : Lattice<Q>::invw(Q - 1))));
-  else if constexpr (S == kF16C)
-    return f16c_from_scaled(
-        v * static_cast<float>(kF16CDown * (cls == 1 ? Lattice<Q>::invw(1) : Lattice<Q>::invw(Q - 1))));
-  else
-    return v;
-}
-
 // Population f_i in FP64 from a stored element.
 template <int S, int Q>
 __device__ __forceinline__ double to_f64(typename Store<S>::T v, int i, double rho0) {
@@ -190,7 +177,8 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
     udc_[q] = udc;
     const double first = udc * c.invC2s + (udc * udc) * (0.5 * c.invC4s);
     const double second = first - uu * (0.5 * c.invC2s);
-    fe[q] = L::w(q) * ((second + 1.0) * rho);
+    // compressible: w ((second + 1) rho); incompressible: w rho + w (rho0 second)
+    fe[q] = c.incompressible ? L::w(q) * rho + L::w(q) * (c.rho0 * second) : L::w(q) * ((second + 1.0) * rho);
   }
   // collisionStep (gpu/fluids.jl:58-101) with guoForcingTerm (:103-127)
   const double cb = -c.dt / c.tau;
@@ -271,6 +259,8 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
 #pragma unroll
   for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
   rho_out = rho;
+  // density multiplying the velocity terms of the equilibrium
+  const float rv = c.incompressible ? c.f_rho0 : rho;
   float uu = 0.0f;
 #pragma unroll
   for (int k = 0; k < D; k++) uu += u[k] * u[k];
@@ -287,7 +277,7 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
   // Rest population (self-conjugate)
   {
     const float w0 = static_cast<float>(L::w(0));
-    const float dfe = w0 * (drho + rho * base);
+    const float dfe = w0 * (drho + rv * base);
     float S = 0.0f;
     if constexpr (FM != LBM_FORCE_NONE) S = w0 * dt * gp * (-uF * invC2s);
     ds[0] = d[0] - wp * (d[0] - dfe) + S;
@@ -299,8 +289,8 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
     if (o < q) continue;
     const float w = static_cast<float>(L::w(q));
     const float cu = ls * (L::cx[q] * u[0] + L::cy[q] * u[1] + (D == 3 ? L::cz[q] * u[2] : 0.0f));
-    const float esym = w * (drho + rho * (0.5f * invC4s * cu * cu + base));
-    const float eanti = w * rho * invC2s * cu;
+    const float esym = w * (drho + rv * (0.5f * invC4s * cu * cu + base));
+    const float eanti = w * rv * invC2s * cu;
     float Ssym = 0.0f, Santi = 0.0f;
     if constexpr (FM != LBM_FORCE_NONE) {
       const float cF =
@@ -339,46 +329,59 @@ __device__ __forceinline__ typename Store<S>::T h16_store(float v) {
     return static_cast<_Float16>(v);
 }
 
+// Whether some velocity of the set has weight class k.
+template <int Q>
+constexpr bool lattice_has_class(int k) {
+  for (int i = 0; i < Q; i++)
+    if (Lattice<Q>::wclass[i] == k) return true;
+  return false;
+}
+
 template <int Q, int COLL, int FM, int SK = kF16>
 __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q], const float *Fn, const CollParams &c,
                                             typename Store<SK>::T (&hs)[Q], float &rho_out) {
 #pragma clang fp contract(off)
   using L = Lattice<Q>;
-  constexpr int D = L::D;
-  // stored unit -> deviation (s) and back (is)
+  constexpr int D = L::D, NC = L::NC;
+  // stored unit -> deviation (s) and back (is), per weight class
   constexpr double kUnit = SK == kF16C ? kF16CUp : 1.0 / kF16Scale;
-  constexpr float s0 = static_cast<float>(L::w(0) * kUnit);
-  constexpr float s1 = static_cast<float>(L::w(1) * kUnit);
-  constexpr float s2 = static_cast<float>(L::w(Q - 1) * kUnit);
-  constexpr float is0 = static_cast<float>(L::invw(0) / kUnit);
-  constexpr float is1 = static_cast<float>(L::invw(1) / kUnit);
-  constexpr float is2 = static_cast<float>(L::invw(Q - 1) / kUnit);
+  constexpr float sc[4] = {static_cast<float>(L::wc(0) * kUnit), static_cast<float>(L::wc(1) * kUnit),
+                           static_cast<float>(L::wc(NC > 2 ? 2 : 1) * kUnit),
+                           static_cast<float>(L::wc(NC > 3 ? 3 : 1) * kUnit)};
+  constexpr float isc[4] = {static_cast<float>(L::invwc(0) / kUnit), static_cast<float>(L::invwc(1) / kUnit),
+                            static_cast<float>(L::invwc(NC > 2 ? 2 : 1) / kUnit),
+                            static_cast<float>(L::invwc(NC > 3 ? 3 : 1) / kUnit)};
   const float ls = c.f_ls;
   const float invC2s = c.f_invC2s;
   const float invC4s = c.f_invC4s;
   // class sums of the stored values and of their first moments
-  float S1 = 0.0f, S2 = 0.0f, A1[3] = {0.0f, 0.0f, 0.0f}, A2[3] = {0.0f, 0.0f, 0.0f};
+  float Sc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, Ac[4][3] = {};
 #pragma unroll
   for (int q = 1; q < Q; q++) {
     const float v = h16_value<SK>(h[q]);
     const int cq[3] = {L::cx[q], L::cy[q], L::cz[q]};
-    if (L::wclass[q] == 1) {
-      S1 = fmaf(v, 1.0f, S1);
+    const int k = L::wclass[q];
+    Sc[k] = fmaf(v, 1.0f, Sc[k]);
 #pragma unroll
-      for (int k = 0; k < D; k++)
-        if (cq[k]) A1[k] = fmaf(v, cq[k] > 0 ? 1.0f : -1.0f, A1[k]);
-    } else {
-      S2 = fmaf(v, 1.0f, S2);
-#pragma unroll
-      for (int k = 0; k < D; k++)
-        if (cq[k]) A2[k] = fmaf(v, cq[k] > 0 ? 1.0f : -1.0f, A2[k]);
-    }
+    for (int a = 0; a < D; a++)
+      if (cq[a]) Ac[k][a] = fmaf(v, cq[a] > 0 ? 1.0f : -1.0f, Ac[k][a]);
   }
   const float h0 = h16_value<SK>(h[0]);
-  const float drho = fmaf(s2, S2, fmaf(s1, S1, s0 * h0));
+  float drho = sc[0] * h0;
+#pragma unroll
+  for (int k = 1; k < 4; k++)
+    if (lattice_has_class<Q>(k)) drho = fmaf(sc[k], Sc[k], drho);
   float j[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int k = 0; k < D; k++) j[k] = fmaf(s2, A2[k], s1 * A1[k]);
+  for (int a = 0; a < D; a++) {
+    bool first = true;
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+      if (lattice_has_class<Q>(k)) {
+        j[a] = first ? sc[k] * Ac[k][a] : fmaf(sc[k], Ac[k][a], j[a]);
+        first = false;
+      }
+  }
   float F[3] = {0.0f, 0.0f, 0.0f};
   if constexpr (FM != LBM_FORCE_NONE) {
     const float half_dt = c.f_half_dt;
@@ -394,6 +397,8 @@ __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q],
 #pragma unroll
   for (int k = 0; k < D; k++) u[k] = j[k] * inv;
   rho_out = rho;
+  // density multiplying the velocity terms of the equilibrium
+  const float rv = c.incompressible ? c.f_rho0 : rho;
   float uu = 0.0f;
 #pragma unroll
   for (int k = 0; k < D; k++) uu = fmaf(u[k], u[k], uu);
@@ -410,27 +415,27 @@ __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q],
   {
     // rest population (self-conjugate)
     const float w0 = static_cast<float>(L::w(0));
-    const float dfe = w0 * fmaf(rho, base, drho);
+    const float dfe = w0 * fmaf(rv, base, drho);
     float S = 0.0f;
     if constexpr (FM != LBM_FORCE_NONE) S = (w0 * dt * gp) * (-uF * invC2s);
-    const float delta = fmaf(-wp, fmaf(s0, h0, -dfe), S);
-    hs[0] = h16_store<SK>(fmaf(delta, is0, h0));
+    const float delta = fmaf(-wp, fmaf(sc[0], h0, -dfe), S);
+    hs[0] = h16_store<SK>(fmaf(delta, isc[0], h0));
   }
   // conjugate pairs (q, q̄), q < q̄: symmetric and antisymmetric parts
 #pragma unroll
   for (int q = 1; q < Q; q++) {
     const int o = L::opp[q];
     if (o < q) continue;
-    const bool c1 = L::wclass[q] == 1;
+    const int k = L::wclass[q];
     const float w = static_cast<float>(L::w(q));
-    const float sq = c1 ? s1 : s2, isq = c1 ? is1 : is2;
+    const float sq = sc[k], isq = isc[k];
     float cu = 0.0f;
     if (L::cx[q]) cu = L::cx[q] > 0 ? cu + u[0] : cu - u[0];
     if (L::cy[q]) cu = L::cy[q] > 0 ? cu + u[1] : cu - u[1];
     if (D == 3 && L::cz[q]) cu = L::cz[q] > 0 ? cu + u[2] : cu - u[2];
     cu = ls * cu;
-    const float esym = w * fmaf(rho, fmaf(0.5f * invC4s, cu * cu, base), drho);
-    const float eanti = (w * rho * invC2s) * cu;
+    const float esym = w * fmaf(rv, fmaf(0.5f * invC4s, cu * cu, base), drho);
+    const float eanti = (w * rv * invC2s) * cu;
     float Ssym = 0.0f, Santi = 0.0f;
     if constexpr (FM != LBM_FORCE_NONE) {
       float cF = 0.0f;
@@ -479,10 +484,56 @@ struct NodeIdx {
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// wallNodes (gpu/misc.jl:7-31): the first and last layer of each walled
+// dimension; with a wall mask, its bit 0 (which includes those layers).
 __device__ __forceinline__ bool is_wall_node(const BoxGeom &g, int x, int y, int z) {
+  if (g.walled & kWallMask) return g.wmask[(int64_t)x + g.nx * ((int64_t)y + (int64_t)g.ny * z)] & 1u;
   return ((g.walled & 1) && (x == 0 || x == g.nx - 1)) ||
          ((g.walled & 2) && (y == 0 || y == g.ny - 1)) ||
          ((g.walled & 4) && (z == 0 || z == g.nz - 1));
+}
+
+// The wall bits of node (x, y, z) in the layout of BoxGeom::wmask: bit 0 the
+// node is a wall, bit q population q bounces back (x - c_q is a wall and x
+// is not), kMovingLinkBit a moving wall among them.  Without a mask, from
+// the walled layers (halfway bounce-back next to them).
+template <int Q>
+__device__ __forceinline__ uint32_t wall_bits(const BoxGeom &g, int x, int y, int z) {
+  using L = Lattice<Q>;
+  if (g.walled & kWallMask) return g.wmask[(int64_t)x + g.nx * ((int64_t)y + (int64_t)g.ny * z)];
+  if (is_wall_node(g, x, y, z)) return 1u;
+  const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
+  const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
+  const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 1; q < Q; q++) {
+    const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+    const bool bb = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) || (cz > 0 && zl) ||
+                    (cz < 0 && zh);
+    m |= (uint32_t)bb << q;
+  }
+  return m;
+}
+
+// The moving-wall term of a bounced population q at fluid node (x, y, z):
+// 2 w_q / c_s^2 rho(x) (u_w(x - c_q) . c_q) (gpu/fluids.jl:168-174), in the
+// reference's order of operations; rho is the node's massDensity.
+template <int Q, typename R>
+__device__ __forceinline__ R moving_wall_term(const BoxGeom &g, const CollParams &c, int q, int x, int y, int z,
+                                             R rho) {
+#pragma clang fp contract(off)
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const int xs = wrapi(x - L::cx[q], g.nx), ys = wrapi(y - L::cy[q], g.ny), zs = wrapi(z - L::cz[q], g.nz);
+  const int64_t nw = (int64_t)xs + g.nx * ((int64_t)ys + (int64_t)g.ny * zs);
+  const R *uw = static_cast<const R *>(g.uwall);
+  const int ck[3] = {L::cx[q], L::cy[q], L::cz[q]};
+  double udc = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) udc = udc + static_cast<double>(uw[k * g.N + nw]) * (ck[k] * c.ls);
+  const double lc = 2.0 * L::w(q) * c.invC2s;
+  return static_cast<R>((lc * static_cast<double>(rho)) * udc);
 }
 
 // (X + (-R))_1^2 + ... at node (x, y, z) (gpu/kernels.jl:344-353)
@@ -618,10 +669,10 @@ __device__ __forceinline__ bool owned(const DevParticle &P, const BoxGeom &g, in
 // Band of nodes the box blocks take for a particle: the nodes of its owned
 // region whose distance from its centre as read at the kernel's start (in
 // node units) lies in [r - kBandIn, r + kBandOut]; the tiles take every other
-// node.  Link fluid nodes lie within [r, r + sqrt(2)) of their tick's centre
-// and a move is below half a node, so the band holds this kernel's links and
-// the nodes whose rho the next kernel's links read, with a quarter-node
-// margin either side.  Tiles and box blocks evaluate the same arithmetic on
+// node.  Link fluid nodes lie within [r, r + |c|) of their tick's centre (|c|
+// at most sqrt(2), or sqrt(3) on the sets with body diagonals) and a move is
+// below half a node, so the band holds this kernel's links and the nodes
+// whose rho the next kernel's links read, with a margin either side.  Tiles and box blocks evaluate the same arithmetic on
 // the same record, so they agree on every node.  A particle whose record is
 // marked whole gives its box blocks its whole owned region instead (radii
 // whose band rows would not fit the blocks' LDS).
@@ -967,7 +1018,8 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   if (bid == 0 && tid < sp.np) sp.parts_out[tid].pending = 1;
   // coordinate k of the node at unwrapped index i (bl_k - 1 <= i <= bl_k + extent)
   auto crd = [&](int k, int i) { return cst[k * cstride + (i - bl[k] + 1)]; };
-  const bool wall = mine && is_wall_node(g, x, y, z);
+  const uint32_t wb = mine ? wall_bits<Q>(g, x, y, z) : 0u;
+  const bool wall = wb & 1u;
   const bool node = mine && !wall;
   const int row = y + g.ny * z;
   const T *__restrict__ fin = static_cast<const T *>(a.fin);
@@ -982,7 +1034,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const int64_t own_idx = (int64_t)row * Q * g.nx + x;  // population 0 of this node
 
   // Geometry of this tick (bounceBackPrep with the getSphere test): only nodes
-  // within 1.5 dx of a sphere surface can be a link's fluid end.
+  // within the shell (sp.shell) of a sphere surface can be a link's fluid end.
   uint32_t own_any = 0;
   uint32_t own[LBM_MAX_PARTICLES];
 #pragma unroll
@@ -1058,7 +1110,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   if (lane == 63 || tid == nthreads - 1) wave_cnt[wave] = incl;
   // rho of this node's populations of the previous tick (massDensity), kept
   // by the collision that produced them
-  rho_l[tid] = own_any ? rho_side[(int64_t)x + g.nx * (int64_t)row] : R(0);
+  rho_l[tid] = (own_any || (wb & kMovingLinkBit)) ? rho_side[(int64_t)x + g.nx * (int64_t)row] : R(0);
   if (tid < 6 * LBM_MAX_PARTICLES) blk[tid / 6][tid % 6] = DD{0.0, 0.0};
   __syncthreads();
   uint32_t off = incl - cnt, total = 0;
@@ -1076,17 +1128,20 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   {
     const GatherRows gr = gather_rows(g, Q, y, z);
     const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
-    const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-    const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-    const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
+    const uint32_t bbm = own_any | wb;  // particle links and wall links
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-      const bool bb = ((own_any >> q) & 1) || (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) ||
-                      (cy < 0 && yh) || (cz > 0 && zl) || (cz < 0 && zh);
+      const bool bb = (bbm >> q) & 1u;
       const int64_t src = bb ? own_idx + (int64_t)L::opp[q] * g.nx
                              : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
       bbv[q] = node ? decode<S, Q>(fin[src], q) : R(0);
+    }
+    if (node && (wb & kMovingLinkBit)) {
+      // moving walls at the wall links (gpu/fluids.jl:168-174)
+#pragma unroll
+      for (int q = 1; q < Q; q++)
+        if ((wb >> q) & 1u) bbv[q] = bbv[q] + moving_wall_term<Q, R>(g, a.c, q, x, y, z, rho_l[tid]);
     }
   }
   for (uint32_t c0 = 0; c0 < total; c0 += cap) {
@@ -1135,7 +1190,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         const int lx = lx0 + (int)(nd & 1023), ly = ly0 + (int)((nd >> 10) & 1023), lz = lz0 + (int)(nd >> 20);
         const int c0 = (int)((pk >> 14) & 3) - 1, c1 = (int)((pk >> 16) & 3) - 1,
                   c2 = (int)((pk >> 18) & 3) - 1;
-        const double wq = ((pk >> 20) & 3) == 1 ? L::w(1) : L::w(Q - 1);
+        const double wq = L::wc((int)((pk >> 20) & 3));
         const double ci[3] = {c0 * a.c.ls, c1 * a.c.ls, c2 * a.c.ls};
         const double lc = 2.0 * wq * a.c.invC2s;
         const double shift = S == kF64 ? 0.0 : 2.0 * wq * a.c.rho0;
@@ -1288,10 +1343,23 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
     }
     return in;
   };
+  // whether no node of this tile is a wall or next to one: walled layers
+  // only, and the tile inside planes 2 .. n - 3 of every walled dimension
+  auto clear_of_walls = [&]() {
+    if (!g.walled) return true;
+    if (g.walled & (kWallMask | kMovingWalls)) return false;
+    const int lo[3] = {tpx * a.t.bx, tpy * a.t.by, tpz * a.t.bz}, ext[3] = {a.t.bx, a.t.by, a.t.bz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if ((g.walled >> k) & 1) ok = ok & (lo[k] >= 2) & (lo[k] + ext[k] - 1 <= nn[k] - 3);
+    return ok;
+  };
   if constexpr (MODE == kFused) {
-    if (!g.walled) {
-      // The run's repeated kernel, no walls: the node's Q gathers go out
-      // first and the particles' regions (scalar, one load each) are
+    if (clear_of_walls()) {
+      // The run's repeated kernel away from walls: the node's Q gathers go
+      // out first and the particles' regions (scalar, one load each) are
       // checked while they are in flight.  A tile that touches none of
       // them finishes here; one that does takes the general path below.
       // Offsets in I: 32 bits when every population's byte offset fits
@@ -1336,11 +1404,13 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   if (MODE != kCollide && boxed) return;  // the box blocks' node
   const int row = y + g.ny * z;
   const int64_t own = (int64_t)row * Q * g.nx + x;
+  const int64_t n = (int64_t)x + g.nx * (int64_t)row;
   const T *__restrict__ fin = static_cast<const T *>(a.fin);
   T *__restrict__ fout = static_cast<T *>(a.fout);
 
-  if (is_wall_node(g, x, y, z)) {
-    // wall layers hold f = 0 (`streamed[wallRegion] .= 0`)
+  const uint32_t wb = wall_bits<Q>(g, x, y, z);
+  if (wb & 1u) {
+    // wall nodes hold f = 0 (`streamed[wallRegion] .= 0`)
 #pragma unroll
     for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], zero_pop<Q, S>(a.c, q));
     return;
@@ -1351,28 +1421,42 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
 #pragma unroll
     for (int q = 0; q < Q; q++) raw[q] = fin[own + (int64_t)q * g.nx];
   } else {
-    // gather f_q(x) = f*_q(x - c_q)
+    // gather f_q(x) = f*_q(x - c_q); where x - c_q is a wall node, halfway
+    // bounce-back f_q(x) = f*_{q̄}(x) (`streamed[cIR] = f*_{q̄}[cIR]`,
+    // gpu/fluids.jl:165)
     const GatherRows gr = gather_rows(g, Q, y, z);
     const int xs[3] = {x == g.nx - 1 ? 0 : x + 1, x, x == 0 ? g.nx - 1 : x - 1};
-    if (!g.walled) {
 #pragma unroll
-      for (int q = 0; q < Q; q++) raw[q] = fin[gr.r[L::cy[q] + 1][L::cz[q] + 1] + (int64_t)q * g.nx + xs[L::cx[q] + 1]];
-    } else {
-      const bool xl = (g.walled & 1) && x == 1, xh = (g.walled & 1) && x == g.nx - 2;
-      const bool yl = (g.walled & 2) && y == 1, yh = (g.walled & 2) && y == g.ny - 2;
-      const bool zl = (g.walled & 4) && z == 1, zh = (g.walled & 4) && z == g.nz - 2;
-#pragma unroll
-      for (int q = 0; q < Q; q++) {
-        const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-        // x - c_q in a wall layer: halfway bounce-back f_q(x) = f*_{q̄}(x)
-        // (`streamed[cIR] = f*_{q̄}[cIR]`, gpu/fluids.jl:165)
-        const bool bb = (cx > 0 && xl) || (cx < 0 && xh) || (cy > 0 && yl) || (cy < 0 && yh) ||
-                        (cz > 0 && zl) || (cz < 0 && zh);
-        const int64_t src = bb ? own + (int64_t)L::opp[q] * g.nx
-                               : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
-        raw[q] = fin[src];
-      }
+    for (int q = 0; q < Q; q++) {
+      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int64_t src = ((wb >> q) & 1u) ? own + (int64_t)L::opp[q] * g.nx
+                                            : gr.r[cy + 1][cz + 1] + (int64_t)q * g.nx + xs[cx + 1];
+      raw[q] = fin[src];
     }
+  }
+
+  R *rho_side = static_cast<R *>(a.rho_side);
+  if (MODE != kCollide && (wb & kMovingLinkBit)) {
+    // moving walls: the bounced populations also take the wall's momentum
+    // (gpu/fluids.jl:168-174), with rho(x) of the node's last collision
+    const R rx = rho_side[n];
+    R f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      f[q] = decode<S, Q>(raw[q], q);
+      if ((wb >> q) & 1u) f[q] = f[q] + moving_wall_term<Q, R>(g, a.c, q, x, y, z, rx);
+    }
+    if constexpr (MODE == kGather) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], encode<S, Q>(f[q], q));
+    } else {
+      R fs[Q], rho;
+      collide_node<Q, S, COLL, FM>(a, n, f, fs, rho);
+#pragma unroll
+      for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], encode<S, Q>(fs[q], q));
+      rho_side[n] = rho;
+    }
+    return;
   }
 
   if constexpr (MODE == kGather) {
@@ -1380,13 +1464,14 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
     for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], raw[q]);
   } else {
     R Fn[3], rho;
-    node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)row, Fn);
+    node_force<Q, S, FM>(a, n, Fn);
     T out[Q];
     collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
 #pragma unroll
     for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], out[q]);
-    // rho of the links' next gather (only the collide-only kernel reaches box nodes)
-    if (MODE == kCollide && boxed) static_cast<R *>(a.rho_side)[(int64_t)x + g.nx * (int64_t)row] = rho;
+    // rho of the next gather's links: particle links (only the collide-only
+    // kernel reaches box nodes) and moving walls
+    if ((MODE == kCollide && boxed) || (wb & kMovingLinkBit)) rho_side[n] = rho;
   }
 }
 
@@ -1587,7 +1672,7 @@ __global__ void hydro_kernel(AuxArgs a) {
 // viscousStressTensor (cpu/fluids.jl:507-520) on the current populations, in
 // FP64: sigma_ab = -(1 - dt/(2 tau)) sum_i c_ia c_ib (f_i - feq_i), with
 // feq from rho and u as equilibriumDistribution builds it
-// (cpu/fluids.jl:46-63) and the integer c_i in the sum.  a.out64 receives
+// (cpu/fluids.jl:46-63, either form) and the integer c_i in the sum.  a.out64 receives
 // sigma as (N, D, D), column major: sigma_ab of node n at n + N (a + D b).
 template <int Q, int S>
 __global__ void stress_kernel(AuxArgs a) {
@@ -1612,7 +1697,8 @@ __global__ void stress_kernel(AuxArgs a) {
     for (int k = 1; k < D; k++) udc = udc + u[k] * (static_cast<double>(c[k]) * a.c.ls);
     const double first = udc * a.c.invC2s + udc * udc * (0.5 * a.c.invC4s);
     const double second = first - uu * (0.5 * a.c.invC2s);
-    const double feq = L::w(q) * ((second + 1.0) * rho);
+    const double feq = a.c.incompressible ? L::w(q) * rho + L::w(q) * (a.c.rho0 * second)
+                                          : L::w(q) * ((second + 1.0) * rho);
     const double neq = f[q] - feq;
 #pragma unroll
     for (int al = 0; al < D; al++)

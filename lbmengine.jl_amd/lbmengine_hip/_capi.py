@@ -19,6 +19,7 @@ COLLISION = {"bgk": 0, "trt": 1}
 STORAGE = {"f64": 0, "f32": 1, "f16": 2, "f16c": 3}
 SEMANTICS = {"gpu": 0, "cpu": 1}
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
+FLAG_INCOMPRESSIBLE = 1
 BEAD, SQUIRMER = 0, 1
 MAX_PARTICLES = 8
 INTERACTION = {"linear": 0, "bistable": 1, "polar": 2, "dipole": 3, "lennardJones": 4}
@@ -39,7 +40,7 @@ class LbmConfig(C.Structure):
     _fields_ = [
         ("dims", C.c_int32), ("q", C.c_int32), ("n", C.c_int64 * 3),
         ("collision", C.c_int32), ("storage", C.c_int32), ("semantics", C.c_int32),
-        ("walled", C.c_int32), ("force", C.c_int32), ("reserved0", C.c_int32),
+        ("walled", C.c_int32), ("force", C.c_int32), ("flags", C.c_int32),
         ("force_uniform", C.c_double * 3),
         ("dt", C.c_double), ("dx", C.c_double), ("tau", C.c_double), ("tau_minus", C.c_double),
         ("rho0", C.c_double),
@@ -78,6 +79,8 @@ SIGNATURES = {
     "lbm_destroy": [_vp],
     "lbm_init_uniform": [_vp, C.c_double, _dp],
     "lbm_set_force_field": [_vp, _dp],
+    "lbm_set_walls": [_vp, C.POINTER(C.c_uint8)],
+    "lbm_set_wall_velocity": [_vp, _dp],
     "lbm_set_populations": [_vp, _dp],
     "lbm_get_populations": [_vp, _dp],
     "lbm_get_hydro": [_vp, _dp, _dp],

@@ -89,6 +89,9 @@ class ResolvedModel:
     init_rho: Any          # number or (N...) field
     init_u: Any            # (d,) vector or (N..., d) field
     fluidParams: dict = field(default_factory=dict)
+    incompressible: bool = False
+    wall_region: Optional[np.ndarray] = None    # the `walls` function's region (N...)
+    wall_velocity: Optional[np.ndarray] = None  # the `movingWalls` function's field (N..., d)
 
     @property
     def N(self) -> int:
@@ -110,6 +113,7 @@ class ResolvedModel:
         cfg.semantics = _capi.SEMANTICS[self.semantics]
         cfg.walled = self.walled
         cfg.force = self.force_mode
+        cfg.flags = _capi.FLAG_INCOMPRESSIBLE if self.incompressible else 0
         for k in range(3):
             cfg.force_uniform[k] = float(self.force_uniform[k]) if k < len(self.force_uniform) else 0.0
         cfg.dt, cfg.dx, cfg.tau, cfg.tau_minus, cfg.rho0 = self.dt, self.dx, self.tau, self.tau_minus, self.rho0
@@ -121,7 +125,9 @@ class ResolvedModel:
         return cfg, keep
 
     def wall_mask(self) -> np.ndarray:
-        """wallNodes (gpu/misc.jl:7-31): first and last layer of each walled dim."""
+        """wallRegion: wallNodes (gpu/misc.jl:7-31, the first and last layer of
+        each walled dim), the `walls` region and the moving walls' nodes
+        (gpu/initMethods.jl:580-620)."""
         m = np.zeros(self.n[: self.dims], dtype=bool, order="F")
         for k in range(self.dims):
             if (self.walled >> k) & 1:
@@ -130,6 +136,10 @@ class ResolvedModel:
                 m[tuple(idx)] = True
                 idx[k] = self.n[k] - 1
                 m[tuple(idx)] = True
+        if self.wall_region is not None:
+            m |= self.wall_region
+        if self.wall_velocity is not None:
+            m |= np.sqrt((self.wall_velocity ** 2).sum(axis=-1)) > 0
         return m
 
     def initial_populations(self) -> np.ndarray:
@@ -213,10 +223,6 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
 
     if dampenEcho:
         raise NotImplementedError(":cbc is not implemented in the gpu modules")
-    if walls is not DEFAULT or movingWalls is not DEFAULT:
-        raise NotImplementedError("walls/movingWalls functions are not implemented in the hip backend")
-    if not isFluidCompressible:
-        raise NotImplementedError("the incompressible equilibrium is not implemented in the hip backend")
     forcingScheme = "guo" if forcingScheme is DEFAULT else _sym(forcingScheme)
     if forcingScheme != "guo":
         raise NotImplementedError(f"forcing scheme {forcingScheme} is not implemented in the hip backend")
@@ -237,7 +243,7 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
         init_u = np.array(fluidVelocity, dtype=np.float64, order="F")
 
     if velocities is DEFAULT:
-        assert dims > 1, "1 dimensional LBM is not implemented in CUDA!"
+        assert dims > 1, "1 dimensional LBM is not implemented in the hip backend!"
         assert dims <= 3
         velocities = ["D1Q3", "D2Q9", "D3Q15"][dims - 1]
     velocities = _sym(velocities)
@@ -272,6 +278,31 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
             walled |= 1 << (d_ - 1)
         schemes.append("bounceBack")
 
+    # walls and moving walls given as functions of the node coordinates
+    # (gpu/initMethods.jl:591-626), evaluated over the grid x fastest
+    wall_region = wall_velocity = None
+    if walls is not DEFAULT:
+        assert callable(walls), "walls must be a function!"
+        vals = [walls(*pt) for pt in _grid_points(coords, shp)]
+        assert all(isinstance(v, (bool, np.bool_)) for v in vals), "walls must return boolean values!"
+        wall_region = np.array(vals, dtype=bool).reshape(shp, order="F")
+        schemes.append("bounceBack")
+    if movingWalls is not DEFAULT:
+        assert callable(movingWalls), "movingWalls must be a function!"
+        vals = [np.asarray(movingWalls(*pt), dtype=np.float64).ravel() for pt in _grid_points(coords, shp)]
+        assert len(vals[0]) == dims, f"movingWalls must return a vector of dimension {dims}!"
+        wall_velocity = np.stack(vals).reshape(shp + (dims,), order="F")
+        schemes += ["movingWalls", "bounceBack"]
+
+    incompressible = not isFluidCompressible
+    fluidParams["isFluidCompressible"] = bool(isFluidCompressible)
+    if incompressible:
+        # equilibriumDistribution's w rho + w rho0 (...) (gpu/fluids.jl:51)
+        # with rho0 the initial density, uniform here (the deviation shift)
+        if np.ndim(init_rho) != 0:
+            raise NotImplementedError("the incompressible equilibrium needs a uniform initial density in the hip backend")
+        rho0 = float(init_rho)
+
     force_mode, force_uniform, force_field = _capi.FORCE_NONE, np.zeros(3), None
     if forceDensity is not DEFAULT:
         fd = np.asarray(forceDensity, dtype=np.float64)
@@ -298,7 +329,15 @@ def resolve_model(*, massDensity=DEFAULT, fluidVelocity=DEFAULT, velocities=DEFA
         dt=dt, dx=dx, tau=float(relaxationTime), tau_minus=float(tau_minus),
         viscosity=float(viscosity), kinetic=float(kin) if not isinstance(kin, str) else float("nan"),
         rho0=float(rho0), latticeSpeed=latticeSpeed, schemes=schemes, init_rho=init_rho, init_u=init_u,
-        fluidParams=fluidParams)
+        fluidParams=fluidParams, incompressible=incompressible, wall_region=wall_region,
+        wall_velocity=wall_velocity)
+
+
+def _grid_points(coords, shp):
+    """The node coordinates of the grid in Julia's order (x fastest)."""
+    grids = np.meshgrid(*coords, indexing="ij")
+    flat = [g.ravel(order="F") for g in grids]
+    return [tuple(float(c[i]) for c in flat) for i in range(int(np.prod(shp)))]
 
 
 def sphere_count(rm: ResolvedModel, radius: float) -> int:
@@ -446,6 +485,18 @@ class _Particle:
         return node, dirs
 
 
+def set_walls(fn: dict, h, rm: ResolvedModel) -> None:
+    """The `walls` region and the `movingWalls` velocities of a resolved
+    model, through lbm_set_walls / lbm_set_wall_velocity (before the
+    populations)."""
+    if rm.wall_region is not None:
+        w = np.asfortranarray(rm.wall_region, dtype=np.uint8)
+        _capi.check(fn["lbm_set_walls"](h, w.ctypes.data_as(C.POINTER(C.c_uint8))), fn)
+    if rm.wall_velocity is not None:
+        u = np.asfortranarray(rm.wall_velocity, dtype=np.float64)
+        _capi.check(fn["lbm_set_wall_velocity"](h, _capi.as_dptr(u)), fn)
+
+
 class LBMmodel:
     """A lattice on one GPU; mirrors the fields of the reference's LBMmodel
     (gpu/structs.jl:207-231) as read-through properties."""
@@ -460,6 +511,7 @@ class LBMmodel:
         self.particles: list[_Particle] = []
         self.particleInteractions: list = []
         self.schemes = list(rm.schemes)
+        set_walls(self._fn, h, rm)
         if rm.force_mode == _capi.FORCE_FIELD:
             F = np.asfortranarray(rm.force_field, dtype=np.float64)
             _capi.check(self._fn["lbm_set_force_field"](h, _capi.as_dptr(F)))

@@ -24,7 +24,7 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 
 from lbmengine_hip import _capi  # noqa: E402
 from lbmengine_hip.model import (DEFAULT, ResolvedModel, propagate_ticks, resolve_model,  # noqa: E402
-                                 resolve_particle)
+                                 resolve_particle, set_walls)
 
 _LIB = None
 _FN: dict = {}
@@ -118,6 +118,7 @@ class OracleModel:
         self._h = h
         self.particles = []
         self.particleInteractions = []
+        set_walls(self._fn, h, rm)
         if rm.force_mode == _capi.FORCE_FIELD:
             F = np.asfortranarray(rm.force_field, dtype=np.float64)
             _check(self._fn["lbm_set_force_field"](h, _capi.as_dptr(F)))

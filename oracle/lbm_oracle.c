@@ -23,7 +23,8 @@
  *   viscousStressTensor   cpu/fluids.jl:507-520 (nonEquilibriumDistribution :65-67)
  *   findInitialConditions gpu/initMethods.jl:417-438
  *   wallNodes             gpu/misc.jl:7-31
- *   D2Q9, D3Q19           gpu/misc.jl:195-210,230-248
+ *   walls, movingWalls    gpu/initMethods.jl:591-626, gpu/fluids.jl:168-174
+ *   D2Q9, D3Q15, D3Q19, D3Q27  gpu/misc.jl:195-272
  *
  * Summation order: where the reference sums with CUDA.jl/BLAS reductions of
  * unspecified order (sum over q for rho, the momentum-exchange sums, norm,
@@ -41,7 +42,7 @@
 
 #include "../include/lbm_hip.h"
 
-#define OMAXQ 19
+#define OMAXQ 27
 
 static __thread char g_err[512];
 
@@ -96,6 +97,8 @@ typedef struct lbmo_model {
   double *F;          /* forceDensity (d x N) when :guo                 */
   uint8_t *wall;      /* boundaryConditionsParams.wallRegion            */
   uint8_t *sir_wall;  /* boundaryConditionsParams.streamingInvasionRegions */
+  double *solid_vel;  /* boundaryConditionsParams.solidNodeVelocity, d x N (movingWalls) */
+  int incompressible; /* fluidParams.isFluidCompressible == false       */
   int guo, bounce_back, ladd;
   int np;
   opart parts[LBM_MAX_PARTICLES];
@@ -111,16 +114,20 @@ typedef struct lbmo_model {
 
 /* ---------------------------------------------------------------- tables */
 
-/* D2Q9 and D3Q19 in the reference's order (gpu/misc.jl:195-248): the
- * comprehensions [[p,q] for p in [-1,1], q in [-1,1]] run p fastest. */
+/* The velocity sets in the reference's order (gpu/misc.jl:195-272): rest,
+ * the axis pairs, then comprehensions [[p,q] for p in [-1,1], q in [-1,1]]
+ * (and [[p,q,r] ...]) run p fastest. */
 static void tables(lbmo_model *m) {
   static const int c2[9][2] = {{0, 0},  {-1, 0}, {1, 0},  {0, -1}, {0, 1},
                                {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
-  static const int c3[19][3] = {
+  /* rest, axes, edges of the yz, xz, xy planes, corners */
+  static const int c3[27][3] = {
       {0, 0, 0},   {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},
       {0, 0, -1},  {0, 0, 1},   {0, -1, -1}, {0, 1, -1}, {0, -1, 1},
       {0, 1, 1},   {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
-      {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0}, {1, 1, 0}};
+      {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0}, {1, 1, 0},
+      {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1},
+      {-1, -1, 1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1}};
   memset(m->c, 0, sizeof m->c);
   if (m->Q == 9) {
     for (int i = 0; i < 9; i++) {
@@ -128,10 +135,21 @@ static void tables(lbmo_model *m) {
       m->c[i][1] = c2[i][1];
       m->w[i] = i == 0 ? 4.0 / 9.0 : (i < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
     }
-  } else {
+  } else if (m->Q == 15) { /* rest, axes, corners: 2/9, 1/9, 1/72 */
+    for (int i = 0; i < 15; i++) {
+      const int src = i < 7 ? i : i + 12;
+      for (int k = 0; k < 3; k++) m->c[i][k] = c3[src][k];
+      m->w[i] = i == 0 ? 2.0 / 9.0 : (i < 7 ? 1.0 / 9.0 : 1.0 / 72.0);
+    }
+  } else if (m->Q == 19) { /* 1/3, 1/18, 1/36 */
     for (int i = 0; i < 19; i++) {
       for (int k = 0; k < 3; k++) m->c[i][k] = c3[i][k];
       m->w[i] = i == 0 ? 1.0 / 3.0 : (i < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+    }
+  } else { /* 8/27, 2/27, 1/54, 1/216 */
+    for (int i = 0; i < 27; i++) {
+      for (int k = 0; k < 3; k++) m->c[i][k] = c3[i][k];
+      m->w[i] = i == 0 ? 8.0 / 27.0 : (i < 7 ? 2.0 / 27.0 : (i < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
     }
   }
   /* oppositeVectorId = [findfirst(x -> x == -1 .* c, cs) for c in cs] */
@@ -207,8 +225,9 @@ static void hydro_update(lbmo_model *m) {
 
 /* ----------------------------------------------------------- collision */
 
-/* equilibriumDistribution (gpu/fluids.jl:39-52), compressible form:
- * w_i ((u.c_i invC2 + (u.c_i)^2 invC4/2 - u.u invC2/2) + 1) rho          */
+/* equilibriumDistribution (gpu/fluids.jl:39-52): compressible
+ * w_i ((u.c_i invC2 + (u.c_i)^2 invC4/2 - u.u invC2/2) + 1) rho, or
+ * incompressible w_i rho + w_i (rho0 (...)) with the initial density rho0 */
 static inline double feq_one(const lbmo_model *m, int q, double rho,
                              const double *u) {
   double ci[3], udc = 0.0, uu = 0.0;
@@ -217,6 +236,7 @@ static inline double feq_one(const lbmo_model *m, int q, double rho,
   for (int k = 0; k < m->d; k++) uu = uu + u[k] * u[k];
   double first = udc * m->invC2s + (udc * udc) * (0.5 * m->invC4s);
   double second = first - uu * (0.5 * m->invC2s);
+  if (m->incompressible) return m->w[q] * rho + m->w[q] * (m->cfg.rho0 * second);
   return m->w[q] * ((second + 1.0) * rho);
 }
 
@@ -682,6 +702,14 @@ static void tick(lbmo_model *m) {
         for (int p = 0; p < m->np; p++) in = in || m->parts[p].sir[(int64_t)o * m->N + x];
       /* streamed[cIR] = f*_{q̄}[cIR] */
       if (in) st[x] = fso[x];
+      /* moving walls: streamed[cIR] += (2 w_q invC2) rho[cIR] .* (u_w(x - c_q) . c_q)
+       * (gpu/fluids.jl:168-174) */
+      if (in && m->solid_vel) {
+        const int64_t y = shifted(m, x, m->c[q]);
+        double udc = 0.0;
+        for (int k = 0; k < d; k++) udc = udc + m->solid_vel[(int64_t)k * m->N + y] * ((double)m->c[q][k] * m->ls);
+        st[x] = st[x] + ((2.0 * m->w[q] * m->invC2s) * m->rho[x]) * udc;
+      }
     }
     if (!m->ladd) continue;
     double ci[3] = {0, 0, 0};
@@ -782,7 +810,7 @@ int lbmo_destroy(lbmo_model *m) {
   if (!m) return LBM_OK;
   for (int k = 0; k < 3; k++) free(m->coords[k]);
   free(m->X); free(m->f); free(m->fs); free(m->fn); free(m->rho); free(m->u);
-  free(m->F); free(m->wall); free(m->sir_wall);
+  free(m->F); free(m->wall); free(m->sir_wall); free(m->solid_vel);
   for (int p = 0; p < m->np; p++) {
     free(m->parts[p].solid); free(m->parts[p].sir); free(m->parts[p].node_vel);
   }
@@ -792,13 +820,14 @@ int lbmo_destroy(lbmo_model *m) {
 
 int lbmo_create(const lbm_config *cfg, lbmo_model **out) {
   if (!cfg || !out) return fail(LBM_EINVAL, "null argument");
-  if (!((cfg->dims == 2 && cfg->q == 9) || (cfg->dims == 3 && cfg->q == 19)))
-    return fail(LBM_EINVAL, "lattice must be D2Q9 or D3Q19");
+  if (!((cfg->dims == 2 && cfg->q == 9) || (cfg->dims == 3 && (cfg->q == 15 || cfg->q == 19 || cfg->q == 27))))
+    return fail(LBM_EINVAL, "lattice must be D2Q9, D3Q15, D3Q19 or D3Q27");
   lbmo_model *m = (lbmo_model *)calloc(1, sizeof *m);
   if (!m) return fail(LBM_ENOMEM, "out of memory");
   m->cfg = *cfg;
   m->d = cfg->dims;
   m->Q = cfg->q;
+  m->incompressible = (cfg->flags & LBM_FLAG_INCOMPRESSIBLE) != 0;
   m->n[0] = cfg->n[0];
   m->n[1] = cfg->n[1];
   m->n[2] = cfg->dims == 3 ? cfg->n[2] : 1;
@@ -845,6 +874,43 @@ int lbmo_create(const lbm_config *cfg, lbmo_model **out) {
           m->F[(int64_t)k * N + x] = m->wall[x] ? 0.0 : cfg->force_uniform[k];
   }
   *out = m;
+  return LBM_OK;
+}
+
+/* forceDensity[wallRegion] .= 0 (gpu/initMethods.jl:643-647) */
+static void zero_force_on_walls(lbmo_model *m) {
+  if (!m->guo) return;
+  for (int64_t x = 0; x < m->N; x++)
+    if (m->wall[x])
+      for (int k = 0; k < m->d; k++) m->F[(int64_t)k * m->N + x] = 0.0;
+}
+
+/* `walls` (gpu/initMethods.jl:591-605): wallRegion .|| the given region;
+ * the invasion regions are those of the final region (:622-627). */
+int lbmo_set_walls(lbmo_model *m, const uint8_t *wall) {
+  for (int64_t x = 0; x < m->N; x++)
+    if (wall[x]) m->wall[x] = 1;
+  m->bounce_back = 1;
+  bounce_back_prep(m, m->wall, m->sir_wall);
+  zero_force_on_walls(m);
+  return LBM_OK;
+}
+
+/* `movingWalls` (gpu/initMethods.jl:607-626): solidNodeVelocity = the given
+ * field; its non-zero nodes join the wall region. */
+int lbmo_set_wall_velocity(lbmo_model *m, const double *u) {
+  free(m->solid_vel);
+  m->solid_vel = (double *)xcalloc((size_t)(m->d * m->N), sizeof(double));
+  if (!m->solid_vel) return fail(LBM_ENOMEM, "out of memory");
+  memcpy(m->solid_vel, u, (size_t)(m->d * m->N) * sizeof(double));
+  for (int64_t x = 0; x < m->N; x++) {
+    double nn = 0.0;
+    for (int k = 0; k < m->d; k++) nn = nn + u[(int64_t)k * m->N + x] * u[(int64_t)k * m->N + x];
+    if (sqrt(nn) > 0) m->wall[x] = 1;
+  }
+  m->bounce_back = 1;
+  bounce_back_prep(m, m->wall, m->sir_wall);
+  zero_force_on_walls(m);
   return LBM_OK;
 }
 

@@ -3,7 +3,7 @@
 // random particles, lattice spacings and dimensions:
 //  - the offset tables + band_node list each band node exactly once, and
 //    only band nodes (the box blocks' enumeration equals the tiles' test);
-//  - every node at distance [r - 1, r + sqrt(2) + 1) from the centre is in
+//  - every node at distance [r - 1, r + sqrt(3) + 1) from the centre is in
 //    the band (this tick's links and the nodes whose rho the next tick's
 //    links read, for moves below half a node);
 //  - the band never holds more nodes than the host's bound for its radius.
@@ -93,7 +93,7 @@ int main(int argc, char **argv) {
             d2 += u * u;
           }
           const double d = std::sqrt(d2);
-          if (d >= rn - 1.0 && d < rn + std::sqrt(2.0) + 1.0 && !in) {
+          if (d >= rn - 1.0 && d < rn + std::sqrt(3.0) + 1.0 && !in) {
             std::printf("case %d: node at distance %.4f (r = %.4f) is outside the band\n", trial, d, rn);
             return 1;
           }

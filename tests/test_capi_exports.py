@@ -43,7 +43,11 @@ def test_bad_configuration_is_rejected_without_device(hiplib):
     fn = _capi.load()
     rm = resolve_model(dims=3, velocities="D3Q19", latticeParameter=1.0, relaxationTimeRatio=0.8, xlims=(0, 7))
     cfg, keep = rm.config()
-    cfg.q = 15
+    cfg.q = 13
     h = ctypes.c_void_p()
     assert fn["lbm_create"](ctypes.byref(cfg), ctypes.byref(h)) == _capi.LBM_EINVAL
     assert b"D3Q19" in fn["lbm_last_error"]()
+    cfg.q = 19
+    cfg.flags = 4  # no such flag
+    assert fn["lbm_create"](ctypes.byref(cfg), ctypes.byref(h)) == _capi.LBM_EINVAL
+    assert b"flags" in fn["lbm_last_error"]()

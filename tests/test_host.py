@@ -46,9 +46,38 @@ def test_debug_kinetic_parameter_makes_trt_bgk():
     assert math.isclose(rm.tau_minus, rm.tau, rel_tol=1e-14)
 
 
-def test_default_velocity_set_d3q15_is_refused():
+def test_default_velocity_set_is_d3q15():
+    # gpu/initMethods.jl:534: D3Q15 unless velocities is given
+    rm = resolve_model(dims=3, latticeParameter=1.0, xlims=(0, 7))
+    assert rm.velocities == "D3Q15" and rm.Q == 15
+
+
+def test_walls_functions_resolution():
+    # walls(x, y, z) and movingWalls(x, y, z) over the grid, x fastest
+    # (gpu/initMethods.jl:591-626); moving nodes join the wall region
+    rm = resolve_model(dims=3, velocities="D3Q27", latticeParameter=1.0, xlims=(0, 5), ylims=(0, 4), zlims=(0, 3),
+                       walls=lambda x, y, z: x == 0 and y == 2,
+                       movingWalls=lambda x, y, z: [0.1, 0.0, 0.0] if z == 3 else [0.0, 0.0, 0.0])
+    assert rm.wall_region.shape == (6, 5, 4) and rm.wall_region[0, 2, :].all() and rm.wall_region.sum() == 4
+    assert rm.wall_velocity.shape == (6, 5, 4, 3) and np.all(rm.wall_velocity[:, :, 3, 0] == 0.1)
+    m = rm.wall_mask()
+    assert m[:, :, 3].all() and m[0, 2, :].all() and m.sum() == 6 * 5 + 3
+    assert "movingWalls" in rm.schemes and "bounceBack" in rm.schemes
+    with pytest.raises(AssertionError):
+        resolve_model(dims=3, velocities="D3Q19", latticeParameter=1.0, xlims=(0, 3), walls=lambda x, y, z: 1)
+    with pytest.raises(AssertionError):
+        resolve_model(dims=3, velocities="D3Q19", latticeParameter=1.0, xlims=(0, 3),
+                      movingWalls=lambda x, y, z: [0.0, 0.0])
+
+
+def test_incompressible_resolution():
+    rm = resolve_model(dims=2, velocities="D2Q9", latticeParameter=1.0, xlims=(0, 7), massDensity=1.3,
+                       isFluidCompressible=False)
+    cfg, _ = rm.config()
+    assert rm.incompressible and cfg.flags == _capi.FLAG_INCOMPRESSIBLE and cfg.rho0 == 1.3
     with pytest.raises(NotImplementedError):
-        resolve_model(dims=3, latticeParameter=1.0, xlims=(0, 7))
+        resolve_model(dims=2, velocities="D2Q9", latticeParameter=1.0, xlims=(0, 3),
+                      massDensity=np.ones((4, 4)), isFluidCompressible=False)
 
 
 def test_walls_and_force_resolution():

@@ -5,7 +5,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from lbmengine_hip.lattice import d2q9, d3q19, velocity_set
+from lbmengine_hip.lattice import d2q9, d3q15, d3q19, d3q27, velocity_set
 
 
 def test_d2q9_order_and_opposites():
@@ -26,7 +26,22 @@ def test_d3q19_order_and_opposites():
     assert (opp + 1).tolist() == [1, 3, 2, 5, 4, 7, 6, 11, 10, 9, 8, 15, 14, 13, 12, 19, 18, 17, 16]
 
 
-@pytest.mark.parametrize("make", [d2q9, d3q19])
+def test_d3q15_d3q27_order_and_opposites():
+    # gpu/misc.jl:212-272: rest, axes, [edges as D3Q19], then [[p,q,r] ...]
+    # with p fastest
+    corners = [[p, q, r] for r in (-1, 1) for q in (-1, 1) for p in (-1, 1)]
+    cs15, w15, opp15 = velocity_set("D3Q15")
+    assert cs15[7:].tolist() == corners
+    assert np.allclose(w15, [2 / 9] + [1 / 9] * 6 + [1 / 72] * 8, rtol=0, atol=0)
+    assert (opp15 + 1).tolist() == [1, 3, 2, 5, 4, 7, 6, 15, 14, 13, 12, 11, 10, 9, 8]
+    cs27, w27, opp27 = velocity_set("D3Q27")
+    cs19, _, _ = velocity_set("D3Q19")
+    assert cs27[:19].tolist() == cs19.tolist() and cs27[19:].tolist() == corners
+    assert np.allclose(w27, [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8, rtol=0, atol=0)
+    assert all((cs27[opp27[i]] == -cs27[i]).all() for i in range(27))
+
+
+@pytest.mark.parametrize("make", [d2q9, d3q15, d3q19, d3q27])
 def test_isotropy_exact(make):
     cs, ws = make()
     d = cs.shape[1]
@@ -52,4 +67,4 @@ def test_isotropy_exact(make):
 
 def test_unsupported_sets_raise():
     with pytest.raises(NotImplementedError):
-        velocity_set("D3Q15")
+        velocity_set("D1Q3")
