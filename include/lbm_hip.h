@@ -237,6 +237,15 @@ int lbm_write_tensor_csv(const char *path, int32_t dims, const int64_t n[3], int
  * (N..., dims, dims), column major, FP64 host. */
 int lbm_viscous_stress(lbm_handle *h, double *sigma);
 
+/* Health check of the current populations (no counterpart in the
+ * reference, which checks nothing while it runs): *nonfinite receives the
+ * number of NaN or infinite populations of fluid nodes, *saturated the
+ * number held at the storage's largest magnitude (F16C's clamp: a
+ * deviation beyond |f - w rho0| >= w/8, outside the storage's range; 0 for
+ * the other storages).  One read of the populations; waits for the stream.
+ * Either pointer may be NULL. */
+int lbm_check_populations(lbm_handle *h, int64_t *nonfinite, int64_t *saturated);
+
 /* Boundary links of particle id in the reference's findall order over its
  * streamingInvasionRegions (gpu/aux.jl:187-198, gpu/fluids.jl:181): link
  * (node, j) means node is fluid and node + c_j is solid.  Writes at most cap

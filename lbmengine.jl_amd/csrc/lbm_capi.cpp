@@ -66,6 +66,7 @@ struct lbm_handle {
   std::vector<uint8_t> wall_h;     // the `walls` region (N), empty if none
   std::vector<double> uwall_h;     // solidNodeVelocity (N, d), empty if none
   uint32_t *wmask_d = nullptr;     // wall bits per node (BoxGeom::wmask)
+  unsigned long long *scan_d = nullptr;  // lbm_check_populations' two counters
   void *uwall_d = nullptr;         // wall velocities, compute type
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
@@ -497,6 +498,7 @@ int lbm_destroy(lbm_handle *h) {
     if (h->f[b]) hipFree(h->f[b]);
   if (h->F) hipFree(h->F);
   if (h->wmask_d) hipFree(h->wmask_d);
+  if (h->scan_d) hipFree(h->scan_d);
   if (h->uwall_d) hipFree(h->uwall_d);
   if (h->parts_d) hipFree(h->parts_d);
   if (h->partial) hipFree(h->partial);
@@ -700,6 +702,34 @@ int lbm_get_hydro(lbm_handle *h, double *rho, double *u) {
   hipFree(tmp);
   if (rc) return rc;
   if (e != hipSuccess) return hip_fail(e, "get hydro");
+  return LBM_OK;
+}
+
+int lbm_check_populations(lbm_handle *h, int64_t *nonfinite, int64_t *saturated) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
+  if (!h->scan_d) HIP_TRY(hipMalloc(&h->scan_d, 2 * sizeof(unsigned long long)), "hipMalloc scan counters");
+  HIP_TRY(hipMemsetAsync(h->scan_d, 0, 2 * sizeof(unsigned long long), h->stream), "clear scan counters");
+  AuxArgs a{};
+  a.f = h->f[h->cur];
+  a.g = geom(h);
+  a.c = coll(h);
+  const int threads = 256;
+  const int64_t want = (h->N * h->Q + threads - 1) / threads;
+  Launch l{dim3((unsigned)std::min<int64_t>(want, 4096)), dim3(threads), h->stream};
+  hipError_t e = hipErrorInvalidValue;
+  switch (h->cfg.storage) {
+    case LBM_STORAGE_F64: e = launch_scan<kF64>(h->Q, a, h->scan_d, l); break;
+    case LBM_STORAGE_F32: e = launch_scan<kF32>(h->Q, a, h->scan_d, l); break;
+    case LBM_STORAGE_F16C: e = launch_scan<kF16C>(h->Q, a, h->scan_d, l); break;
+    default: e = launch_scan<kF16>(h->Q, a, h->scan_d, l); break;
+  }
+  if (e != hipSuccess) return hip_fail(e, "scan launch");
+  unsigned long long c[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(c, h->scan_d, sizeof c, hipMemcpyDeviceToHost, h->stream), "copy scan counters");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (nonfinite) *nonfinite = (int64_t)c[0];
+  if (saturated) *saturated = (int64_t)c[1];
   return LBM_OK;
 }
 

@@ -29,6 +29,8 @@ template <int S>
 hipError_t launch_hydro(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_stress(int Q, const AuxArgs &a, const Launch &l);
+template <int S>
+hipError_t launch_scan(int Q, const AuxArgs &a, unsigned long long *counts, const Launch &l);
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);

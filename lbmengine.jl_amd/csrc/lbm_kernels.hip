@@ -83,13 +83,13 @@ hipError_t launch_step<S>(int Q, int coll, int fm, int mode, const StepArgs &a, 
   LBM_KEY_SWITCH(9, 19, launch_step_qcf, mode, a, l)
 }
 
-#define LBM_DISPATCH_Q(KERNEL, ARGS)                                                         \
+#define LBM_DISPATCH_Q(KERNEL, ...)                                                          \
   do {                                                                                       \
     switch (Q) {                                                                             \
-      case 9: hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
-      case 15: hipLaunchKernelGGL((KERNEL<15, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 19: hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-      case 27: hipLaunchKernelGGL((KERNEL<27, S>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+      case 9: hipLaunchKernelGGL((KERNEL<9, S>), l.grid, l.block, l.shmem, l.stream, __VA_ARGS__); break;   \
+      case 15: hipLaunchKernelGGL((KERNEL<15, S>), l.grid, l.block, l.shmem, l.stream, __VA_ARGS__); break; \
+      case 19: hipLaunchKernelGGL((KERNEL<19, S>), l.grid, l.block, l.shmem, l.stream, __VA_ARGS__); break; \
+      case 27: hipLaunchKernelGGL((KERNEL<27, S>), l.grid, l.block, l.shmem, l.stream, __VA_ARGS__); break; \
       default: return hipErrorInvalidValue;                                                   \
     }                                                                                        \
   } while (0)
@@ -121,6 +121,12 @@ hipError_t launch_hydro<S>(int Q, const AuxArgs &a, const Launch &l) {
 template <>
 hipError_t launch_stress<S>(int Q, const AuxArgs &a, const Launch &l) {
   LBM_DISPATCH_Q(stress_kernel, a);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_scan<S>(int Q, const AuxArgs &a, unsigned long long *counts, const Launch &l) {
+  LBM_DISPATCH_Q(scan_populations_kernel, a, counts);
   return hipGetLastError();
 }
 

@@ -1601,6 +1601,42 @@ __global__ void get_populations_kernel(AuxArgs a) {
     a.out64[q * a.g.N + n] = wall ? 0.0 : to_f64<S, Q>(f[pop_index(a.g, Q, q, x, row)], q, a.c.rho0);
 }
 
+// Health of the stored populations: counts[0] += the non-finite ones,
+// counts[1] += those at the storage's clamp (f16c's largest magnitude),
+// over the fluid nodes (wall nodes hold no meaningful value).  Grid-stride
+// over the row-blocked buffer, one pair of atomics per block.
+template <int Q, int S>
+__global__ void scan_populations_kernel(AuxArgs a, unsigned long long *counts) {
+  using T = typename Store<S>::T;
+  const T *f = static_cast<const T *>(a.f);
+  const int64_t total = a.g.N * Q;
+  unsigned long long bad = 0, sat = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % a.g.nx);
+    const int64_t rq = e / a.g.nx;
+    const int q = (int)(rq % Q);
+    const int64_t row = rq / Q;
+    const int y = (int)(row % a.g.ny), z = (int)(row / a.g.ny);
+    if (is_wall_node(a.g, x, y, z)) continue;
+    const T v = f[e];
+    if constexpr (S == kF16C) {
+      sat += ((uint32_t)(uint16_t)v & 0x7FFFu) == 0x7FFFu;
+    } else {
+      if (!isfinite(static_cast<double>(decode<S, Q>(v, q)))) bad++;
+    }
+  }
+  __shared__ unsigned long long sb[2];
+  if (threadIdx.x == 0) sb[0] = sb[1] = 0;
+  __syncthreads();
+  if (bad) atomicAdd(&sb[0], bad);
+  if (sat) atomicAdd(&sb[1], sat);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (sb[0]) atomicAdd(&counts[0], sb[0]);
+    if (sb[1]) atomicAdd(&counts[1], sb[1]);
+  }
+}
+
 // hydroVariablesUpdate! (gpu/fluids.jl:7-37, cpu/fluids.jl:7-38) at node n
 // from its populations f (FP64, read from storage): rho and u.
 template <int Q, int S>

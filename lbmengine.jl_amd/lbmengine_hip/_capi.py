@@ -97,6 +97,7 @@ SIGNATURES = {
     "lbm_write_tensor_csv": [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.c_int64, C.c_double,
                              C.POINTER(_dp), _dp],
     "lbm_viscous_stress": [_vp, _dp],
+    "lbm_check_populations": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "lbm_run": [_vp, C.c_int64],
     "lbm_run_noalloc": [_vp, C.c_int64],
     "lbm_sync": [_vp],

@@ -623,6 +623,13 @@ class LBMmodel:
     def sync(self) -> None:
         _capi.check(self._fn["lbm_sync"](self._h))
 
+    def check(self):
+        """(non-finite populations, populations at the storage's clamp) of
+        the fluid nodes."""
+        bad, sat = C.c_int64(), C.c_int64()
+        _capi.check(self._fn["lbm_check_populations"](self._h, C.byref(bad), C.byref(sat)))
+        return bad.value, sat.value
+
     def run(self, ticks: int) -> None:
         _capi.check(self._fn["lbm_run"](self._h, int(ticks)))
 
@@ -764,6 +771,8 @@ def LBMpropagate(model, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
         n = propagate_ticks(model.models[0].resolved.dt, simulationTime, ticks)
         model.run(n)
         model.sync()
+        for m in model.models:
+            guard(m)
         if verbose:
             print(f"\r t = {round(model.models[0].time, 2)}")
         return
@@ -774,5 +783,20 @@ def LBMpropagate(model, *, simulationTime=DEFAULT, ticks=DEFAULT, verbose=False,
     else:
         model.run(n)
         model.sync()
+    guard(model)
     if verbose:
         print(f"\r t = {round(model.time, 2)}")
+
+
+def guard(model) -> None:
+    """After a run: a NaN or infinite population raises FloatingPointError
+    (the run went unstable, e.g. tau <= 1/2 or flow beyond the lattice's
+    range); populations held at the f16c storage's clamp warn.  The
+    reference checks nothing while it runs."""
+    bad, sat = model.check()
+    if bad:
+        raise FloatingPointError(f"{bad} non-finite populations after tick {model.tick}: the run went unstable")
+    if sat:
+        import warnings
+        warnings.warn(f"{sat} populations at the f16c storage's clamp after tick {model.tick}: deviations beyond "
+                      "w/8 are not representable; use f32 storage for this flow", RuntimeWarning)

@@ -109,3 +109,31 @@ def test_headline_workload_against_fp64(gpu, beta):
         print(f"  {storage}: exterior u rel L2 {e:.3e}, speed rel {es:.3e}", flush=True)
         assert e < bound and es < bound
         del m
+
+
+def test_population_guard(gpu):
+    """lbm_check_populations and LBMpropagate's guard: a NaN planted from the
+    host is reported and raises; an f16c population beyond the storage's
+    range is clamped and reported; wall nodes are not counted."""
+    import warnings
+    for storage in ("f64", "f32", "f16", "f16c"):
+        m = lh.modelInit(**KW3, xlims=(0, 7), storage=storage, walledDimensions=[3])
+        assert m.check() == (0, 0)
+        f = m.distributions
+        f[3, 4, 2, 5] = np.nan
+        m.distributions = f
+        bad, sat = m.check()
+        # (f16c stores no NaN: the encoder's clamp turns it into the clamp value)
+        assert (bad, sat) == ((0, 1) if storage == "f16c" else (1, 0))
+        if storage != "f16c":
+            with pytest.raises(FloatingPointError):
+                lh.LBMpropagate(m, ticks=2)
+    m = lh.modelInit(**KW3, xlims=(0, 7), storage="f16c")
+    f = m.distributions
+    f[1, 2, 3, 7] = f[1, 2, 3, 7] * 1.5  # h = 0.5, beyond 1/8
+    m.distributions = f
+    assert m.check() == (0, 1)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        lh.guard(m)
+    assert any("clamp" in str(x.message) for x in w)
