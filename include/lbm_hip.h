@@ -188,14 +188,28 @@ typedef struct lbm_interaction {
                         BISTABLE: trapRadius, a, b
                         POLAR: equilibriumAngle, stiffness
                         DIPOLE: dipoleConstant, scalar-field flag (1: B is
-                                the number field[0], 2D)
+                                the number field[0], 2D), schedule flag
+                                (1: B changes with time and is read, tick
+                                by tick, from lbm_set_field_schedule)
                         LENNARD_JONES: epsilon, sigma, cutoff             */
-  double field[3];   /* DIPOLE: the magnetic field B (constant in time)     */
+  double field[3];   /* DIPOLE: the magnetic field B (when not scheduled)   */
 } lbm_interaction;
 
 /* Replaces the handle's interaction terms with terms[0..n) (n = 0 clears
  * them).  GPU semantics only, as the CPU backend has no interactions. */
 int lbm_set_interactions(lbm_handle *h, const lbm_interaction *terms, int32_t n);
+
+/* A magnetic field that is a function of time (addDipoles! with
+ * magneticField::Function, gpu/initMethods.jl:337-346, evaluated as
+ * magneticField(model.time) at gpu/particles.jl:161): B[i][0..3) is the field
+ * of tick first_tick + i, i < count, that is the field at the model time
+ * that tick ends at (the time molecularDynamicsTick! sees, after tick!
+ * advanced it).  Dipole terms with the schedule flag read it; a tick whose
+ * field is zero adds no dipole force (`norm(B) ≈ 0`).  lbm_run and
+ * lbm_batch_run refuse (LBM_ESTATE) a run with such terms that reaches a
+ * tick outside the schedule.  Replaces the previous schedule; count = 0
+ * clears it.  Waits for the handle's stream. */
+int lbm_set_field_schedule(lbm_handle *h, const double *B, int64_t first_tick, int64_t count);
 
 /* ---- per-tick particle log (saveData's particlesTrj.csv,
  * gpu/fluids.jl:252-259, gpu/aux.jl:319-362) ----

@@ -94,6 +94,10 @@ struct lbm_handle {
   double *traj_d = nullptr;
   int64_t traj_cap = 0;
   int64_t traj_from = 0;  // tick at which the log was turned on
+  // scheduled magnetic field (lbm_set_field_schedule): [bsched_count][3]
+  double *bsched_d = nullptr;
+  int64_t bsched_cap = 0, bsched_first = 0, bsched_count = 0;
+  int64_t tick0 = 0;  // tick at which particle 0 was added
   // captured blocks of graph_k ticks, one per starting ping-pong buffer
   // graphs of graph_k fused kernels, per starting population and particle parity
   hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -253,6 +257,10 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   sp.ninter = (int32_t)h->inter_h.size();
   sp.traj = h->traj_d;
   sp.traj_cap = h->traj_cap;
+  sp.bsched = h->bsched_d;
+  sp.bsched_first = h->bsched_first;
+  sp.bsched_count = h->bsched_count;
+  sp.tick0 = h->tick0;
   sp.np = (int32_t)h->parts_h.size();
   sp.semantics = h->cfg.semantics;
   sp.dims = h->dims;
@@ -506,6 +514,7 @@ int lbm_destroy(lbm_handle *h) {
   if (h->linkval) hipFree(h->linkval);
   if (h->inter_d) hipFree(h->inter_d);
   if (h->traj_d) hipFree(h->traj_d);
+  if (h->bsched_d) hipFree(h->bsched_d);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -800,6 +809,7 @@ int lbm_add_particle(lbm_handle *h, const lbm_particle *p, int32_t *id) {
     h->parts_h.pop_back();
     return rc;
   }
+  if (idx == 0) h->tick0 = h->tick;
   drop_graph(h);
   if (id) *id = idx;
   return LBM_OK;
@@ -864,6 +874,26 @@ int lbm_set_interactions(lbm_handle *h, const lbm_interaction *terms, int32_t n)
   if (n > 0)
     HIP_TRY(hipMemcpy(h->inter_d, terms, sizeof(lbm_interaction) * n, hipMemcpyHostToDevice), "copy interactions");
   h->inter_h.assign(terms, terms + n);
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_set_field_schedule(lbm_handle *h, const double *B, int64_t first_tick, int64_t count) {
+  if (!h || (count > 0 && !B)) return fail(LBM_EINVAL, "null argument");
+  if (count < 0 || count > (int64_t(1) << 26)) return fail(LBM_EINVAL, "schedule length out of range");
+  for (int64_t i = 0; i < 3 * count; i++)
+    if (!std::isfinite(B[i])) return fail(LBM_EINVAL, "the scheduled field must be finite");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  if (count > h->bsched_cap) {
+    if (h->bsched_d) HIP_TRY(hipFree(h->bsched_d), "free schedule");
+    h->bsched_d = nullptr;
+    h->bsched_cap = 0;
+    HIP_TRY(hipMalloc(&h->bsched_d, sizeof(double) * 3 * count), "allocate schedule");
+    h->bsched_cap = count;
+  }
+  if (count > 0) HIP_TRY(hipMemcpy(h->bsched_d, B, sizeof(double) * 3 * count, hipMemcpyHostToDevice), "copy schedule");
+  h->bsched_first = first_tick;
+  h->bsched_count = count;
   drop_graph(h);
   return LBM_OK;
 }
@@ -1126,7 +1156,23 @@ int lbm_get_links(lbm_handle *h, int32_t id, int64_t *node, int32_t *dir, int64_
 
 namespace {
 
+// A run of nsteps ticks reads the scheduled field of ticks tick + 1 ..
+// tick + nsteps when a dipole term has the schedule flag: refuse it when the
+// schedule does not hold them all.
+int check_schedule(const lbm_handle *h, int64_t nsteps) {
+  bool scheduled = false;
+  for (const lbm_interaction &I : h->inter_h)
+    scheduled = scheduled || (I.kind == LBM_INTERACTION_DIPOLE && I.params[2] != 0.0);
+  if (!scheduled || h->parts_h.empty()) return LBM_OK;
+  const int64_t a = h->tick + 1, b = h->tick + nsteps;
+  if (h->bsched_count == 0 || a < h->bsched_first || b > h->bsched_first + h->bsched_count - 1)
+    return fail(LBM_ESTATE, "the magnetic-field schedule does not cover ticks " + std::to_string(a) + " .. " +
+                                std::to_string(b) + " (lbm_set_field_schedule)");
+  return LBM_OK;
+}
+
 int run_ticks(lbm_handle *h, int64_t nsteps, bool direct) {
+  if (const int rc = check_schedule(h, nsteps)) return rc;
   const int64_t nfused = nsteps - 1;
   const bool timing = h->timing && !direct;
   if (timing && nfused > 0) {
@@ -1755,6 +1801,8 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
   if (!b) return fail(LBM_EINVAL, "null batch");
   if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
   if (nsteps == 0) return LBM_OK;
+  for (const lbm_handle *h : b->boxes)
+    if (const int rc = check_schedule(h, nsteps)) return rc;
   if (const int rc = prepare_boxes(b)) return rc;
   const int s = b->boxes[0]->cur;
   if (const int rc = upload_table(b, s)) return rc;

@@ -75,6 +75,9 @@ struct StepParticles {
   const lbm_interaction *inter;  // particle-particle interaction terms, in application order
   double *traj;              // per-tick log: [particle][traj_cap][9], or null
   int64_t traj_cap;
+  const double *bsched;      // scheduled magnetic field: [bsched_count][3], entry i of tick bsched_first + i
+  int64_t bsched_first, bsched_count;
+  int64_t tick0;             // the tick at which particle 0 was added: a move m of it ends tick tick0 + m
   int32_t ninter;
   int32_t np, semantics, dims;
   // box blocks: particle p's are [first_block[p], first_block[p + 1]);
@@ -273,7 +276,8 @@ __host__ __device__ inline void cross_sv(double s, const double *V, double *o) {
 // in fa and force123 (on id3) in fb (id2 takes -(fa + fb)).  Returns false
 // when the term adds nothing (Lennard-Jones beyond its cutoff).
 __host__ __device__ inline bool interaction_term(const lbm_interaction &I, const double *R1, const double *R2,
-                                                 const double *R3, int d, double fa[3], double fb[3]) {
+                                                 const double *R3, int d, double fa[3], double fb[3],
+                                                 const double *Bt = nullptr) {
 #pragma clang fp contract(off)
   for (int k = 0; k < 3; k++) fa[k] = fb[k] = 0.0;
   if (I.kind == LBM_INTERACTION_POLAR) {
@@ -342,7 +346,9 @@ __host__ __device__ inline bool interaction_term(const lbm_interaction &I, const
       break;
     }
     default: {  // LBM_INTERACTION_DIPOLE, gpu/particles.jl:160-177
-      const double *B = I.field;
+      // the field of this tick (Bt, when scheduled) or the constant one
+      const double *B = Bt ? Bt : I.field;
+      if (Bt && B[0] == 0.0 && B[1] == 0.0 && B[2] == 0.0) return false;
       const double pre = -I.params[0] / julia_pow(disp, 4);
       double ccb[3] = {0, 0, 0}, BB, s2;
       if (d == 3) {
@@ -384,7 +390,15 @@ __device__ inline void add_interaction_forces(const StepParticles &sp, const Dev
     const int i3 = polar ? I.ids[2] : -1;
     if (p != i1 && p != i2 && p != i3) continue;
     double fa[3], fb[3];
-    if (!interaction_term(I, recs[i1].R, recs[i2].R, polar ? recs[i3].R : nullptr, d, fa, fb)) continue;
+    const double *Bt = nullptr;
+    if (I.kind == LBM_INTERACTION_DIPOLE && I.params[2] != 0.0) {
+      // every particle moves once a tick: the move about to be applied,
+      // particle 0's move number moves + 1, ends tick tick0 + moves + 1
+      int64_t i = sp.tick0 + recs[0].moves + 1 - sp.bsched_first;
+      i = i < 0 ? 0 : (i >= sp.bsched_count ? sp.bsched_count - 1 : i);  // the host keeps runs inside
+      Bt = sp.bsched + 3 * i;
+    }
+    if (!interaction_term(I, recs[i1].R, recs[i2].R, polar ? recs[i3].R : nullptr, d, fa, fb, Bt)) continue;
     if (p == i1)
       for (int k = 0; k < d; k++) F[k] = F[k] + fa[k];
     if (polar && p == i3)

@@ -54,7 +54,7 @@ class PolarInteraction:
 class DipoleDipoleInteraction:
     pairs: list
     dipoleConstant: float
-    magneticField: object  # a constant vector, or a number (a 2D field along z)
+    magneticField: object  # a constant vector, a number (a 2D field along z), or a function of time giving either
 
 
 @dataclass
@@ -171,9 +171,10 @@ def addDipoles(model, *ids, magneticField=(1, 0), susceptibility=0.5, permeabili
     """addDipoles! (gpu/initMethods.jl:316-359): one dipole-dipole interaction
     over every pair of the given particles (all of them when none is given);
     a later call adds its ids to the existing pairs unless rewrite.  The field
-    is constant in time: a vector, or in 2D a number (a field along z)."""
-    if callable(magneticField):
-        raise NotImplementedError("a time-dependent magnetic field is not supported by the hip backend")
+    is a vector, in 2D a number (a field along z), or a function of the model
+    time returning either; the function is evaluated at every tick's time
+    (gpu/particles.jl:161), its vectors rounded to Float32 as the
+    reference's `magneticField(t) |> cu` does (gpu/initMethods.jl:339)."""
     ids = list(ids) if ids else list(range(1, len(model.particles) + 1))
     _check_id(model, *ids)
     at = next((i for i, t in enumerate(model.particleInteractions) if isinstance(t, DipoleDipoleInteraction)), None)
@@ -186,7 +187,10 @@ def addDipoles(model, *ids, magneticField=(1, 0), susceptibility=0.5, permeabili
     if write_new:
         if rewrite and at is not None:
             del model.particleInteractions[at]
-        if np.ndim(magneticField) == 0:
+        if callable(magneticField):
+            B = magneticField
+            field_value(B, model.time, model.dims)  # shape check at the current time
+        elif np.ndim(magneticField) == 0:
             if model.dims != 2:
                 raise NotImplementedError("a number as magneticField is a 2D field")
             B = float(magneticField)
@@ -236,6 +240,43 @@ def addWCA(model, *, epsilon=DEFAULT, sigma=DEFAULT) -> None:
     addLennardJones(model, epsilon=epsilon, sigma=sigma, cutoff=2.0 ** (1 / 6) * sigma)
 
 
+def field_value(fn, t: float, dims: int):
+    """(B, scalar) of a time-dependent field at time t: a number (2D only)
+    stays a Float64 number; a vector is rounded to Float32, as `cu` rounds a
+    Float64 array (gpu/initMethods.jl:339)."""
+    v = fn(t)
+    if np.ndim(v) == 0:
+        if dims != 2:
+            raise NotImplementedError("a number as magneticField is a 2D field")
+        return np.array([float(v), 0.0, 0.0]), True
+    a = np.asarray(v)
+    assert a.shape == (dims,), f"magneticField must have {dims} components"
+    if np.issubdtype(a.dtype, np.floating):
+        a = a.astype(np.float32)
+    out = np.zeros(3)
+    out[:dims] = a.astype(np.float64)
+    return out, False
+
+
+def field_schedule(fn, time0: float, dt: float, count: int, dims: int) -> np.ndarray:
+    """The field of the next `count` ticks, (count, 3): tick k ends at the
+    time tick! reaches by adding dt k times to time0 (gpu/fluids.jl:212)."""
+    out = np.zeros((count, 3))
+    t = time0
+    for k in range(count):
+        t = t + dt
+        out[k] = field_value(fn, t, dims)[0]
+    return out
+
+
+def scheduled_field(interactions):
+    """The time-dependent field function of the interactions, or None."""
+    for it in interactions:
+        if isinstance(it, DipoleDipoleInteraction) and callable(it.magneticField):
+            return it.magneticField
+    return None
+
+
 # --- expansion into the library's terms ---------------------------------------
 def expand(interactions, dims: int):
     """The terms molecularDynamicsTick! applies, in its order
@@ -263,6 +304,12 @@ def expand(interactions, dims: int):
             term("polar", (it.id1, it.id2, it.id3), (it.equilibriumAngle, it.stiffness))
         elif isinstance(it, DipoleDipoleInteraction):
             B = it.magneticField
+            if callable(B):
+                # the field comes from the handle's schedule, tick by tick
+                scalar = field_value(B, 0.0, dims)[1]
+                for a, b in it.pairs:
+                    term("dipole", (a, b), (it.dipoleConstant, 1.0 if scalar else 0.0, 1.0))
+                continue
             scalar = np.ndim(B) == 0
             Bv = [float(B)] if scalar else [float(x) for x in B]
             if all(x == 0.0 for x in Bv):
@@ -284,3 +331,4 @@ def sync_interactions(model) -> None:
     """Hand model.particleInteractions, expanded, to the model's backend."""
     arr, n = expand(model.particleInteractions, model.dims)
     model._set_interactions(arr, n)
+    model._field_fn = scheduled_field(model.particleInteractions)

@@ -497,6 +497,10 @@ def set_walls(fn: dict, h, rm: ResolvedModel) -> None:
         _capi.check(fn["lbm_set_wall_velocity"](h, _capi.as_dptr(u)), fn)
 
 
+# ticks per uploaded chunk of a time-dependent magnetic field (24 B a tick)
+SCHEDULE_CHUNK = 1 << 16
+
+
 class LBMmodel:
     """A lattice on one GPU; mirrors the fields of the reference's LBMmodel
     (gpu/structs.jl:207-231) as read-through properties."""
@@ -631,7 +635,26 @@ class LBMmodel:
         return bad.value, sat.value
 
     def run(self, ticks: int) -> None:
-        _capi.check(self._fn["lbm_run"](self._h, int(ticks)))
+        ticks = int(ticks)
+        if getattr(self, "_field_fn", None) is None:
+            _capi.check(self._fn["lbm_run"](self._h, ticks))
+            return
+        # a time-dependent magnetic field: its values for the ticks of each
+        # chunk go to the device before the chunk runs
+        done = 0
+        while done < ticks:
+            n = min(ticks - done, SCHEDULE_CHUNK)
+            self._upload_schedule(n)
+            _capi.check(self._fn["lbm_run"](self._h, n))
+            done += n
+
+    def _upload_schedule(self, n: int) -> None:
+        from .interactions import field_schedule
+        t = C.c_int64()
+        tm = C.c_double()
+        _capi.check(self._fn["lbm_get_tick"](self._h, C.byref(t), C.byref(tm)))
+        B = np.ascontiguousarray(field_schedule(self._field_fn, tm.value, self.resolved.dt, n, self.dims))
+        _capi.check(self._fn["lbm_set_field_schedule"](self._h, _capi.as_dptr(B), t.value + 1, n))
 
     def _set_interactions(self, terms, n: int) -> None:
         _capi.check(self._fn["lbm_set_interactions"](self._h, terms, int(n)))
@@ -671,6 +694,9 @@ class LBMbatch:
             self._b = None
 
     def run(self, ticks: int) -> None:
+        for m in self.models:
+            if getattr(m, "_field_fn", None) is not None:
+                m._upload_schedule(int(ticks))
         _capi.check(self._fn["lbm_batch_run"](self._b, int(ticks)))
 
     def sync(self) -> None:

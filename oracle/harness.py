@@ -28,6 +28,7 @@ from lbmengine_hip.model import (DEFAULT, ResolvedModel, propagate_ticks, resolv
 
 _LIB = None
 _FN: dict = {}
+_FIELD_CB = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 
 def build_oracle() -> Path:
@@ -46,6 +47,8 @@ def load():
         fn.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]
         fn.restype = C.c_int
         _FN["lbm_get_solid"] = fn
+        _LIB.lbmo_set_field_fn.argtypes = [C.c_void_p, _FIELD_CB, C.c_void_p]
+        _LIB.lbmo_set_field_fn.restype = C.c_int
     return _FN
 
 
@@ -147,6 +150,34 @@ class OracleModel:
 
     def _set_interactions(self, terms, n: int) -> None:
         _check(self._fn["lbm_set_interactions"](self._h, terms, int(n)))
+
+    @property
+    def _field_fn(self):
+        return getattr(self, "_field_py", None)
+
+    @_field_fn.setter
+    def _field_fn(self, fn) -> None:
+        """A time-dependent magnetic field: the oracle calls back with
+        model.time at every tick's molecularDynamicsTick! and receives
+        magneticField(model.time) |> cu (gpu/initMethods.jl:339,
+        gpu/particles.jl:161): a number as it is, a Float64 vector rounded to
+        Float32."""
+        self._field_py = fn
+        if fn is None:
+            self._field_cb = None
+            _check(_LIB.lbmo_set_field_fn(self._h, C.cast(None, _FIELD_CB), None))
+            return
+
+        def cb(t, out, _ctx):
+            v = fn(t)
+            if np.ndim(v) == 0:
+                out[0], out[1], out[2] = float(v), 0.0, 0.0
+                return
+            for k, x in enumerate(list(v)):
+                out[k] = float(np.float32(x)) if isinstance(x, (float, np.floating)) else float(x)
+
+        self._field_cb = _FIELD_CB(cb)
+        _check(_LIB.lbmo_set_field_fn(self._h, self._field_cb, None))
 
     def add(self, kind: str, **kw):
         p = resolve_particle(self.resolved, kind=kind, **kw)

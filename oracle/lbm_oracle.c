@@ -104,6 +104,10 @@ typedef struct lbmo_model {
   opart parts[LBM_MAX_PARTICLES];
   int ninter;
   lbm_interaction inter[LBM_MAX_INTERACTIONS]; /* particleInteractions, pairs expanded */
+  /* a magnetic field that is a function of time: field_fn(model.time, B, ctx)
+   * (DipoleDipoleInteraction.magneticField(model.time), gpu/particles.jl:161) */
+  void (*field_fn)(double t, double *B, void *ctx);
+  void *field_ctx;
   int64_t tick;
   double time;
   int c[OMAXQ][3];
@@ -633,7 +637,14 @@ static void apply_interaction(lbmo_model *m, const lbm_interaction *I) {
       break;
     }
     case LBM_INTERACTION_DIPOLE: { /* :160-177, one pair */
+      double Bt[3] = {0, 0, 0};
       const double *B = I->field;
+      if (I->params[2] != 0.0) { /* B = magneticField(model.time), after tick! advanced the time */
+        if (!m->field_fn) return;
+        m->field_fn(m->time, Bt, m->field_ctx);
+        if (Bt[0] == 0.0 && Bt[1] == 0.0 && Bt[2] == 0.0) return; /* norm(B) ≈ 0 */
+        B = Bt;
+      }
       for (int k = 0; k < d; k++) u[k] = disp21[k] / disp;
       double t1[3] = {0, 0, 0}, BB, cc;
       if (d == 3) {
@@ -1029,6 +1040,12 @@ int lbmo_run(lbmo_model *m, int64_t nsteps) {
 int lbmo_get_tick(lbmo_model *m, int64_t *t, double *time) {
   *t = m->tick;
   *time = m->time;
+  return LBM_OK;
+}
+
+int lbmo_set_field_fn(lbmo_model *m, void (*fn)(double, double *, void *), void *ctx) {
+  m->field_fn = fn;
+  m->field_ctx = ctx;
   return LBM_OK;
 }
 

@@ -69,6 +69,15 @@ CASES = {
     "lennard_jones_3d": (KW3, [bead([-3.9, 0.5, 0.0]), bead([4.0, 0.0, 0.6]), bead([0.0, 8.5, -0.5])],
                          lambda it, m: it.addLennardJones(m, epsilon=0.02, sigma=7.8, cutoff=9.0)),
     "wca_2d": (KW2, [bead([-3.6, 0.0]), bead([3.6, 0.3])], lambda it, m: it.addWCA(m, epsilon=0.05)),
+    # fields that change with time: read tick by tick from the device's
+    # schedule (the oracle calls the function at every tick)
+    "dipoles_3d_rotating": (KW3, [bead([-5.0, 1.0, 0.5]), bead([4.0, -1.5, 2.0]), bead([0.5, 7.0, -4.0])],
+                            lambda it, m: it.addDipoles(m, magneticField=lambda t: [0.6 * np.cos(0.07 * t),
+                                                                                     0.6 * np.sin(0.07 * t), 0.3],
+                                                        susceptibility=0.5, permeability=1.0)),
+    "dipoles_2d_pulsed_scalar": (KW2, [bead([-5.0, 1.0]), bead([4.0, -1.5])],
+                                 lambda it, m: it.addDipoles(m, 1, 2, magneticField=lambda t: 0.9 if t % 8 < 4 else 0.0,
+                                                             permeability=1.0)),
     "all_3d": (KW3, [squirmer([-5.0, 0.5, 0.2], [0, 1, 0]), bead([3.5, -1.0, 0.4]), bead([0.5, 7.0, -3.0])],
                lambda it, m: (it.addLinearBond(m, 1, 2, stiffness=0.03, equilibriumDisplacement=8.0),
                               it.addBistableBond(m, 2, 3, hillHeight=0.2),
@@ -111,3 +120,30 @@ def test_interactions_need_gpu_semantics(gpu):
     gpu.addBead(m, radius=3.0, position=[4.0, 0.0])
     with pytest.raises(gpu.LBMError):
         it.addLinearBond(m, 1, 2)
+
+
+def test_time_dependent_field_in_chunks_and_refused_outside(gpu, oracle, monkeypatch):
+    """Runs longer than one uploaded chunk of the field re-upload it between
+    chunks; a bare lbm_run past the uploaded ticks is refused."""
+    from lbmengine_hip import model as gm
+    kw, parts, add = CASES["dipoles_3d_rotating"]
+    g, o = build(gpu, oracle, kw, parts, add)
+    monkeypatch.setattr(gm, "SCHEDULE_CHUNK", 16)
+    gpu.LBMpropagate(g, ticks=45)
+    o.run(45)
+    compare(g, o)
+    # the last chunk uploaded ticks 33 .. 45: tick 46 is outside
+    assert g._fn["lbm_run"](g._h, 1) == -4
+
+
+def test_time_dependent_field_in_a_batch(gpu, oracle):
+    kw, parts, add = CASES["dipoles_2d_pulsed_scalar"]
+    g1, o1 = build(gpu, oracle, kw, parts, add)
+    g2, o2 = build(gpu, oracle, kw, parts, lambda it, m: it.addDipoles(m, magneticField=lambda t: 0.5 + 0.01 * t,
+                                                                      permeability=1.0))
+    b = gpu.LBMbatch([g1, g2])
+    gpu.LBMpropagate(b, ticks=37)
+    o1.run(37)
+    o2.run(37)
+    compare(g1, o1)
+    compare(g2, o2)
