@@ -126,6 +126,8 @@ def batch_bench(lh, args):
         b.set_block(*block)
         if len(v) > 3:
             b.set_tile_order(v[3])
+        if len(v) > 4:
+            b.set_occupancy(v[4])
     else:
         block, tuned_ms = b.autotune(5)
     b.run(args.batch_warmup)
@@ -145,7 +147,7 @@ def batch_bench(lh, args):
            "boxes": len(boxes), "nodes": nodes, "steps": args.batch_steps, "warmup": args.batch_warmup,
            "MLUPS": round(nodes * args.batch_steps / wall / 1e6, 1),
            "ms_per_step": round(wall / args.batch_steps * 1e3, 5), "block": list(block),
-           "tile_order": b.get_tile_order(),
+           "tile_order": b.get_tile_order(), "low_occupancy": b.get_occupancy(),
            "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
            "hbm_frac": round(bytes_per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if args.batch_accuracy:
@@ -411,13 +413,13 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
-    ap.add_argument("--batch-block", default="", help="bx,by,bz[,tile order] of the batched kernel instead of "
+    ap.add_argument("--batch-block", default="", help="bx,by,bz[,tile order[,low occupancy]] of the batched kernel instead of "
                                                       "autotuning")
     ap.add_argument("--no-configs", action="store_true", help="skip the D2Q9 and channel configurations")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-ticks", type=int, default=2000)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
-    ap.add_argument("--block", default="", help="bx,by,bz[,tile order] of the step kernel instead of autotuning")
+    ap.add_argument("--block", default="", help="bx,by,bz[,tile order[,low occupancy]] of the step kernel instead of autotuning")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of this script into the traffic file and exit")
     args = ap.parse_args()
@@ -447,6 +449,8 @@ def main():
         m.set_block(*v[:3])
         if len(v) > 3:
             m.set_tile_order(v[3])
+        if len(v) > 4:
+            m.set_occupancy(v[4])
         tuned_ms = float("nan")
     else:
         block, tuned_ms = m.autotune(20)
@@ -473,7 +477,7 @@ def main():
         "data": "synthetic (uniform fluid at rest, squirmer position/orientation from numpy seed)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 tau=0.8 {args.n}^3 periodic, squirmer r={args.radius:g} "
                                f"B1=1e-3 beta={args.beta:g}, Ladd + momentum exchange, {args.storage} storage",
-                   "nodes": nodes, "block": list(m.get_block()), "tile_order": m.get_tile_order(), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
+                   "nodes": nodes, "block": list(m.get_block()), "tile_order": m.get_tile_order(), "low_occupancy": m.get_occupancy(), "autotuned_kernel_ms": None if tuned_ms != tuned_ms else round(tuned_ms, 5)},
         "event_timed_MLUPS": round(nodes / (sms * 1e-3) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -499,7 +503,8 @@ def main():
             k, _ = kernel_times(mv, min(ks, 500))
             var[st] = {"MLUPS": round(nodes * ks / w / 1e6, 1), "kernel_ms": round(k, 5),
                        "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "block": list(mv.get_block()), "tile_order": mv.get_tile_order()}
+                       "block": list(mv.get_block()), "tile_order": mv.get_tile_order(),
+                       "low_occupancy": mv.get_occupancy()}
             del mv
         var["speedup_f16c_over_f64"] = round(var["f16c"]["MLUPS"] / var["f64"]["MLUPS"], 2)
         var["speedup_f16_over_f64"] = round(var["f16"]["MLUPS"] / var["f64"]["MLUPS"], 2)

@@ -309,6 +309,16 @@ int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz);
 int lbm_set_tile_order(lbm_handle *h, int32_t xcd_remap);
 int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap);
 
+/* Occupancy variant of the gathering kernels (no reference counterpart; a
+ * tuning knob like the block shape): 0 (default) builds them for the most
+ * waves per SIMD the tiles use (FP32/FP16: 8, FP64: 4), capping registers so
+ * that the particles' box path spills; 1 for half the waves and twice the
+ * registers, which pays on lattices up to about 10^6 nodes with particles,
+ * where the box path's dependent chain sets the kernel time.  lbm_autotune
+ * times both when the lattice has particles. */
+int lbm_set_occupancy(lbm_handle *h, int32_t low);
+int lbm_get_occupancy(lbm_handle *h, int32_t *low);
+
 /* Autotuner: times the fused step kernel over candidate block shapes and
  * tile orders (reading the current populations, writing a scratch buffer,
  * so the state is untouched), keeps the fastest and returns it with its time
@@ -373,6 +383,8 @@ int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz);
 int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz);
 int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap);
 int lbm_batch_get_tile_order(lbm_batch *b, int32_t *xcd_remap);
+int lbm_batch_set_occupancy(lbm_batch *b, int32_t low);  /* as lbm_set_occupancy */
+int lbm_batch_get_occupancy(lbm_batch *b, int32_t *low);
 
 /* Autotuner of the batched fused kernel over the block shapes that fit every
  * box and both tile orders (the boxes' state is untouched). */

@@ -46,6 +46,7 @@ __host__ __device__ inline void collide_vjp(R rho, const R (&u)[3], const R (&F)
     for (int k = 0; k < D; k++) uF += u[k] * F[k];
   R prho = 0, pu[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
   R lin[Q];
+#pragma unroll
   for (int q = 0; q < Q; q++) {
     const int o = L::opp[q];
     // omega = cb (f+ - feq+) + cm (f- - feq-): kappa_q = cb lp+_q + cm lp-_q
@@ -75,6 +76,7 @@ __host__ __device__ inline void collide_vjp(R rho, const R (&u)[3], const R (&F)
   const R inv = rho != (R)0 ? (R)1 / rho : (R)0;
   R pus = 0;
   for (int k = 0; k < D; k++) pus += pu[k] * u[k];
+#pragma unroll
   for (int q = 0; q < Q; q++) {
     const R cq[3] = {(R)L::cx[q], (R)L::cy[q], (R)L::cz[q]};
     R pc = 0;
@@ -88,7 +90,7 @@ __host__ __device__ inline void collide_vjp(R rho, const R (&u)[3], const R (&F)
 // Vector-Jacobian product of node_velocity (GPU semantics, 3D) at the solid
 // node with coordinates xc: given ub = dL/du_b, adds dL/d(B1, B2, nR, nV,
 // nW, ne) of particle p to acc.
-__host__ __device__ inline void node_velocity_vjp(const double (&xc)[3], const DevParticle &p, const double (&ub)[3],
+__host__ __device__ __forceinline__ void node_velocity_vjp(const double (&xc)[3], const DevParticle &p, const double (&ub)[3],
                                                   double (&acc)[kAdjSums]) {
   double r[3];
   for (int k = 0; k < 3; k++) r[k] = xc[k] + (-p.nR[k]);
@@ -139,7 +141,28 @@ struct AdjArgs {
   double coeff, ls;     // momentum-exchange coefficient and lattice speed
   double shell;         // 1.5 dx: link nodes lie within r + shell of a centre
   const double *phitau;  // [particle][6]: dL/dF_p (0..2) and dL/dT_p (3..5) of the tick's move
+  // the near kernel's blocks: particle p's are [near_first[p], near_first[p + 1]),
+  // enough for near_ext[p]^3 nodes of its near box
+  int32_t near_first[LBM_MAX_PARTICLES + 1];
+  int32_t near_ext[LBM_MAX_PARTICLES];
 };
+
+// Near box of a particle: its link box grown by one node (clamped to the
+// lattice).  Every node that holds a link slot, or whose stream a link
+// replaces, lies inside; the near kernel takes these nodes, the bulk kernel
+// every other.
+__device__ __forceinline__ void near_box(const DevParticle &P, const BoxGeom &g, int lo[3], int hi[3]) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  for (int k = 0; k < 3; k++) {
+    lo[k] = P.lo[k] - 1 < 0 ? 0 : P.lo[k] - 1;
+    hi[k] = P.hi[k] + 1 > n[k] - 1 ? n[k] - 1 : P.hi[k] + 1;
+  }
+}
+__device__ __forceinline__ bool in_near_box(const DevParticle &P, const BoxGeom &g, int x, int y, int z) {
+  int lo[3], hi[3];
+  near_box(P, g, lo, hi);
+  return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
+}
 
 // The reverse of one tick's particle moves, on the device (block p, lane j <
 // kDualN carries the tangent of input j), and the update of the particles'
@@ -162,19 +185,85 @@ struct AdjFinishArgs {
   double gscale;               // 2 (mean - target) / window
 };
 
-// Squared distance of node (x, y, z) from Rg in getSphere's order, and the
-// link predicates of the forward box blocks: (x, q) is a link slot of p when
-// x lies in p's link box, r^2 <= d2(x) < (r + 1.5 dx)^2 and d2(x - c_q) < r^2.
-__device__ __forceinline__ double adj_d2(const AdjArgs &a, const DevParticle &P, int x, int y, int z) {
-#pragma clang fp contract(off)
-  const double u = a.coords[0][x] + (-P.Rg[0]), v = a.coords[1][y] + (-P.Rg[1]), w = a.coords[2][z] + (-P.Rg[2]);
-  const double su = u * u, sv = v * v, sw = w * w;
-  return (su + sv) + sw;
+// Element i of a small register array, and an update of it, without a
+// dynamic index: a private array indexed by a value the compiler cannot
+// fold is placed in scratch memory, and every access to it then goes through
+// the caches to HBM.
+// (The read masks the elements' bits: a chain of selects of the elements
+// is folded back into one load at a selected address.)
+template <int N, class T>
+__device__ __forceinline__ T reg_get(const T (&a)[N], int i) {
+  using U = std::conditional_t<sizeof(T) == 8, uint64_t, uint32_t>;
+  static_assert(sizeof(T) == sizeof(U), "4- or 8-byte elements");
+  U v = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) v |= __builtin_bit_cast(U, a[k]) & ((U)0 - (U)(i == k));
+  return __builtin_bit_cast(T, v);
+}
+template <int N, class T>
+__device__ __forceinline__ void reg_add(T (&a)[N], int i, T d) {
+#pragma unroll
+  for (int k = 0; k < N; k++) a[k] = i == k ? a[k] + d : a[k];
+}
+template <int N, class T>
+__device__ __forceinline__ void reg_set(T (&a)[N], int i, T v) {
+#pragma unroll
+  for (int k = 0; k < N; k++) a[k] = i == k ? v : a[k];
 }
 
-// One adjoint tick: dL/dS_k from dL/dS_{k+1}, with dL/dF += and the
-// per-particle sums of the block.  One thread per node (x fastest).
-template <int Q, int S, int COLL, int FM>
+// Velocity q's components, weight and opposite for a q that varies across
+// the lanes (a lane walks its own link slots), from bit masks of the table.
+template <int Q>
+struct DirMasks {
+  static constexpr uint32_t where(const int (&c)[Q], int v) {
+    uint32_t m = 0;
+    for (int q = 0; q < Q; q++)
+      if (c[q] == v) m |= 1u << q;
+    return m;
+  }
+  static constexpr uint32_t cls(int k) {
+    uint32_t m = 0;
+    for (int q = 0; q < Q; q++)
+      if (Lattice<Q>::wclass[q] == k) m |= 1u << q;
+    return m;
+  }
+  static constexpr uint32_t pos[3] = {where(Lattice<Q>::cx, 1), where(Lattice<Q>::cy, 1), where(Lattice<Q>::cz, 1)};
+  static constexpr uint32_t neg[3] = {where(Lattice<Q>::cx, -1), where(Lattice<Q>::cy, -1), where(Lattice<Q>::cz, -1)};
+  static constexpr uint32_t zero[3] = {where(Lattice<Q>::cx, 0), where(Lattice<Q>::cy, 0), where(Lattice<Q>::cz, 0)};
+  static constexpr uint32_t cls1 = cls(1), cls2 = cls(2);
+};
+template <int Q>
+__device__ __forceinline__ int dir_c(int k, int q) {
+  using M = DirMasks<Q>;
+  return (int)((M::pos[k] >> q) & 1u) - (int)((M::neg[k] >> q) & 1u);
+}
+template <int Q>
+__device__ __forceinline__ double dir_w(int q) {
+  using M = DirMasks<Q>;
+  using L = Lattice<Q>;
+  return ((M::cls1 >> q) & 1u) ? L::wc(1) : (((M::cls2 >> q) & 1u) ? L::wc(2) : L::wc(3));
+}
+// the index of -c_q: the one velocity whose components are those of c_q negated
+template <int Q>
+__device__ __forceinline__ int dir_opp(int q) {
+  using M = DirMasks<Q>;
+  uint32_t m = ~0u;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int c = dir_c<Q>(k, q);
+    m &= c > 0 ? M::neg[k] : (c < 0 ? M::pos[k] : M::zero[k]);
+  }
+  return __builtin_ctz(m);
+}
+
+// One adjoint tick: dL/dS_k from dL/dS_{k+1}, with dL/dF +=.  One thread
+// per node.  NEAR = false: the bulk, every node outside the particles' near
+// boxes (x fastest over the lattice; no particle term reaches these nodes).
+// NEAR = true: the nodes of the near boxes (block b of particle p covers
+// nodes [j 256, (j + 1) 256) of p's box, j = b - near_first[p]; a node in
+// the boxes of several particles belongs to the first), with the particle
+// terms and the per-particle sums of the block.
+template <int Q, int S, int COLL, int FM, bool NEAR>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
@@ -182,11 +271,39 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
   static_assert(Q == 19, "the adjoint is built for D3Q19");
   const BoxGeom &g = a.g;
   const int tid = threadIdx.x;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + tid;
-  const bool live = n < g.N;
-  const int x = live ? (int)(n % g.nx) : 0;
-  const int64_t row = live ? n / g.nx : 0;
-  const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+  int x = 0, y = 0, z = 0;
+  bool live;
+  if constexpr (!NEAR) {
+    const int64_t nn = (int64_t)blockIdx.x * blockDim.x + tid;
+    live = nn < g.N;
+    if (live) {
+      x = (int)(nn % g.nx);
+      const int64_t r = nn / g.nx;
+      y = (int)(r % g.ny);
+      z = (int)(r / g.ny);
+      for (int p = 0; p < a.np; p++)
+        if (in_near_box(a.parts[p], g, x, y, z)) live = false;
+    }
+    if (!live) return;  // (no block-wide step follows in the bulk kernel)
+  } else {
+    const int b = blockIdx.x;
+    int pb = 0;
+    while (pb + 1 < a.np && b >= a.near_first[pb + 1]) pb++;
+    int lo[3], hi[3];
+    near_box(a.parts[pb], g, lo, hi);
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    const int64_t i = (int64_t)(b - a.near_first[pb]) * blockDim.x + tid;
+    live = i < (int64_t)ex * ey * ez;
+    if (live) {
+      x = lo[0] + (int)(i % ex);
+      y = lo[1] + (int)((i / ex) % ey);
+      z = lo[2] + (int)(i / ((int64_t)ex * ey));
+      for (int p = 0; p < pb; p++)
+        if (in_near_box(a.parts[p], g, x, y, z)) live = false;
+    }
+  }
+  const int64_t row = (int64_t)y + (int64_t)g.ny * z;
+  const int64_t n = row * g.nx + x;
   const T *Sk = static_cast<const T *>(a.S);
   const R *lin = static_cast<const R *>(a.lam_in);
   R *lout = static_cast<R *>(a.lam_out);
@@ -218,31 +335,53 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       collide_dev<Q, COLL, FM>(f, Fn, a.c, P, rho);
   }
 
-  // link geometry near the particles: own[p] = link slots (x, q) of p,
+  // the coordinates of the node and of its neighbours (x - 1 .. x + 1 along
+  // each axis, periodic), loaded together (near kernel)
+  double cxs[3] = {0.0, 0.0, 0.0}, cys[3] = {0.0, 0.0, 0.0}, czs[3] = {0.0, 0.0, 0.0};
+  if (NEAR && live) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      cxs[k] = a.coords[0][wrapi(x + k - 1, g.nx)];
+      cys[k] = a.coords[1][wrapi(y + k - 1, g.ny)];
+      czs[k] = a.coords[2][wrapi(z + k - 1, g.nz)];
+    }
+  }
+  // (X + (-R))_1^2 + ... at the neighbour x + (ox, oy, oz), getSphere's order
+  auto d2_at = [&](const DevParticle &Pp, int ox, int oy, int oz) {
+#pragma clang fp contract(off)
+    const double u = cxs[ox + 1] + (-Pp.Rg[0]), v = cys[oy + 1] + (-Pp.Rg[1]), w = czs[oz + 1] + (-Pp.Rg[2]);
+    const double su = u * u, sv = v * v, sw = w * w;
+    return (su + sv) + sw;
+  };
+
+  // link geometry near the particles, the forward box blocks' predicates:
+  // (x, q) is a link slot of p when x lies in p's link box, r^2 <= d2(x) <
+  // (r + 1.5 dx)^2 and d2(x - c_q) < r^2.  own[p] = link slots (x, q) of p,
   // drop = directions q whose stream x -> x + c_q a link of some particle replaces
   uint32_t own[LBM_MAX_PARTICLES];
   uint32_t own_any = 0, drop = 0;
   for (int p = 0; p < LBM_MAX_PARTICLES; p++) own[p] = 0;
-  if (live && !wall) {
+  if (NEAR && live && !wall) {
     for (int p = 0; p < a.np; p++) {
       const DevParticle &Pp = a.parts[p];
       if (x < Pp.lo[0] - 1 || x > Pp.hi[0] + 1 || y < Pp.lo[1] - 1 || y > Pp.hi[1] + 1 || z < Pp.lo[2] - 1 ||
           z > Pp.hi[2] + 1)
         continue;
       const double r2 = Pp.radius * Pp.radius, ro = Pp.radius + a.shell;
-      const double d0 = adj_d2(a, Pp, x, y, z);
+      const double d0 = d2_at(Pp, 0, 0, 0);
       const bool fl0 = in_box(Pp, x, y, z) && d0 >= r2 && d0 < ro * ro;  // may hold link slots
       uint32_t op = 0;
+#pragma unroll
       for (int q = 1; q < Q; q++) {
         int xn, yn, zn;
         nb(q, xn, yn, zn);
-        const double dn = adj_d2(a, Pp, xn, yn, zn);
+        const double dn = d2_at(Pp, L::cx[q], L::cy[q], L::cz[q]);
         // (x, q̄) is a link slot: x - c_q̄ = x + c_q solid
         if (fl0 && dn < r2) op |= 1u << L::opp[q];
         // (x + c_q, q) is a link slot: x + c_q fluid within the shell, x solid
         if (d0 < r2 && in_box(Pp, xn, yn, zn) && dn >= r2 && dn < ro * ro) drop |= 1u << q;
       }
-      own[p] = op;
+      reg_set(own, p, op);
       own_any |= op;
     }
   }
@@ -250,8 +389,15 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
   // dL/dP_k at this node
   R lp[Q];
   R grho = 0;
-  double coords_x[3] = {0.0, 0.0, 0.0};
-  if (live) coords_x[0] = a.coords[0][x], coords_x[1] = a.coords[1][y], coords_x[2] = a.coords[2][z];
+  const double coords_x[3] = {cxs[1], cys[1], czs[1]};
+  // dL/dS_{k+1} of this node's own populations, which its link slots read
+  R lown[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) lown[q] = 0;
+  if (NEAR && own_any)
+#pragma unroll
+    for (int q = 1; q < Q; q++)
+      if ((own_any >> q) & 1) lown[q] = lin[own_idx + (int64_t)q * g.nx];
   if (live && !wall) {
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -266,50 +412,45 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       if (wn) v += lin[own_idx + (int64_t)L::opp[q] * g.nx];
       lp[q] = v;
     }
-    // particle link slots (x, q'): S_q'(x) = P_q̄'(x) + the owners' Ladd terms
-    for (int qq = 1; qq < Q; qq++) {
-      if (!((own_any >> qq) & 1)) continue;
-      const int ob = L::opp[qq];
-      const double lam = (double)lin[own_idx + (int64_t)qq * g.nx];
-      const double cq[3] = {(double)L::cx[qq], (double)L::cy[qq], (double)L::cz[qq]};
-      const double xs[3] = {a.coords[0][wrapi(x - L::cx[qq], g.nx)], a.coords[1][wrapi(y - L::cy[qq], g.ny)],
-                            a.coords[2][wrapi(z - L::cz[qq], g.nz)]};
-      const double lc = 2.0 * L::w(qq) * a.c.invC2s;
-      const double fs = (double)P[ob];
-      // owners in particle order: running value v_j, s_j = fs + v_j, shat_j
-      double v = fs, shat_sum = 0.0;
+    // particle link slots (x, q'): S_q'(x) = P_q̄'(x) + the owners' Ladd
+    // terms; each lane walks its own slots in increasing q'
+    uint32_t slots = NEAR ? own_any : 0u;
+    while (slots) {
+      const int qq = __builtin_ctz(slots);
+      slots &= slots - 1;
+      const int ob = dir_opp<Q>(qq);
+      const int c0 = dir_c<Q>(0, qq), c1 = dir_c<Q>(1, qq), c2 = dir_c<Q>(2, qq);
+      const double lam = (double)reg_get(lown, qq);
+      const double cq[3] = {(double)c0, (double)c1, (double)c2};
+      const double xs[3] = {reg_get(cxs, 1 - c0), reg_get(cys, 1 - c1), reg_get(czs, 1 - c2)};
+      const double lc = 2.0 * dir_w<Q>(qq) * a.c.invC2s;
+      // owners in particle order: u_b . c_q ls and shat_j of each
+      double udc_o[LBM_MAX_PARTICLES], sh_o[LBM_MAX_PARTICLES];
+      double shat_sum = 0.0;
       for (int p = 0; p < a.np; p++) {
-        if (!((own[p] >> qq) & 1)) continue;
+        if (!((reg_get(own, p) >> qq) & 1)) continue;
         const DevParticle &Pp = a.parts[p];
         double ubv[3];
         node_velocity<3>(xs, Pp, LBM_SEMANTICS_GPU, 0.0, ubv);
         const double udc = (ubv[0] * cq[0] + ubv[1] * cq[1] + ubv[2] * cq[2]) * a.ls;
-        v = v + lc * (double)rho * udc;
         const double arm[3] = {coords_x[0] - Pp.R[0], coords_x[1] - Pp.R[1], coords_x[2] - Pp.R[2]};
         const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
                               arm[0] * cq[1] - arm[1] * cq[0]};
+        const double *pt = a.phitau + p * 6;
         const double sh = -a.coeff * a.ls *
-                          (a.phitau[p * 6 + 0] * cq[0] + a.phitau[p * 6 + 1] * cq[1] + a.phitau[p * 6 + 2] * cq[2] + a.phitau[p * 6 + 3] * cr[0] +
-                           a.phitau[p * 6 + 4] * cr[1] + a.phitau[p * 6 + 5] * cr[2]);
+                          (pt[0] * cq[0] + pt[1] * cq[1] + pt[2] * cq[2] + pt[3] * cr[0] + pt[4] * cr[1] + pt[5] * cr[2]);
+        reg_set(udc_o, p, udc);
+        reg_set(sh_o, p, sh);
         shat_sum += sh;
       }
       // dL/dfs = lam + 2 sum shat; dL/dv_j = lam + sum_{i >= j} shat_i
-      lp[ob] += (R)(lam + 2.0 * shat_sum);
+      reg_add(lp, ob, (R)(lam + 2.0 * shat_sum));
       double pre = 0.0;
       for (int p = 0; p < a.np; p++) {
-        if (!((own[p] >> qq) & 1)) continue;
-        const DevParticle &Pp = a.parts[p];
-        double ubv[3];
-        node_velocity<3>(xs, Pp, LBM_SEMANTICS_GPU, 0.0, ubv);
-        const double udc = (ubv[0] * cq[0] + ubv[1] * cq[1] + ubv[2] * cq[2]) * a.ls;
+        if (!((reg_get(own, p) >> qq) & 1)) continue;
         const double aj = lam + shat_sum - pre;
-        grho += (R)(aj * lc * udc);
-        const double arm[3] = {coords_x[0] - Pp.R[0], coords_x[1] - Pp.R[1], coords_x[2] - Pp.R[2]};
-        const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
-                              arm[0] * cq[1] - arm[1] * cq[0]};
-        pre += -a.coeff * a.ls *
-               (a.phitau[p * 6 + 0] * cq[0] + a.phitau[p * 6 + 1] * cq[1] + a.phitau[p * 6 + 2] * cq[2] + a.phitau[p * 6 + 3] * cr[0] +
-                a.phitau[p * 6 + 4] * cr[1] + a.phitau[p * 6 + 5] * cr[2]);
+        grho += (R)(aj * lc * reg_get(udc_o, p));
+        pre += reg_get(sh_o, p);
       }
     }
   }
@@ -343,6 +484,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
     }
   }
 
+  if constexpr (!NEAR) return;
+
   // per-particle sums over the block (only blocks that hold link slots)
   __shared__ int any_link;
   __shared__ DD red[4][kAdjSums];
@@ -353,23 +496,27 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
   const int wave = tid >> 6, lane = __lane_id();
   for (int p = 0; p < a.np; p++) {
     double acc[kAdjSums];
+#pragma unroll
     for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
-    if (any_link && (own[p] != 0)) {
+    const uint32_t own_p = reg_get(own, p);
+    if (any_link && own_p != 0) {
       const DevParticle &Pp = a.parts[p];
-      for (int qq = 1; qq < Q; qq++) {
-        if (!((own[p] >> qq) & 1)) continue;
-        const int ob = L::opp[qq];
-        const double lam = (double)lin[own_idx + (int64_t)qq * g.nx];
-        const double cq[3] = {(double)L::cx[qq], (double)L::cy[qq], (double)L::cz[qq]};
-        const double xs[3] = {a.coords[0][wrapi(x - L::cx[qq], g.nx)], a.coords[1][wrapi(y - L::cy[qq], g.ny)],
-                              a.coords[2][wrapi(z - L::cz[qq], g.nz)]};
-        const double lc = 2.0 * L::w(qq) * a.c.invC2s;
-        const double shift = S == kF64 ? 0.0 : 2.0 * L::w(qq) * a.c.rho0;
-        const double fs = (double)P[ob];
+      uint32_t slots = own_p;
+      while (slots) {
+        const int qq = __builtin_ctz(slots);
+        slots &= slots - 1;
+        const int ob = dir_opp<Q>(qq);
+        const int c0 = dir_c<Q>(0, qq), c1 = dir_c<Q>(1, qq), c2 = dir_c<Q>(2, qq);
+        const double lam = (double)reg_get(lown, qq);
+        const double cq[3] = {(double)c0, (double)c1, (double)c2};
+        const double xs[3] = {reg_get(cxs, 1 - c0), reg_get(cys, 1 - c1), reg_get(czs, 1 - c2)};
+        const double lc = 2.0 * dir_w<Q>(qq) * a.c.invC2s;
+        const double shift = S == kF64 ? 0.0 : 2.0 * dir_w<Q>(qq) * a.c.rho0;
+        const double fs = (double)reg_get(P, ob);
         // the owner chain up to and after p
         double v = fs, shat_sum = 0.0, pre = 0.0, s_p = 0.0;
         for (int i = 0; i < a.np; i++) {
-          if (!((own[i] >> qq) & 1)) continue;
+          if (!((reg_get(own, i) >> qq) & 1)) continue;
           const DevParticle &Pi = a.parts[i];
           double ubv[3];
           node_velocity<3>(xs, Pi, LBM_SEMANTICS_GPU, 0.0, ubv);
@@ -402,6 +549,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       if (tid < kAdjSums) dst[tid] = 0.0;
       continue;
     }
+#pragma unroll
     for (int k = 0; k < kAdjSums; k++) {
       const DD s = wave_sum(DD{acc[k], 0.0});
       if (lane == kWaveSumLane) red[wave][k] = s;

@@ -150,6 +150,8 @@ Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
   t.mtx = div_magic((uint32_t)t.tx, (uint64_t)t.ntiles);
   t.mty = div_magic((uint32_t)t.ty, (uint64_t)t.ntiles / (uint64_t)t.tx + 1);
   t.remap = 1;
+  t.occ = 0;
+  t.reserved = 0;
   return t;
 }
 
@@ -1349,6 +1351,7 @@ int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz) {
       return fail(LBM_EINVAL, "block shape must have 64..512 threads, a multiple of 64");
     t = make_tiling(h, bx, by, bz);
   }
+  t.occ = h->tiling.occ;  // (the occupancy variant stays)
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   int rc = plan_boxes(h, tile_threads(t));
   if (rc) return rc;
@@ -1406,9 +1409,11 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   std::sort(bxs.begin(), bxs.end());
   bxs.erase(std::unique(bxs.begin(), bxs.end()), bxs.end());
   struct Cand {
-    int32_t bx, by, bz, remap;
+    int32_t bx, by, bz, remap, occ;
   };
   std::vector<Cand> cands;
+  // the low-occupancy kernels only change the box path: timed with particles
+  const int nocc = h->parts_h.empty() ? 1 : 2;
   for (int32_t bx : bxs)
     for (int32_t by : {1, 2, 4, 8, 16})
       for (int32_t bz : {1, 2, 4}) {
@@ -1417,8 +1422,10 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
         const Tiling tl = make_tiling(h, bx, by, bz);
         const double waste = (double)tl.ntiles * t / (double)h->N - 1.0;
         if (waste > 0.25) continue;
-        cands.push_back({bx, by, bz, 1});
-        cands.push_back({bx, by, bz, 0});
+        for (int occ = 0; occ < nocc; occ++) {
+          cands.push_back({bx, by, bz, 1, occ});
+          cands.push_back({bx, by, bz, 0, occ});
+        }
       }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits this lattice");
   // The trial kernels step a copy of the populations back and forth between
@@ -1445,6 +1452,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     // parity, which the next tick rewrites
     Tiling t = make_tiling(h, c.bx, c.by, c.bz);
     t.remap = c.remap;
+    t.occ = c.occ;
     const int rc = plan_boxes(h, tile_threads(t));
     if (rc) {
       h->tiling = saved;
@@ -1486,6 +1494,7 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   }
   h->tiling = make_tiling(h, best_c.bx, best_c.by, best_c.bz);
   h->tiling.remap = best_c.remap;
+  h->tiling.occ = best_c.occ;
   if (const int rc = plan_boxes(h, tile_threads(h->tiling))) return rc;
   drop_graph(h);
   if (best) {
@@ -1550,6 +1559,20 @@ int lbm_set_tile_order(lbm_handle *h, int32_t xcd_remap) {
 int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap) {
   if (!h || !xcd_remap) return fail(LBM_EINVAL, "null argument");
   *xcd_remap = h->tiling.remap;
+  return LBM_OK;
+}
+
+int lbm_set_occupancy(lbm_handle *h, int32_t low) {
+  if (!h) return fail(LBM_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  h->tiling.occ = low != 0;
+  drop_graph(h);
+  return LBM_OK;
+}
+
+int lbm_get_occupancy(lbm_handle *h, int32_t *low) {
+  if (!h || !low) return fail(LBM_EINVAL, "null argument");
+  *low = h->tiling.occ;
   return LBM_OK;
 }
 
@@ -1689,10 +1712,10 @@ hipError_t launch_batch_mode(lbm_batch *b, int mode, const StepArgs *host, const
   BatchArgs ba;
   Launch l = batch_launch(b, host, dev, mode, ba);
   switch (b->storage) {
-    case LBM_STORAGE_F64: return launch_batch_step<kF64>(b->Q, b->coll, b->force, mode, ba, l);
-    case LBM_STORAGE_F32: return launch_batch_step<kF32>(b->Q, b->coll, b->force, mode, ba, l);
-    case LBM_STORAGE_F16C: return launch_batch_step<kF16C>(b->Q, b->coll, b->force, mode, ba, l);
-    default: return launch_batch_step<kF16>(b->Q, b->coll, b->force, mode, ba, l);
+    case LBM_STORAGE_F64: return launch_batch_step<kF64>(b->Q, b->coll, b->force, mode, b->shape.occ, ba, l);
+    case LBM_STORAGE_F32: return launch_batch_step<kF32>(b->Q, b->coll, b->force, mode, b->shape.occ, ba, l);
+    case LBM_STORAGE_F16C: return launch_batch_step<kF16C>(b->Q, b->coll, b->force, mode, b->shape.occ, ba, l);
+    default: return launch_batch_step<kF16>(b->Q, b->coll, b->force, mode, b->shape.occ, ba, l);
   }
 }
 
@@ -1885,6 +1908,7 @@ int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz) {
   t.by = by;
   t.bz = bz;
   t.remap = 1;
+  t.occ = b->shape.occ;  // (the occupancy variant stays)
   b->shape = t;
   drop_batch_graphs(b);
   for (lbm_handle *h : b->boxes)
@@ -1905,6 +1929,20 @@ int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap) {
 int lbm_batch_get_tile_order(lbm_batch *b, int32_t *xcd_remap) {
   if (!b || !xcd_remap) return fail(LBM_EINVAL, "null argument");
   *xcd_remap = b->shape.remap;
+  return LBM_OK;
+}
+
+int lbm_batch_set_occupancy(lbm_batch *b, int32_t low) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  b->shape.occ = low != 0;
+  drop_batch_graphs(b);
+  return LBM_OK;
+}
+
+int lbm_batch_get_occupancy(lbm_batch *b, int32_t *low) {
+  if (!b || !low) return fail(LBM_EINVAL, "null argument");
+  *low = b->shape.occ;
   return LBM_OK;
 }
 
@@ -1986,14 +2024,16 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   std::vector<Tiling> cands;
   for (const Shape &sh : shapes) {
     if (sh.waste > std::max(0.25, least + 0.15)) continue;
-    for (int remap : {1, 0}) {
-      Tiling c{};
-      c.bx = sh.bx;
-      c.by = sh.by;
-      c.bz = sh.bz;
-      c.remap = remap;
-      cands.push_back(c);
-    }
+    for (int occ = 0; occ < 2; occ++)
+      for (int remap : {1, 0}) {
+        Tiling c{};
+        c.bx = sh.bx;
+        c.by = sh.by;
+        c.bz = sh.bz;
+        c.remap = remap;
+        c.occ = occ;
+        cands.push_back(c);
+      }
   }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits the boxes");
   char *scratch = nullptr;
@@ -2128,7 +2168,18 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   const size_t pop_bytes = N * h->Q * elem_size(h->cfg.storage);
   const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
   const int threads = 256;
-  const int64_t nblocks = ((int64_t)N + threads - 1) / threads;
+  const int64_t nblocks = ((int64_t)N + threads - 1) / threads;  // the bulk kernel's
+  // the near kernel's blocks: particle p's near box (link box and one node)
+  // spans at most 2 r / dx + 9 nodes an axis
+  int32_t near_first[LBM_MAX_PARTICLES + 1] = {0};
+  int32_t near_ext[LBM_MAX_PARTICLES] = {0};
+  for (int p = 0; p < np; p++) {
+    const int64_t e = (int64_t)std::ceil(2.0 * h->parts_h[p].radius / h->cfg.dx) + 10;
+    near_ext[p] = (int32_t)e;
+    near_first[p + 1] = near_first[p] + (int32_t)((e * e * e + threads - 1) / threads);
+  }
+  for (int p = np; p < LBM_MAX_PARTICLES; p++) near_first[p + 1] = near_first[p];
+  const int64_t nblocks_near = near_first[np];
   const int64_t nck = (T + K - 1) / K, seg_len = std::min(K, T);
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
 
@@ -2142,7 +2193,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       buf.alloc(&seg, 2 * (size_t)seg_len * pop_bytes) != hipSuccess ||
       buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
       buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
-      buf.alloc(&partial, (size_t)nblocks * np * kAdjSums * sizeof(double)) != hipSuccess ||
+      buf.alloc(&partial, (size_t)std::max<int64_t>(nblocks_near, 1) * np * kAdjSums * sizeof(double)) != hipSuccess ||
       buf.alloc(&sums_d, (size_t)np * kAdjSums * sizeof(double)) != hipSuccess ||
       buf.alloc(&recs_seg, 2 * (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess ||
       buf.alloc(&ck_recs_d, (size_t)nck * np * sizeof(DevParticle)) != hipSuccess)
@@ -2282,6 +2333,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.ls = h->cfg.dx / h->cfg.dt;
   aa.shell = 1.5 * h->cfg.dx;
   aa.phitau = pt_d;
+  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) aa.near_first[p] = near_first[p];
+  for (int p = 0; p < LBM_MAX_PARTICLES; p++) aa.near_ext[p] = near_ext[p];
   AdjMoveArgs ma{};
   ma.mu = mu_d;
   ma.mu_in = muin_d;
@@ -2300,30 +2353,37 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   // the two share only the read-only checkpoints.  The recomputation uses
   // the lattice's own working state (populations' spare buffer, records,
   // partials, rho), which the reverse sweep does not touch.
-  hipStream_t s_rev = h->stream, s_rec = nullptr;
+  // Within a reverse tick the near kernel (a few latency-bound blocks) runs
+  // on a third stream beside the bulk kernel: both start after the move's
+  // reverse and write disjoint nodes; the reduction waits for both.
+  hipStream_t s_rev = h->stream, s_rec = nullptr, s_near = nullptr;
   hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+  constexpr int kEvents = 6;
   struct Streams {
     lbm_handle *h;
-    hipStream_t main, *extra;
+    hipStream_t main, *extra[2];
     hipEvent_t *evs;
     ~Streams() {
       h->stream = main;
-      if (*extra) {
-        hipStreamSynchronize(*extra);
-        hipStreamDestroy(*extra);
-      }
-      for (int i = 0; i < 4; i++)
+      for (hipStream_t *s : extra)
+        if (*s) {
+          hipStreamSynchronize(*s);
+          hipStreamDestroy(*s);
+        }
+      for (int i = 0; i < kEvents; i++)
         if (evs[i]) hipEventDestroy(evs[i]);
     }
   };
-  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
-  Streams guard{h, s_rev, &s_rec, evs};
+  hipEvent_t evs[kEvents] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  Streams guard{h, s_rev, {&s_rec, &s_near}, evs};
   HIP_TRY(hipStreamCreate(&s_rec), "create recomputation stream");
-  for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming), "create event");
+  HIP_TRY(hipStreamCreate(&s_near), "create near-kernel stream");
+  for (int i = 0; i < kEvents; i++) HIP_TRY(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming), "create event");
   ev_rec[0] = evs[0];
   ev_rec[1] = evs[1];
   ev_rev[0] = evs[2];
   ev_rev[1] = evs[3];
+  hipEvent_t ev_moved = evs[4], ev_near = evs[5];
   HIP_TRY(hipEventRecord(ev_rev[0], s_rev), "event");  // (both sets free)
   HIP_TRY(hipEventRecord(ev_rev[1], s_rev), "event");
   // S_k of segment c's ticks: the checkpoint itself, then slots 1.. of its
@@ -2363,6 +2423,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     return LBM_OK;
   };
   const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
+  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_near};
   if (const int rc = recompute(nck - 1)) return rc;
   for (int64_t c = nck - 1; c >= 0; c--) {
     const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
@@ -2379,8 +2440,13 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       aa.S = Sk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
-      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
-      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks, np, sums_d, s_rev);
+      if (e == hipSuccess) e = hipEventRecord(ev_moved, s_rev);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s_near, ev_moved, 0);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, true, aa, ln);
+      if (e == hipSuccess) e = hipEventRecord(ev_near, s_near);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s_rev, ev_near, 0);
+      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks_near, np, sums_d, s_rev);
       fa2.recs_k = ma.recs_k;
       fa2.direct = k > ws;  // V_k is one of the averaged speeds
       if (e == hipSuccess) e = launch_adjoint_finish(fa2, s_rev);

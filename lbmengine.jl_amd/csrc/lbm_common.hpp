@@ -168,6 +168,9 @@ struct Tiling {
   int32_t ntiles;
   int32_t remap;        // 1: xcd_remap the tile order
   uint32_t mtx, mty;    // tile_div multipliers for tx and ty (0: divide)
+  int32_t occ;          // 1: the gathering kernels built for half the waves per
+                        // SIMD and twice the registers (no spills in the box path)
+  int32_t reserved;
 };
 
 // Multiplier m with n / d == umulhi(n, m) for every n < nmax, or 0 when

@@ -18,7 +18,7 @@ hipError_t launch_step(int Q, int coll, int fm, int mode, const StepArgs &a, con
 template <int S>
 hipError_t launch_step_q15_q27(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
-hipError_t launch_batch_step(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l);
+hipError_t launch_batch_step(int Q, int coll, int fm, int mode, int occ, const BatchArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
@@ -38,7 +38,9 @@ hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t strea
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
 struct AdjArgs;
-hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l);
+// near = false: the bulk kernel (nodes outside the particles' near boxes);
+// true: the near kernel (their nodes, with the particle terms and sums)
+hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArgs &a, const Launch &l);
 hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream);
 struct AdjMoveArgs;
 struct AdjFinishArgs;

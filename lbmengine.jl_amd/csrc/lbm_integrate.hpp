@@ -127,10 +127,16 @@ __host__ __device__ inline void link_box(const double Rg[3], double r, const dou
 // Largest link-box extent along one dimension for radius r.
 inline int32_t link_box_extent(double r, double dx) { return (int32_t)ceil(2.0 * r / dx) + 7; }
 
+// (Loops over the dims components run a fixed 3 times with a guard: a small
+// array indexed in a loop of runtime trip count is placed in scratch memory,
+// and the single thread that moves a particle then waits on a memory round
+// trip at every access.)
 __host__ __device__ inline bool any_nonzero(const double *v, int n) {
-  for (int k = 0; k < n; k++)
-    if (v[k] != 0.0) return true;
-  return false;
+  bool nz = false;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k < n && v[k] != 0.0) nz = true;
+  return nz;
 }
 
 // moveParticles!, gpu/particles.jl:7-92.  The half-kick branch
@@ -142,17 +148,22 @@ __host__ __device__ inline void move_gpu(DevParticle &p, const double F[3], cons
   moved = false;
   find_nv = false;
   if (p.couple_forces) {
-    double A[3];
-    for (int k = 0; k < dims; k++) A[k] = F[k] * p.inv_mass;
-    for (int k = 0; k < dims; k++) p.V[k] = p.V[k] + dt * A[k];
-    for (int k = 0; k < dims; k++) p.R[k] = p.R[k] + dt * p.V[k];
+    double A[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) if (k < dims) A[k] = F[k] * p.inv_mass;
+#pragma unroll
+    for (int k = 0; k < 3; k++) if (k < dims) p.V[k] = p.V[k] + dt * A[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) if (k < dims) p.R[k] = p.R[k] + dt * p.V[k];
     moved = any_nonzero(p.V, dims);
   }
   if (p.couple_torques) {
     const int nw = dims == 2 ? 1 : 3;
     double Al[3];
-    for (int k = 0; k < nw; k++) Al[k] = T[k] * p.inv_I;
-    for (int k = 0; k < nw; k++) p.W[k] = p.W[k] + dt * Al[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) Al[k] = k < nw ? T[k] * p.inv_I : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) if (k < nw) p.W[k] = p.W[k] + dt * Al[k];
     if (p.kind == LBM_PARTICLE_SQUIRMER) {
       // Rodrigues' rotation of the swimming direction by |omega| dt
       double *e = p.e;
@@ -162,6 +173,7 @@ __host__ __device__ inline void move_gpu(DevParticle &p, const double F[3], cons
         theta = fabs(W[0]);
       } else {
         double s = 0.0;
+#pragma unroll
         for (int k = 0; k < 3; k++) s = s + W[k] * W[k];
         theta = sqrt(s);
       }
@@ -184,13 +196,18 @@ __host__ __device__ inline void move_gpu(DevParticle &p, const double F[3], cons
           const double cr[3] = {kk[1] * e[2] - kk[2] * e[1], -kk[0] * e[2] + kk[2] * e[0],
                                 kk[0] * e[1] - kk[1] * e[0]};
           double vr[3];
+#pragma unroll
           for (int k = 0; k < 3; k++) vr[k] = e[k] * c + cr[k] * s;
           double kd = 0.0;
+#pragma unroll
           for (int k = 0; k < 3; k++) kd = kd + kk[k] * e[k];
+#pragma unroll
           for (int k = 0; k < 3; k++) vr[k] = vr[k] + kk[k] * kd * (1.0 - c);
           double nv = 0.0;
+#pragma unroll
           for (int k = 0; k < 3; k++) nv = nv + vr[k] * vr[k];
           nv = sqrt(nv);
+#pragma unroll
           for (int k = 0; k < 3; k++) e[k] = vr[k] / nv;
         }
       }
@@ -206,11 +223,14 @@ __device__ inline void move_cpu(DevParticle &p, const double dP[3], const double
                                 double dt, bool &moved, bool &find_nv) {
 #pragma clang fp contract(off)
   const int nw = dims == 2 ? 1 : 3;
-  for (int k = 0; k < dims; k++) p.V[k] = p.V[k] + p.inv_mass * dP[k];
-  for (int k = 0; k < nw; k++) p.W[k] = p.W[k] + p.inv_I * dL[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++) if (k < dims) p.V[k] = p.V[k] + p.inv_mass * dP[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++) if (k < nw) p.W[k] = p.W[k] + p.inv_I * dL[k];
   moved = false;
   if (any_nonzero(p.V, dims)) {
-    for (int k = 0; k < dims; k++) p.R[k] = p.R[k] + dt * p.V[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) if (k < dims) p.R[k] = p.R[k] + dt * p.V[k];
     moved = true;
   }
   find_nv = any_nonzero(dL, nw) || moved;
@@ -247,7 +267,9 @@ __host__ __device__ inline double julia_pow(double x, int n) {
 __host__ __device__ inline double norm_seq(const double *v, int n) {
 #pragma clang fp contract(off)
   double s = v[0] * v[0];
-  for (int k = 1; k < n; k++) s = s + v[k] * v[k];
+#pragma unroll
+  for (int k = 1; k < 3; k++)
+    if (k < n) s = s + v[k] * v[k];
   return sqrt(s);
 }
 
@@ -283,17 +305,23 @@ __host__ __device__ inline bool interaction_term(const lbm_interaction &I, const
   if (I.kind == LBM_INTERACTION_POLAR) {
     // gpu/particles.jl:136-158
     double d12[3] = {0, 0, 0}, d32[3] = {0, 0, 0}, u12[3] = {0, 0, 0}, u32[3] = {0, 0, 0};
-    for (int k = 0; k < d; k++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (k >= d) continue;
       d12[k] = R1[k] - R2[k];
       d32[k] = R3[k] - R2[k];
     }
     const double n12 = norm_seq(d12, d), n32 = norm_seq(d32, d);
-    for (int k = 0; k < d; k++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (k >= d) continue;
       u12[k] = d12[k] / n12;
       u32[k] = d32[k] / n32;
     }
     double c = u12[0] * u32[0];
-    for (int k = 1; k < d; k++) c = c + u12[k] * u32[k];
+#pragma unroll
+    for (int k = 1; k < 3; k++)
+      if (k < d) c = c + u12[k] * u32[k];
     if (fabs(c) > 1.0) c = c > 0.0 ? 1.0 : -1.0;
     const double angle = acos(c);
     const double s = I.params[1] * (angle - I.params[0]);
@@ -326,10 +354,14 @@ __host__ __device__ inline bool interaction_term(const lbm_interaction &I, const
     return true;
   }
   double d21[3] = {0, 0, 0}, u[3] = {0, 0, 0};
-  for (int k = 0; k < d; k++) d21[k] = R2[k] - R1[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k < d) d21[k] = R2[k] - R1[k];
   const double disp = norm_seq(d21, d);
   if (I.kind == LBM_INTERACTION_LENNARD_JONES && !(disp < I.params[2])) return false;
-  for (int k = 0; k < d; k++) u[k] = d21[k] / disp;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k < d) u[k] = d21[k] / disp;
   double s;
   switch (I.kind) {
     case LBM_INTERACTION_LINEAR:  // gpu/particles.jl:110-121
@@ -369,11 +401,15 @@ __host__ __device__ inline bool interaction_term(const lbm_interaction &I, const
         BB = B[0] * B[0] + B[1] * B[1];
         s2 = cb * cb;
       }
-      for (int k = 0; k < d; k++) fa[k] = pre * (2.0 * ccb[k] - 2.0 * BB * u[k] + 5.0 * u[k] * s2);
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (k < d) fa[k] = pre * (2.0 * ccb[k] - 2.0 * BB * u[k] + 5.0 * u[k] * s2);
       return true;
     }
   }
-  for (int k = 0; k < d; k++) fa[k] = s * u[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k < d) fa[k] = s * u[k];
   return true;
 }
 
@@ -400,11 +436,17 @@ __device__ inline void add_interaction_forces(const StepParticles &sp, const Dev
     }
     if (!interaction_term(I, recs[i1].R, recs[i2].R, polar ? recs[i3].R : nullptr, d, fa, fb, Bt)) continue;
     if (p == i1)
-      for (int k = 0; k < d; k++) F[k] = F[k] + fa[k];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (k < d) F[k] = F[k] + fa[k];
     if (polar && p == i3)
-      for (int k = 0; k < d; k++) F[k] = F[k] + fb[k];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (k < d) F[k] = F[k] + fb[k];
     if (p == i2)
-      for (int k = 0; k < d; k++) F[k] = F[k] - (polar ? fa[k] + fb[k] : fa[k]);
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (k < d) F[k] = F[k] - (polar ? fa[k] + fb[k] : fa[k]);
   }
 }
 

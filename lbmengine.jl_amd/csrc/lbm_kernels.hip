@@ -36,11 +36,15 @@ constexpr int S = LBM_STORE;
     default: return hipErrorInvalidValue;                 \
   }
 
+// (the collide-only mode has no box path: one occupancy variant)
 #define LBM_MODE_SWITCH(KERNEL, ARGS)                                                                        \
-  switch (mode) {                                                                                            \
-    case kCollide: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kCollide>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
-    case kFused: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kFused>), l.grid, l.block, l.shmem, l.stream, ARGS); break;     \
-    case kGather: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kGather>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
+  switch (mode * 2 + (occ ? 1 : 0)) {                                                                        \
+    case 2 * kCollide:                                                                                       \
+    case 2 * kCollide + 1: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kCollide, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break; \
+    case 2 * kFused: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kFused, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;         \
+    case 2 * kFused + 1: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kFused, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;     \
+    case 2 * kGather: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kGather, 0>), l.grid, l.block, l.shmem, l.stream, ARGS); break;       \
+    case 2 * kGather + 1: hipLaunchKernelGGL((KERNEL<Q, S, COLL, FM, kGather, 1>), l.grid, l.block, l.shmem, l.stream, ARGS); break;   \
     default: return hipErrorInvalidValue;                                                                    \
   }                                                                                                          \
   return hipGetLastError();
@@ -49,19 +53,20 @@ constexpr int S = LBM_STORE;
 
 // (internal linkage: each storage's object file has its own)
 template <int Q, int COLL, int FM>
-static hipError_t launch_batch_qcf(int mode, const BatchArgs &a, const Launch &l) {
+static hipError_t launch_batch_qcf(int mode, int occ, const BatchArgs &a, const Launch &l) {
   LBM_MODE_SWITCH(batch_step_kernel, a)
 }
 
 template <>
-hipError_t launch_batch_step<S>(int Q, int coll, int fm, int mode, const BatchArgs &a, const Launch &l) {
-  LBM_KEY_SWITCH(9, 19, launch_batch_qcf, mode, a, l)
+hipError_t launch_batch_step<S>(int Q, int coll, int fm, int mode, int occ, const BatchArgs &a, const Launch &l) {
+  LBM_KEY_SWITCH(9, 19, launch_batch_qcf, mode, occ, a, l)
 }
 
 #elif LBM_PART == 2
 
 template <int Q, int COLL, int FM>
 static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) {
+  const int occ = a.t.occ;
   LBM_MODE_SWITCH(step_kernel, a)
 }
 
@@ -74,6 +79,7 @@ hipError_t launch_step_q15_q27<S>(int Q, int coll, int fm, int mode, const StepA
 
 template <int Q, int COLL, int FM>
 static hipError_t launch_step_qcf(int mode, const StepArgs &a, const Launch &l) {
+  const int occ = a.t.occ;
   LBM_MODE_SWITCH(step_kernel, a)
 }
 

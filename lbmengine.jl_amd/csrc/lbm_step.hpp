@@ -1475,11 +1475,20 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   }
 }
 
-// One step kernel over one lattice.  Registers are capped at the plain
-// path's needs (FP32/FP16: 64, eight waves per SIMD; FP64: 128, four), so the
+// Waves per SIMD a step kernel is built for.  OCC = 0: registers capped at
+// the tiles' needs (FP32/FP16: 64, eight waves; FP64: 128, four), so the
 // wider box path spills rather than lowering the occupancy of every tile.
-template <int Q, int S, int COLL, int FM, int MODE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
+// OCC = 1: twice the registers and half the waves; the box path then runs
+// without spills, which pays where the box blocks' chain of dependent steps
+// outlasts the tiles (lattices of up to about 10^6 nodes with particles:
+// 96^3 FP32 with a squirmer r = 10, 50 -> 34 us per fused kernel; 192^3
+// FP16 r = 12, 94 -> 110 us).  The autotuner times both.
+template <int S, int OCC>
+constexpr int kWavesPerEU = OCC ? (S == kF64 ? 2 : 4) : (S == kF64 ? 4 : 8);
+
+// One step kernel over one lattice.
+template <int Q, int S, int COLL, int FM, int MODE, int OCC>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<S, OCC>)))
 step_kernel(StepArgs a) {
   step_block<Q, S, COLL, FM, MODE>(a, blockIdx.x);
 }
@@ -1500,8 +1509,8 @@ struct BatchArgs {
   uint16_t seg[2 * kMaxBatch];
 };
 
-template <int Q, int S, int COLL, int FM, int MODE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == kF64 ? 4 : 8)))
+template <int Q, int S, int COLL, int FM, int MODE, int OCC>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<S, OCC>)))
 batch_step_kernel(BatchArgs ba) {
   const int b = blockIdx.x;
   int lo = 0, hi = ba.n - 1;  // the last segment starting at or before b

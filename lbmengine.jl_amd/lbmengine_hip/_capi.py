@@ -91,6 +91,10 @@ SIGNATURES = {
                       C.POINTER(C.c_int64)],
     "lbm_set_interactions": [_vp, C.POINTER(LbmInteraction), C.c_int32],
     "lbm_set_field_schedule": [_vp, _dp, C.c_int64, C.c_int64],
+    "lbm_set_occupancy": [_vp, C.c_int32],
+    "lbm_get_occupancy": [_vp, C.POINTER(C.c_int32)],
+    "lbm_batch_set_occupancy": [_vp, C.c_int32],
+    "lbm_batch_get_occupancy": [_vp, C.POINTER(C.c_int32)],
     "lbm_set_trajectory_log": [_vp, C.c_int64],
     "lbm_get_trajectory": [_vp, C.c_int32, C.c_int64, C.c_int64, _dp],
     "lbm_write_fluid_csv": [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.c_int64, C.c_double,

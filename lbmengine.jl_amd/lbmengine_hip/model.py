@@ -613,6 +613,16 @@ class LBMmodel:
         _capi.check(self._fn["lbm_get_tile_order"](self._h, C.byref(v)))
         return v.value
 
+    def set_occupancy(self, low: int) -> None:
+        """1: the gathering kernels built for half the waves per SIMD and
+        twice the registers (faster box path); 0: the default."""
+        _capi.check(self._fn["lbm_set_occupancy"](self._h, int(low)))
+
+    def get_occupancy(self) -> int:
+        v = C.c_int32()
+        _capi.check(self._fn["lbm_get_occupancy"](self._h, C.byref(v)))
+        return v.value
+
     def autotune(self, reps: int = 20):
         """Pick the fastest block shape of the stream-collide kernel."""
         best = (C.c_int32 * 3)()
@@ -716,6 +726,14 @@ class LBMbatch:
     def get_tile_order(self) -> int:
         v = C.c_int32()
         _capi.check(self._fn["lbm_batch_get_tile_order"](self._b, C.byref(v)))
+        return v.value
+
+    def set_occupancy(self, low: int) -> None:
+        _capi.check(self._fn["lbm_batch_set_occupancy"](self._b, int(low)))
+
+    def get_occupancy(self) -> int:
+        v = C.c_int32()
+        _capi.check(self._fn["lbm_batch_get_occupancy"](self._b, C.byref(v)))
         return v.value
 
     def autotune(self, reps: int = 10):
