@@ -37,6 +37,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -178,16 +179,13 @@ def adjoint_bench(lh, args):
     """The gradient workload: adjoint time and the check against FP64
     central differences."""
     from lbmengine_hip import adjoint
-    n, r, ticks, K, window = 96, 10.0, args.adjoint_ticks, 100, min(500, args.adjoint_ticks)
-    rng = np.random.default_rng(args.seed + 5)
-    F = rng.normal(0.0, 1e-7, (n, n, n, 3))
-    lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
-    pos = rng.uniform(lo, hi, 3)
-    e = rng.standard_normal(3)
-    e /= np.linalg.norm(e)
+    n, r, ticks, K, window = ADJ_N, ADJ_R, args.adjoint_ticks, 100, min(500, args.adjoint_ticks)
+    F, pos, e = adjoint_workload(args.seed)
+    rng = np.random.default_rng(args.seed + 6)
     B1, B2 = 1e-3, 0.0
     opts = dict(ticks=ticks, window=window, target_speed=1e-4, force_penalty=1.0)
     m = adjoint_model(lh, "f32", n, r, B1, B2, F, pos, e)
+    m.autotune(5)  # the recomputation's forward kernels
     g = adjoint.swimming_loss_gradient(m, checkpoint_every=K, **opts)
     del m
     # the gradient holds the starting populations fixed; so do the runs below
@@ -247,23 +245,51 @@ def kernel_times(m, steps):
 # kernel reads the tick's populations, reads and writes the adjoint
 # populations (compute type), reads the force field and updates dL/dF (FP64):
 # 19*4*3 + 12 + 48 = 288 B per node in FP32.
-ADJ_N, ADJ_BYTES_PER_NODE = 96, 288
+ADJ_N, ADJ_BYTES_PER_NODE, ADJ_R = 96, 288, 10.0
+
+
+def adjoint_workload(seed):
+    """(force field, swimmer position, swimming direction) of the gradient
+    workload."""
+    n, r = ADJ_N, ADJ_R
+    rng = np.random.default_rng(seed + 5)
+    F = rng.normal(0.0, 1e-7, (n, n, n, 3))
+    lo, hi = -n // 2 + r + 6, n // 2 - 1 - r - 6
+    pos = rng.uniform(lo, hi, 3)
+    e = rng.standard_normal(3)
+    e /= np.linalg.norm(e)
+    return F, pos, e
+
+
+def adjoint_near_nodes(seed):
+    """Nodes of the swimmer's near box (its link box grown by one node) at the
+    start: the adjoint's near kernel takes these, the bulk kernel the rest."""
+    _, pos, _ = adjoint_workload(seed)
+    x0, count = -(ADJ_N // 2), 1
+    for k in range(3):
+        lo = math.floor(pos[k] - ADJ_R - x0) - 3
+        hi = math.ceil(pos[k] + ADJ_R - x0) + 3
+        count *= min(hi, ADJ_N - 1) - max(lo, 0) + 1
+    return count
 
 
 def _kernel_family(name):
-    """Short key of a traced kernel: the fused step kernel of a storage, the
-    fused batched kernel, the adjoint kernel or the copy kernel."""
+    """Short key of a traced kernel: the fused step kernel of a storage (either
+    occupancy build), the fused batched kernel, the adjoint's bulk and near
+    kernels or the copy kernel."""
     n = name.replace(" ", "")
     if n.startswith("void"):
         n = n[4:]
     if "copy_kernel" in n:
         return "copy_kernel"
-    if n.startswith("lbm::step_kernel<") and n.endswith(",1>(lbm::StepArgs)"):
-        return "step_kernel/" + {"0": "f64", "1": "f32", "2": "f16", "3": "f16c"}[n.split("<")[1].split(",")[1]]
-    if n.startswith("lbm::batch_step_kernel<") and n.endswith(",1>(lbm::BatchArgs)"):
+    targs = n.split("<", 1)[1].split(">", 1)[0].split(",") if "<" in n else []
+    # step_kernel<Q, S, COLL, FM, MODE, OCC>: MODE 1 is the fused kernel
+    if n.startswith("lbm::step_kernel<") and len(targs) >= 5 and targs[4] == "1":
+        return "step_kernel/" + {"0": "f64", "1": "f32", "2": "f16", "3": "f16c"}[targs[1]]
+    if n.startswith("lbm::batch_step_kernel<") and len(targs) >= 5 and targs[4] == "1":
         return "batch_step_kernel"
     if n.startswith("lbm::adjoint_kernel<"):
-        return "adjoint_kernel"
+        return "adjoint_near_kernel" if targs[-1] == "true" else "adjoint_kernel"
     return None
 
 
@@ -294,7 +320,8 @@ def traffic_summary(fetch_dir, write_dir, args, argv):
     kf = copy_bytes / (f[("copy_kernel", "FETCH_SIZE")] * 1024)
     kw = copy_bytes / (w[("copy_kernel", "WRITE_SIZE")] * 1024)
     algo = {"batch_step_kernel": 2 * 19 * 4 * sum(s[0] ** 3 for s in batch_specs(args.seed)),
-            "adjoint_kernel": ADJ_BYTES_PER_NODE * ADJ_N ** 3}
+            "adjoint_kernel": ADJ_BYTES_PER_NODE * (ADJ_N ** 3 - adjoint_near_nodes(args.seed)),
+            "adjoint_near_kernel": ADJ_BYTES_PER_NODE * adjoint_near_nodes(args.seed)}
     # (only the headline storage's step kernel: the others in a run belong
     # to other workloads, such as the adjoint's 96^3 forward steps)
     algo[f"step_kernel/{args.storage}"] = 2 * 19 * BYTES[args.storage] * args.n ** 3
@@ -428,7 +455,8 @@ def main():
         argv = " ".join(a for a in sys.argv[1:] if a not in ("--traffic-from", *args.traffic_from))
         out = traffic_summary(*args.traffic_from, args, argv)
         (ROOT / "profiles").mkdir(exist_ok=True)
-        names = {"batch_step_kernel": "traffic_batch_64_boxes.json", "adjoint_kernel": "traffic_adjoint_96.json"}
+        names = {"batch_step_kernel": "traffic_batch_64_boxes.json", "adjoint_kernel": "traffic_adjoint_96.json",
+                 "adjoint_near_kernel": "traffic_adjoint_near_96.json"}
         for fam, rec in out.items():
             dst = ROOT / "profiles" / (names[fam] if fam in names else f"traffic_step_{fam.split('/')[1]}_{args.n}.json")
             dst.write_text(json.dumps(rec, indent=1) + "\n")
