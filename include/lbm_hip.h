@@ -314,8 +314,9 @@ int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap);
  * waves per SIMD the tiles use (FP32/FP16: 8, FP64: 4), capping registers so
  * that the particles' box path spills; 1 for half the waves and twice the
  * registers, which pays on lattices up to about 10^6 nodes with particles,
- * where the box path's dependent chain sets the kernel time.  lbm_autotune
- * times both when the lattice has particles. */
+ * where the box path's dependent chain sets the kernel time, and for FP64
+ * with a force, whose collision spills under the default cap.  lbm_autotune
+ * times both. */
 int lbm_set_occupancy(lbm_handle *h, int32_t low);
 int lbm_get_occupancy(lbm_handle *h, int32_t *low);
 

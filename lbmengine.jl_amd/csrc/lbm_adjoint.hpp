@@ -386,6 +386,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
     }
   }
 
+  // with a single particle, its per-particle sums (near kernel)
+  double acc1[kAdjSums];
+#pragma unroll
+  for (int k = 0; k < kAdjSums; k++) acc1[k] = 0.0;
+
   // dL/dP_k at this node
   R lp[Q];
   R grho = 0;
@@ -413,7 +418,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       lp[q] = v;
     }
     // particle link slots (x, q'): S_q'(x) = P_q̄'(x) + the owners' Ladd
-    // terms; each lane walks its own slots in increasing q'
+    // terms; each lane walks its own slots in increasing q' (with a single
+    // particle, its per-particle sums are taken in the same walk)
     uint32_t slots = NEAR ? own_any : 0u;
     while (slots) {
       const int qq = __builtin_ctz(slots);
@@ -442,6 +448,23 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
         reg_set(udc_o, p, udc);
         reg_set(sh_o, p, sh);
         shat_sum += sh;
+        if (a.np == 1) {
+          // the sums below for particle 0 (the only owner): v = fs + its
+          // Ladd term, s = fs + v (+ the deviation shift), dL/dv = lam + sh
+          const double fs = (double)reg_get(P, ob);
+          const double shift = S == kF64 ? 0.0 : 2.0 * dir_w<Q>(qq) * a.c.rho0;
+          const double v = fs + lc * (double)rho * udc;
+          const double s_p = fs + v + shift;
+          const double aj = lam + sh - 0.0;
+          const double ubar[3] = {aj * lc * (double)rho * cq[0] * a.ls, aj * lc * (double)rho * cq[1] * a.ls,
+                                  aj * lc * (double)rho * cq[2] * a.ls};
+          node_velocity_vjp(xs, Pp, ubar, acc1);
+          const double *tp = pt + 3;
+          const double cxt[3] = {cq[1] * tp[2] - cq[2] * tp[1], cq[2] * tp[0] - cq[0] * tp[2],
+                                 cq[0] * tp[1] - cq[1] * tp[0]};
+#pragma unroll
+          for (int k = 0; k < 3; k++) acc1[kAdjR + k] += a.coeff * a.ls * s_p * cxt[k];
+        }
       }
       // dL/dfs = lam + 2 sum shat; dL/dv_j = lam + sum_{i >= j} shat_i
       reg_add(lp, ob, (R)(lam + 2.0 * shat_sum));
@@ -499,7 +522,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
 #pragma unroll
     for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
     const uint32_t own_p = reg_get(own, p);
-    if (any_link && own_p != 0) {
+    if (a.np == 1) {
+#pragma unroll
+      for (int k = 0; k < kAdjSums; k++) acc[k] = acc1[k];
+    } else if (any_link && own_p != 0) {
       const DevParticle &Pp = a.parts[p];
       uint32_t slots = own_p;
       while (slots) {
@@ -549,10 +575,16 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       if (tid < kAdjSums) dst[tid] = 0.0;
       continue;
     }
+    if (__any(own_p != 0)) {
 #pragma unroll
-    for (int k = 0; k < kAdjSums; k++) {
-      const DD s = wave_sum(DD{acc[k], 0.0});
-      if (lane == kWaveSumLane) red[wave][k] = s;
+      for (int k = 0; k < kAdjSums; k++) {
+        const DD s = wave_sum(DD{acc[k], 0.0});
+        if (lane == kWaveSumLane) red[wave][k] = s;
+      }
+    } else if (lane == kWaveSumLane) {
+      // (a wave without a link slot of p adds zeros)
+#pragma unroll
+      for (int k = 0; k < kAdjSums; k++) red[wave][k] = DD{0.0, 0.0};
     }
     __syncthreads();
     if (tid < kAdjSums) {

@@ -1412,8 +1412,10 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     int32_t bx, by, bz, remap, occ;
   };
   std::vector<Cand> cands;
-  // the low-occupancy kernels only change the box path: timed with particles
-  const int nocc = h->parts_h.empty() ? 1 : 2;
+  // both occupancy builds: the low one runs the box path without spills,
+  // and the tiles too where the collision needs more registers than the
+  // default cap (FP64 with a force: 157 against 128)
+  const int nocc = 2;
   for (int32_t bx : bxs)
     for (int32_t by : {1, 2, 4, 8, 16})
       for (int32_t bz : {1, 2, 4}) {

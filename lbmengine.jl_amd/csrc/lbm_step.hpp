@@ -139,15 +139,18 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
   double rho = f[0];
 #pragma unroll
   for (int q = 1; q < Q; q++) rho = rho + f[q];
+  // Terms whose velocity component is 0 are left out of the sums here and
+  // below: each would add a signed zero to a sum that starts at +0 and so
+  // changes no bit of it (the compiler may not drop them itself, since f *
+  // 0.0 is not 0 for every f).
   double mom[3];
-  const int cq0[3] = {L::cx[0], L::cy[0], L::cz[0]};
 #pragma unroll
   for (int k = 0; k < D; k++) {
-    double acc = f[0] * static_cast<double>(cq0[k]);
+    double acc = 0.0;
 #pragma unroll
-    for (int q = 1; q < Q; q++) {
+    for (int q = 0; q < Q; q++) {
       const int ck = k == 0 ? L::cx[q] : (k == 1 ? L::cy[q] : L::cz[q]);
-      acc = acc + f[q] * static_cast<double>(ck);
+      if (ck != 0) acc = acc + f[q] * static_cast<double>(ck);
     }
     mom[k] = acc;
   }
@@ -170,10 +173,12 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
   double fe[Q], udc_[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const double ci[3] = {L::cx[q] * c.ls, L::cy[q] * c.ls, L::cz[q] * c.ls};
+    const int cq[3] = {L::cx[q], L::cy[q], L::cz[q]};
+    const double ci[3] = {cq[0] * c.ls, cq[1] * c.ls, cq[2] * c.ls};
     double udc = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; k++) udc = udc + u[k] * ci[k];
+    for (int k = 0; k < D; k++)
+      if (cq[k] != 0) udc = udc + u[k] * ci[k];
     udc_[q] = udc;
     const double first = udc * c.invC2s + (udc * udc) * (0.5 * c.invC4s);
     const double second = first - uu * (0.5 * c.invC2s);
@@ -195,7 +200,10 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
       om = cb * (fp - ep) + cm * (fm - em);
     }
     if constexpr (FM != LBM_FORCE_NONE) {
-      const double ci[3] = {L::cx[q] * c.ls, L::cy[q] * c.ls, L::cz[q] * c.ls};
+      // (with c_qk = 0, the c_qk and (u.c) c_qk terms are signed zeros added
+      // to nonzero or +0 values: left out)
+      const int cq[3] = {L::cx[q], L::cy[q], L::cz[q]};
+      const double ci[3] = {cq[0] * c.ls, cq[1] * c.ls, cq[2] * c.ls};
       const double udc = udc_[q];
       double second[3];
 #pragma unroll
@@ -204,16 +212,23 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
       if constexpr (COLL == LBM_COLLISION_BGK) {
         double inter = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; k++) inter = inter + ((-u[k] + ci[k]) * c.invC2s + second[k]) * F[k];
+        for (int k = 0; k < D; k++) {
+          const double a = cq[k] != 0 ? (-u[k] + ci[k]) * c.invC2s + second[k] : (-u[k]) * c.invC2s;
+          inter = inter + a * F[k];
+        }
         S = L::w(q) * c.dt * (1.0 - c.dt / (2.0 * c.tau)) * inter;
       } else {
         double inter = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; k++) inter = inter + ((-u[k]) * c.invC2s + second[k]) * F[k];
+        for (int k = 0; k < D; k++) {
+          const double a = cq[k] != 0 ? (-u[k]) * c.invC2s + second[k] : (-u[k]) * c.invC2s;
+          inter = inter + a * F[k];
+        }
         const double plus = (1.0 - c.dt / (2.0 * c.tau)) * inter;
         double inter2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; k++) inter2 = inter2 + F[k] * (ci[k] * c.invC2s);
+        for (int k = 0; k < D; k++)
+          if (cq[k] != 0) inter2 = inter2 + F[k] * (ci[k] * c.invC2s);
         const double minus = (1.0 - c.dt / (2.0 * c.tau_minus)) * inter2;
         S = L::w(q) * c.dt * (plus + minus);
       }
@@ -1482,7 +1497,8 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
 // without spills, which pays where the box blocks' chain of dependent steps
 // outlasts the tiles (lattices of up to about 10^6 nodes with particles:
 // 96^3 FP32 with a squirmer r = 10, 50 -> 34 us per fused kernel; 192^3
-// FP16 r = 12, 94 -> 110 us).  The autotuner times both.
+// FP16 r = 12, 94 -> 110 us), and so do the tiles of FP64 with a force,
+// whose collision needs 157 registers.  The autotuner times both.
 template <int S, int OCC>
 constexpr int kWavesPerEU = OCC ? (S == kF64 ? 2 : 4) : (S == kF64 ? 4 : 8);
 
@@ -1665,15 +1681,18 @@ __device__ inline void node_hydro(const AuxArgs &a, int64_t n, int x, int y, int
   rho = f[0];
 #pragma unroll
   for (int q = 1; q < Q; q++) rho = rho + f[q];
+  // Terms whose velocity component is 0 are left out of the sums here and
+  // below: each would add a signed zero to a sum that starts at +0 and so
+  // changes no bit of it (the compiler may not drop them itself, since f *
+  // 0.0 is not 0 for every f).
   double mom[3];
-  const int cq0[3] = {L::cx[0], L::cy[0], L::cz[0]};
 #pragma unroll
   for (int k = 0; k < D; k++) {
-    double acc = f[0] * static_cast<double>(cq0[k]);
+    double acc = 0.0;
 #pragma unroll
-    for (int q = 1; q < Q; q++) {
+    for (int q = 0; q < Q; q++) {
       const int ck = k == 0 ? L::cx[q] : (k == 1 ? L::cy[q] : L::cz[q]);
-      acc = acc + f[q] * static_cast<double>(ck);
+      if (ck != 0) acc = acc + f[q] * static_cast<double>(ck);
     }
     mom[k] = acc;
   }
