@@ -54,7 +54,13 @@ extern "C" {
 #define LBM_PARTICLE_BEAD 0     /* addBead!      gpu/initMethods.jl:7-89   */
 #define LBM_PARTICLE_SQUIRMER 1 /* addSquirmer!  gpu/initMethods.jl:91-222 */
 
-#define LBM_MAX_PARTICLES 8
+/* Particles per lattice.  Up to 8 (the particles a box block works with),
+ * every kernel of a run applies the previous one's moves itself; with more,
+ * a small move kernel follows each step kernel, and the run fails with
+ * LBM_ERANGE if more than 8 particles come within reach of one another
+ * (their link boxes, grown by two nodes, overlap).  The adjoint, the boxes
+ * of a batch and lbm_run_noalloc take at most 8. */
+#define LBM_MAX_PARTICLES 64
 
 /* lbm_config.flags */
 #define LBM_FLAG_INCOMPRESSIBLE 1 /* isFluidCompressible = false: the equilibrium

@@ -70,8 +70,14 @@ struct lbm_handle {
   void *uwall_d = nullptr;         // wall velocities, compute type
   DevParticle *parts_d = nullptr;
   std::vector<DevParticle> parts_h;  // parameters as added
-  double *partial = nullptr;     // 2 parities x [box block][particle][6][hi, lo]
+  double *partial = nullptr;     // 2 parities x [box block][local particle][6][hi, lo]
   size_t partial_cap = 0;        // doubles per parity
+  // more than kLocalParticles particles: the ids the partials are written
+  // under (2 parities x [box block][local particle]) and the flag word the
+  // box blocks raise when too many particles reach one of them
+  int32_t *ids = nullptr;
+  size_t ids_cap = 0;            // ints per parity
+  int32_t *flags_d = nullptr;
   // box blocks leading the step kernel's grid (see StepParticles)
   int32_t first_block[LBM_MAX_PARTICLES + 1] = {0};
   Tiling tiling{};
@@ -213,8 +219,23 @@ int plan_boxes(lbm_handle *h, int64_t threads) {
     HIP_TRY(hipMalloc(&h->linkval, lv), "hipMalloc link values");
     h->linkval_cap = lv;
   }
+  if (h->parts_h.size() > (size_t)kLocalParticles) {
+    const size_t nid = (size_t)std::max(blocks, 1) * kLocalParticles;
+    if (nid > h->ids_cap) {
+      HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+      if (h->ids) hipFree(h->ids);
+      h->ids = nullptr;
+      h->ids_cap = 0;
+      HIP_TRY(hipMalloc(&h->ids, 2 * nid * sizeof(int32_t)), "hipMalloc partial ids");
+      h->ids_cap = nid;
+    }
+    if (!h->flags_d) {
+      HIP_TRY(hipMalloc(&h->flags_d, 256), "hipMalloc flags");
+      HIP_TRY(hipMemset(h->flags_d, 0, 256), "clear flags");
+    }
+  }
   // (between runs no move is pending, so the partials may be reallocated)
-  const size_t need = (size_t)std::max(blocks, 1) * LBM_MAX_PARTICLES * 6 * 2;
+  const size_t need = (size_t)std::max(blocks, 1) * kLocalParticles * 6 * 2;
   if (need > h->partial_cap) {
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     if (h->partial) hipFree(h->partial);
@@ -254,6 +275,12 @@ StepArgs step_args(lbm_handle *h, const void *fin, void *fout, int pin) {
   for (int k = 0; k < 3; k++) sp.coords[k] = h->coords_d[k];
   sp.partial_in = h->partial + (size_t)pin * h->partial_cap;
   sp.partial_out = h->partial + (size_t)(pin ^ 1) * h->partial_cap;
+  sp.many = h->parts_h.size() > (size_t)kLocalParticles;
+  if (sp.many) {
+    sp.ids_in = h->ids + (size_t)pin * h->ids_cap;
+    sp.ids_out = h->ids + (size_t)(pin ^ 1) * h->ids_cap;
+    sp.flags = h->flags_d;
+  }
   sp.linkval = h->linkval;
   sp.inter = h->inter_d;
   sp.ninter = (int32_t)h->inter_h.size();
@@ -326,13 +353,27 @@ hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int
   }
 }
 
+// The move kernel after a gathering kernel that read records `pin` (more
+// than kLocalParticles particles): it reads that kernel's partials and ids
+// (parity pin ^ 1) and records pin, and writes the moved records to pin ^ 1.
+hipError_t launch_moves(lbm_handle *h, int pin) {
+  StepArgs m = step_args(h, nullptr, nullptr, pin ^ 1);
+  m.sp.parts_in = parts_of(h, pin);
+  m.sp.parts_out = parts_of(h, pin ^ 1);
+  return launch_move(m.sp, h->stream);
+}
+
+bool many_particles(const lbm_handle *h) { return h->parts_h.size() > (size_t)kLocalParticles; }
+
 // One fused kernel (gather tick k, collide tick k + 1) from buffer `in`
-// with particle records `pin`; the timing events bracket it.
+// with particle records `pin` (then, with many particles, the move kernel);
+// the timing events bracket the step kernel.
 hipError_t fused_once(lbm_handle *h, int in, int pin, hipEvent_t e0, hipEvent_t e1) {
   if (e0) hipEventRecord(e0, h->stream);
   hipError_t e = launch_mode(h, kFused, h->f[in], h->f[in ^ 1], pin);
   if (e != hipSuccess) return e;
   if (e1) hipEventRecord(e1, h->stream);
+  if (many_particles(h)) e = launch_moves(h, pin);
   return e;
 }
 
@@ -471,6 +512,16 @@ int upload_walls(lbm_handle *h) {
 
 int check_particles(lbm_handle *h) {
   if (h->parts_h.empty()) return LBM_OK;
+  if (h->flags_d) {
+    int32_t fl = 0;
+    HIP_TRY(hipMemcpy(&fl, h->flags_d, sizeof fl, hipMemcpyDeviceToHost), "copy flags");
+    if (fl) {
+      HIP_TRY(hipMemset(h->flags_d, 0, sizeof fl), "clear flags");
+      return fail(LBM_ERANGE, "more than " + std::to_string(kLocalParticles) +
+                                  " particles came within reach of one another (their owned regions, the link "
+                                  "boxes grown by two nodes, overlapped); the populations are not reliable");
+    }
+  }
   std::vector<DevParticle> tmp(h->parts_h.size());
   HIP_TRY(hipMemcpy(tmp.data(), parts_of(h, h->pcur), tmp.size() * sizeof(DevParticle), hipMemcpyDeviceToHost),
           "copy particles");
@@ -517,6 +568,8 @@ int lbm_destroy(lbm_handle *h) {
   if (h->inter_d) hipFree(h->inter_d);
   if (h->traj_d) hipFree(h->traj_d);
   if (h->bsched_d) hipFree(h->bsched_d);
+  if (h->ids) hipFree(h->ids);
+  if (h->flags_d) hipFree(h->flags_d);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->run_start) hipEventDestroy(h->run_start);
   if (h->run_stop) hipEventDestroy(h->run_stop);
@@ -1214,10 +1267,14 @@ int run_ticks(lbm_handle *h, int64_t nsteps, bool direct) {
   }
   e = launch_mode(h, kGather, h->f[a], h->f[a ^ 1], h->pcur);
   if (e != hipSuccess) return hip_fail(e, "step launch");
+  if (many_particles(h)) {
+    e = launch_moves(h, h->pcur);
+    if (e != hipSuccess) return hip_fail(e, "move launch");
+  }
   a ^= 1;
   h->pcur ^= 1;
   h->cur = a;
-  if (!h->parts_h.empty()) {
+  if (!h->parts_h.empty() && !many_particles(h)) {
     // the last tick's move
     StepArgs fa = step_args(h, nullptr, nullptr, h->pcur);
     fa.sp.parts_out = parts_of(h, h->pcur);  // in place
@@ -1290,6 +1347,9 @@ int lbm_run(lbm_handle *h, int64_t nsteps) {
 
 int lbm_run_noalloc(lbm_handle *h, int64_t nsteps) {
   if (!h) return fail(LBM_EINVAL, "null handle");
+  if (h->parts_h.size() > (size_t)kLocalParticles)
+    return fail(LBM_EINVAL, "lbm_run_noalloc takes at most " + std::to_string(kLocalParticles) +
+                                " particles; use lbm_run");
   if (nsteps < 0) return fail(LBM_EINVAL, "nsteps must be non-negative");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised (lbm_init_uniform or lbm_set_populations)");
   if (nsteps == 0) return LBM_OK;
@@ -1755,6 +1815,9 @@ bool batch_has_particles(const lbm_batch *b) {
 int prepare_boxes(lbm_batch *b) {
   for (lbm_handle *h : b->boxes) {
     if (!h->populated) return fail(LBM_ESTATE, "a box of the batch has no populations yet");
+    if (h->parts_h.size() > (size_t)kLocalParticles)
+      return fail(LBM_EINVAL, "a box of a batch takes at most " + std::to_string(kLocalParticles) +
+                                  " particles; run a box with more alone");
     if (const int rc = adopt_shape(h, b->shape)) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   }
@@ -2160,6 +2223,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   if (h->cfg.semantics != LBM_SEMANTICS_GPU) return fail(LBM_EINVAL, "the adjoint follows the GPU semantics");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
   if (!h->inter_h.empty()) return fail(LBM_EINVAL, "the adjoint does not differentiate particle interactions");
+  if (h->parts_h.size() > (size_t)kLocalParticles)
+    return fail(LBM_EINVAL, "the adjoint takes at most " + std::to_string(kLocalParticles) + " particles");
   const int np = (int)h->parts_h.size();
   const int64_t T = cfg->ticks, K = cfg->checkpoint_every, Wn = cfg->window;
   if (T < 1 || K < 1 || Wn < 1 || Wn > T) return fail(LBM_EINVAL, "need ticks >= 1, checkpoint_every >= 1, 1 <= window <= ticks");

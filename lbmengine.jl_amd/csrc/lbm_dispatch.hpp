@@ -34,6 +34,7 @@ hipError_t launch_scan(int Q, const AuxArgs &a, unsigned long long *counts, cons
 
 hipError_t launch_link_mask(int Q, const LinkMaskArgs &a, const Launch &l);
 hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);
+hipError_t launch_move(const StepParticles &sp, hipStream_t stream);
 hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 

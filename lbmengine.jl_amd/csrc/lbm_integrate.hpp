@@ -59,6 +59,11 @@ __device__ __forceinline__ DD wave_sum(DD v) {
 }
 constexpr int kWaveSumLane = 63;
 
+// Most particles one box block works with: the lattice's particles when it
+// has at most this many, else those whose owned regions meet its own
+// particle's (a link or band node of any other particle lies outside).
+constexpr int kLocalParticles = 8;
+
 // What the kernels of a tick need for the particles.  A box kernel reads
 // the records of its tick (parts_in) and, when they are pending, first
 // applies the move of the previous box kernel from its partials
@@ -69,8 +74,18 @@ struct StepParticles {
   const DevParticle *parts_in;
   DevParticle *parts_out;
   const double *coords[3];
-  const double *partial_in;  // [box block][particle][6][hi, lo] of the previous box kernel
+  const double *partial_in;  // [box block][local particle][6][hi, lo] of the previous box kernel
   double *partial_out;       // the same, of this kernel
+  // Lattices with more than kLocalParticles particles (many = 1): the box
+  // blocks take the records as moved by the move kernel after each
+  // gathering kernel, write their partials under the ids of their local
+  // particles, and raise flags[0] when more than kLocalParticles particles
+  // reach one block.
+  const int32_t *ids_in;     // [box block][local particle]: particle id, or -1
+  int32_t *ids_out;
+  int32_t *flags;
+  int32_t many;
+  int32_t reserved_m;
   void *linkval;             // [box block][T (Q - 1)]: link-slot values (compute type)
   const lbm_interaction *inter;  // particle-particle interaction terms, in application order
   double *traj;              // per-tick log: [particle][traj_cap][9], or null
@@ -509,7 +524,7 @@ __device__ inline DD prefetch_partial(const StepParticles &sp) {
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int pk = tid >> 6, b = __lane_id();
   if (pk >= 6 * sp.np || b >= sp.nbox) return DD{0.0, 0.0};
-  const double *src = sp.partial_in + (((int64_t)b * LBM_MAX_PARTICLES + pk / 6) * 6 + pk % 6) * 2;
+  const double *src = sp.partial_in + (((int64_t)b * kLocalParticles + pk / 6) * 6 + pk % 6) * 2;
   return DD{src[0], src[1]};
 }
 
@@ -517,7 +532,7 @@ __device__ inline DD prefetch_partial(const StepParticles &sp) {
 // the per-tick log.
 __device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle *recs, const DD *pre = nullptr,
                                            bool log = false) {
-  __shared__ double tot[LBM_MAX_PARTICLES][6];
+  __shared__ double tot[kLocalParticles][6];
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   // block sizes are whole waves, so every wave reaches wave_sum with 64 lanes
@@ -531,7 +546,7 @@ __device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle 
         s = dd_merge(s, *pre);
         continue;
       }
-      const double *src = sp.partial_in + (((int64_t)b * LBM_MAX_PARTICLES + p) * 6 + k) * 2;
+      const double *src = sp.partial_in + (((int64_t)b * kLocalParticles + p) * 6 + k) * 2;
       s = dd_merge(s, DD{src[0], src[1]});
     }
     s = wave_sum(s);
@@ -552,6 +567,45 @@ __device__ inline void apply_pending_moves(const StepParticles &sp, DevParticle 
   }
   if (tid < sp.np && recs[tid].pending) move_record(recs[tid], tid, tot[tid], sp, log);
   __syncthreads();
+}
+
+// The move of particle p of a lattice with more than kLocalParticles
+// particles, after a gathering kernel (one wave): its momentum exchange is
+// the sum, over the box blocks in order, of the partials they wrote under
+// its id (each lane takes the blocks lane, lane + 64, ...; the waves' sums
+// then combine in a fixed order); then the interactions, from the positions
+// before any particle of the tick moves, and moveParticles!.  Records from
+// parts_in, moved ones to parts_out.
+__device__ inline void move_particle_block(const StepParticles &sp, int p) {
+  const int lane = __lane_id();
+  DD s[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) s[k] = DD{0.0, 0.0};
+  for (int b = lane; b < sp.nbox; b += 64)
+    for (int j = 0; j < kLocalParticles; j++) {
+      if (sp.ids_in[(int64_t)b * kLocalParticles + j] != p) continue;
+      const double *src = sp.partial_in + ((int64_t)b * kLocalParticles + j) * 12;
+#pragma unroll
+      for (int k = 0; k < 6; k++) s[k] = dd_merge(s[k], DD{src[2 * k], src[2 * k + 1]});
+    }
+  __shared__ double tot[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const DD t = wave_sum(s[k]);
+    if (lane == kWaveSumLane) {
+#pragma clang fp contract(off)
+      tot[k] = -sp.coeff * ((t.hi + t.lo) * sp.ls);
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double F[6];
+    for (int k = 0; k < 6; k++) F[k] = tot[k];
+    if (sp.ninter > 0) add_interaction_forces(sp, sp.parts_in, p, F);
+    DevParticle P = sp.parts_in[p];
+    move_record(P, p, F, sp, true);
+    sp.parts_out[p] = P;
+  }
 }
 
 }  // namespace lbm

@@ -1,6 +1,8 @@
-// lbm_particles.hip — the link-bit kernel used to list a particle's
-// boundary links (lbm_get_links), and the plain copy kernel that measures
-// the HBM ceiling.
+// lbm_particles.hip — the kernels that move the particles between step
+// kernels (the last pending move of a run; every move on lattices with more
+// than kLocalParticles particles), the link-bit kernel used to list a
+// particle's boundary links (lbm_get_links), and the plain copy kernel that
+// measures the HBM ceiling.
 #include "lbm_dispatch.hpp"
 
 namespace lbm {
@@ -26,7 +28,7 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
 
 // Applies the move that the last box kernel of a run left pending.
 __device__ __forceinline__ void finalize_block(const StepParticles &sp) {
-  __shared__ DevParticle recs[LBM_MAX_PARTICLES];
+  __shared__ DevParticle recs[kLocalParticles];
   constexpr int kWords = sizeof(DevParticle) / 8;
   const int tid = threadIdx.x;
   for (int w = tid; w < sp.np * kWords; w += blockDim.x)
@@ -38,6 +40,15 @@ __device__ __forceinline__ void finalize_block(const StepParticles &sp) {
 }
 
 __global__ void __launch_bounds__(256) finalize_kernel(StepParticles sp) { finalize_block(sp); }
+
+// Lattices with more than kLocalParticles particles: every particle's move
+// after a gathering kernel, one wave per particle.
+__global__ void __launch_bounds__(64) move_kernel(StepParticles sp) { move_particle_block(sp, blockIdx.x); }
+
+hipError_t launch_move(const StepParticles &sp, hipStream_t stream) {
+  hipLaunchKernelGGL(move_kernel, dim3((unsigned)sp.np), dim3(64), 0, stream, sp);
+  return hipGetLastError();
+}
 
 // One block per lattice of a batch; boxes[i].sp names lattice i's records
 // and the partials of its last box kernel.

@@ -883,29 +883,69 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 
-  // The particle records of this tick, in LDS (read throughout), with the
-  // regions this kernel's box blocks own (from the records as read); then
-  // the previous box kernel's move, applied by every box block alike.
-  __shared__ DevParticle prec[LBM_MAX_PARTICLES];
-  __shared__ Band bands[LBM_MAX_PARTICLES];
+  // The records of the particles this block works with (its local set), in
+  // LDS (read throughout), with the regions this kernel's box blocks own
+  // (from the records as read).  With at most kLocalParticles particles the
+  // local set is all of them, and the previous box kernel's move is applied
+  // here by every box block alike; with more, the move kernel has moved them
+  // already, and the local set is the particles whose owned regions meet
+  // this block's particle's, in id order (local index j < lpb: id < pb).
+  __shared__ DevParticle prec[kLocalParticles];
+  __shared__ Band bands[kLocalParticles];
+  __shared__ int32_t lid[kLocalParticles];  // id of local particle j
+  __shared__ int32_t nloc_s, lpb_s;
+  int pb = 0;
+  while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
   // the records and the previous kernel's partials load together
-  const DD pre = prefetch_partial(sp);
+  const DD pre = sp.many ? DD{0.0, 0.0} : prefetch_partial(sp);
+  if (!sp.many) {
+    if (tid < sp.np) lid[tid] = tid;
+    if (tid == 0) {
+      nloc_s = sp.np;
+      lpb_s = pb;
+    }
+  } else if (tid < 64) {
+    // (one wave decides, over at most 64 particles)
+    bool meet = false;
+    if (tid < sp.np) {
+      meet = true;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        int32_t lo, hi, lo_b, hi_b;
+        owned_range(sp.parts_in[tid], g, k, lo, hi);
+        owned_range(sp.parts_in[pb], g, k, lo_b, hi_b);
+        meet = meet && lo <= hi_b && lo_b <= hi;
+      }
+    }
+    const uint64_t m = __ballot(meet);
+    const int j = __popcll(m & ((uint64_t(1) << tid) - 1));
+    if (meet && j < kLocalParticles) lid[j] = tid;
+    if (tid == 0) {
+      const int cnt = __popcll(m);
+      const int at = __popcll(m & ((uint64_t(1) << pb) - 1));
+      nloc_s = cnt < kLocalParticles ? cnt : kLocalParticles;
+      // too many particles reach this block: the host reports it after the
+      // run (the block still works, on the first kLocalParticles of them)
+      if (cnt > kLocalParticles) atomicOr(sp.flags, 1);
+      lpb_s = at < nloc_s ? at : 0;
+    }
+  }
+  __syncthreads();
+  const int nloc = nloc_s, lpb = lpb_s;
   {
     static_assert(sizeof(DevParticle) % 8 == 0, "records are copied as 8-byte words");
     constexpr int kWords = sizeof(DevParticle) / 8;
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(sp.parts_in);
     uint64_t *dst = reinterpret_cast<uint64_t *>(prec);
-    for (int w = tid; w < sp.np * kWords; w += nthreads) dst[w] = src[w];
+    for (int w = tid; w < nloc * kWords; w += nthreads)
+      dst[w] = reinterpret_cast<const uint64_t *>(sp.parts_in + lid[w / kWords])[w % kWords];
   }
   __syncthreads();
-  int pb = 0;
-  while (pb + 1 < sp.np && bid >= sp.first_block[pb + 1]) pb++;
   // pb's owned region (from the records as read) and the coordinates to
   // stage for it, loaded now so that they arrive during the move
   int bl[3], ext[3];
   for (int k = 0; k < 3; k++) {
     int32_t lo, hi;
-    owned_range(prec[pb], g, k, lo, hi);
+    owned_range(prec[lpb], g, k, lo, hi);
     bl[k] = lo;
     ext[k] = hi - lo + 1;
   }
@@ -928,10 +968,11 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       }
     }
   }
-  if (tid < sp.np) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
-  apply_pending_moves(sp, prec, &pre, bid == 0);  // (synchronises the block)
+  if (tid < nloc) band_of(prec[tid], sp, g, bands[tid]);  // (before the move rewrites prec)
+  if (!sp.many) apply_pending_moves(sp, prec, &pre, bid == 0);  // (synchronises the block)
+  else __syncthreads();
 
-  const Band &B = bands[pb];
+  const Band &B = bands[lpb];
   const int ex = ext[0], ey = ext[1], ez = ext[2];
 
   // Dynamic LDS: coordinate stage (3 axes of coord_stride doubles), link
@@ -981,8 +1022,8 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
     x = bl[0] + gi % ex;
     y = bl[1] + (gi / ex) % ey;
     z = bl[2] + gi / (ex * ey);
-    for (int p = 0; p < pb; p++)
-      if (in_band(bands[p], x, y, z)) mine = false;
+    for (int j = 0; j < lpb; j++)
+      if (in_band(bands[j], x, y, z)) mine = false;
   } else if (mine) {
     band_node(B, plane_pre, row_off, ey, ez, gi, x, y, z);
     // (a node off its region would mean inconsistent tables: never touch it)
@@ -990,12 +1031,12 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       x = bl[0];
       mine = false;
     }
-    for (int p = 0; p < pb; p++)
-      if (in_band(bands[p], x, y, z)) mine = false;
+    for (int j = 0; j < lpb; j++)
+      if (in_band(bands[j], x, y, z)) mine = false;
   }
   node_l[tid] = (uint32_t)(x - bl[0]) | (uint32_t)(y - bl[1]) << 10 | (uint32_t)(z - bl[2]) << 20;
   // box block 0 hands the moved records to the next kernel
-  if (bid == 0) {
+  if (!sp.many && bid == 0) {
     constexpr int kWords = sizeof(DevParticle) / 8;
     for (int w = tid; w < sp.np * kWords; w += nthreads) {
       uint64_t v = reinterpret_cast<const uint64_t *>(prec)[w];
@@ -1030,7 +1071,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   }
   __syncthreads();
   // this kernel's partials are to be applied by the next one
-  if (bid == 0 && tid < sp.np) sp.parts_out[tid].pending = 1;
+  if (!sp.many && bid == 0 && tid < sp.np) sp.parts_out[tid].pending = 1;
   // coordinate k of the node at unwrapped index i (bl_k - 1 <= i <= bl_k + extent)
   auto crd = [&](int k, int i) { return cst[k * cstride + (i - bl[k] + 1)]; };
   const uint32_t wb = mine ? wall_bits<Q>(g, x, y, z) : 0u;
@@ -1051,9 +1092,9 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // Geometry of this tick (bounceBackPrep with the getSphere test): only nodes
   // within the shell (sp.shell) of a sphere surface can be a link's fluid end.
   uint32_t own_any = 0;
-  uint32_t own[LBM_MAX_PARTICLES];
+  uint32_t own[kLocalParticles];
 #pragma unroll
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++) own[p] = 0;
+  for (int p = 0; p < kLocalParticles; p++) own[p] = 0;
   if (node) {
     // coordinates of the node and its neighbours, [1 - c] for offset -c
     double cxv[3], cyv[3], czv[3] = {0.0, 0.0, 0.0};
@@ -1063,7 +1104,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       cyv[i] = crd(1, y - 1 + i);
       if (D == 3) czv[i] = crd(2, z - 1 + i);
     }
-    for (int p = 0; p < sp.np; p++) {
+    for (int p = 0; p < nloc; p++) {
       const DevParticle &P = prec[p];
       if (!in_box(P, x, y, z)) continue;
       // (X + (-R))_1^2 + ... in the getSphere order (gpu/kernels.jl:344-353),
@@ -1095,7 +1136,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         if (d2at(1 - L::cx[q], 1 - L::cy[q], 1 - L::cz[q]) < r2) op |= 1u << q;
       }
 #pragma unroll
-      for (int i = 0; i < LBM_MAX_PARTICLES; i++)
+      for (int i = 0; i < kLocalParticles; i++)
         if (i == p) own[i] = op;
       own_any |= op;
     }
@@ -1111,7 +1152,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // pk: lane | q << 9 | (c + 1) << 14, 16, 18 | weight class << 20 | owners << 22
   __shared__ uint32_t wave_cnt[8];
   __shared__ DD red[8][6];
-  __shared__ DD blk[LBM_MAX_PARTICLES][6];
+  __shared__ DD blk[kLocalParticles][6];
   const int wave = tid >> 6, lane = __lane_id();
   const int nwaves = (nthreads + 63) >> 6;
 
@@ -1126,7 +1167,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
   // rho of this node's populations of the previous tick (massDensity), kept
   // by the collision that produced them
   rho_l[tid] = (own_any || (wb & kMovingLinkBit)) ? rho_side[(int64_t)x + g.nx * (int64_t)row] : R(0);
-  if (tid < 6 * LBM_MAX_PARTICLES) blk[tid / 6][tid % 6] = DD{0.0, 0.0};
+  if (tid < 6 * kLocalParticles) blk[tid / 6][tid % 6] = DD{0.0, 0.0};
   __syncthreads();
   uint32_t off = incl - cnt, total = 0;
   for (int w = 0; w < nwaves; w++) {
@@ -1168,7 +1209,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         if (idx >= c0 && idx < c0 + cap) {
           uint32_t owners = 0;
 #pragma unroll
-          for (int i = 0; i < LBM_MAX_PARTICLES; i++) owners |= ((own[i] >> q) & 1u) << i;
+          for (int i = 0; i < kLocalParticles; i++) owners |= ((own[i] >> q) & 1u) << i;
           fs_l[idx - c0] = bbv[q];
           v_l[idx - c0] = bbv[q];
           pk_l[idx - c0] = (uint32_t)tid | (uint32_t)q << 9 | (uint32_t)(L::cx[q] + 1) << 14 |
@@ -1179,7 +1220,7 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       }
     }
     __syncthreads();
-    for (int p = 0; p < sp.np; p++) {
+    for (int p = 0; p < nloc; p++) {
       const DevParticle &P = prec[p];
       const bool coupled = P.couple_forces || P.couple_torques;
       // Each pass of the block over its slots is reduced over every wave at
@@ -1276,9 +1317,10 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
       for (int e = tid; e < m; e += nthreads) linkval[c0 + e] = v_l[e];
     __syncthreads();  // the next round reuses the list
   }
-  if (tid < 6 * sp.np) {
+  if (sp.many && tid < kLocalParticles) sp.ids_out[(int64_t)bid * kLocalParticles + tid] = tid < nloc ? lid[tid] : -1;
+  if (tid < 6 * nloc) {
     const DD s = blk[tid / 6][tid % 6];
-    double *dst = sp.partial_out + ((int64_t)bid * LBM_MAX_PARTICLES * 6 + tid) * 2;
+    double *dst = sp.partial_out + ((int64_t)bid * kLocalParticles * 6 + tid) * 2;
     dst[0] = s.hi;
     dst[1] = s.lo;
   }

@@ -21,7 +21,8 @@ SEMANTICS = {"gpu": 0, "cpu": 1}
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 FLAG_INCOMPRESSIBLE = 1
 BEAD, SQUIRMER = 0, 1
-MAX_PARTICLES = 8
+MAX_PARTICLES = 64
+LOCAL_PARTICLES = 8  # particles within reach of one box block; also the adjoint's, a batch box's and lbm_run_noalloc's limit
 INTERACTION = {"linear": 0, "bistable": 1, "polar": 2, "dipole": 3, "lennardJones": 4}
 MAX_INTERACTIONS = 256
 
