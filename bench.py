@@ -288,8 +288,10 @@ def _kernel_family(name):
         return "step_kernel/" + {"0": "f64", "1": "f32", "2": "f16", "3": "f16c"}[targs[1]]
     if n.startswith("lbm::batch_step_kernel<") and len(targs) >= 5 and targs[4] == "1":
         return "batch_step_kernel"
+    if n.startswith("lbm::adjoint_near_kernel<"):
+        return "adjoint_near_kernel"
     if n.startswith("lbm::adjoint_kernel<"):
-        return "adjoint_near_kernel" if targs[-1] == "true" else "adjoint_kernel"
+        return "adjoint_kernel"
     return None
 
 

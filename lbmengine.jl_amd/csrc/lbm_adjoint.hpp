@@ -21,6 +21,11 @@
 
 namespace lbm {
 
+// The adjoint takes at most kLocalParticles particles (lbm_adjoint_run
+// refuses more): its per-particle arrays live in registers, and each slot of
+// them costs every thread of the near kernel registers and selects.
+constexpr int kAdjParticles = kLocalParticles;
+
 // Per-particle sums of one adjoint tick: dL/dB1, dL/dB2, then dL/d of the
 // node-velocity snapshot (nR, nV, nW, ne) and of the position R (torque arm).
 enum AdjSum { kAdjB1 = 0, kAdjB2 = 1, kAdjNR = 2, kAdjNV = 5, kAdjNW = 8, kAdjNE = 11, kAdjR = 14, kAdjSums = 17 };
@@ -143,8 +148,8 @@ struct AdjArgs {
   const double *phitau;  // [particle][6]: dL/dF_p (0..2) and dL/dT_p (3..5) of the tick's move
   // the near kernel's blocks: particle p's are [near_first[p], near_first[p + 1]),
   // enough for near_ext[p]^3 nodes of its near box
-  int32_t near_first[LBM_MAX_PARTICLES + 1];
-  int32_t near_ext[LBM_MAX_PARTICLES];
+  int32_t near_first[kAdjParticles + 1];
+  int32_t near_ext[kAdjParticles];
 };
 
 // Near box of a particle: its link box grown by one node (clamped to the
@@ -176,7 +181,6 @@ struct AdjMoveArgs {
   double dt;
 };
 struct AdjFinishArgs {
-  const double *sums;          // [np][kAdjSums] of tick k's adjoint kernel
   const double *mu_in;
   double *mu;                  // [np][kPState] <- dL/d(state at the start of tick k)
   double *gB;                  // [np][2] += dL/dB1, dL/dB2
@@ -185,184 +189,277 @@ struct AdjFinishArgs {
   double gscale;               // 2 (mean - target) / window
 };
 
-// Element i of a small register array, and an update of it, without a
-// dynamic index: a private array indexed by a value the compiler cannot
-// fold is placed in scratch memory, and every access to it then goes through
-// the caches to HBM.
-// (The read masks the elements' bits: a chain of selects of the elements
-// is folded back into one load at a selected address.)
-template <int N, class T>
-__device__ __forceinline__ T reg_get(const T (&a)[N], int i) {
-  using U = std::conditional_t<sizeof(T) == 8, uint64_t, uint32_t>;
-  static_assert(sizeof(T) == sizeof(U), "4- or 8-byte elements");
-  U v = 0;
-#pragma unroll
-  for (int k = 0; k < N; k++) v |= __builtin_bit_cast(U, a[k]) & ((U)0 - (U)(i == k));
-  return __builtin_bit_cast(T, v);
-}
-template <int N, class T>
-__device__ __forceinline__ void reg_add(T (&a)[N], int i, T d) {
-#pragma unroll
-  for (int k = 0; k < N; k++) a[k] = i == k ? a[k] + d : a[k];
-}
-template <int N, class T>
-__device__ __forceinline__ void reg_set(T (&a)[N], int i, T v) {
-#pragma unroll
-  for (int k = 0; k < N; k++) a[k] = i == k ? v : a[k];
-}
-
-// Velocity q's components, weight and opposite for a q that varies across
-// the lanes (a lane walks its own link slots), from bit masks of the table.
-template <int Q>
-struct DirMasks {
-  static constexpr uint32_t where(const int (&c)[Q], int v) {
-    uint32_t m = 0;
-    for (int q = 0; q < Q; q++)
-      if (c[q] == v) m |= 1u << q;
-    return m;
-  }
-  static constexpr uint32_t cls(int k) {
-    uint32_t m = 0;
-    for (int q = 0; q < Q; q++)
-      if (Lattice<Q>::wclass[q] == k) m |= 1u << q;
-    return m;
-  }
-  static constexpr uint32_t pos[3] = {where(Lattice<Q>::cx, 1), where(Lattice<Q>::cy, 1), where(Lattice<Q>::cz, 1)};
-  static constexpr uint32_t neg[3] = {where(Lattice<Q>::cx, -1), where(Lattice<Q>::cy, -1), where(Lattice<Q>::cz, -1)};
-  static constexpr uint32_t zero[3] = {where(Lattice<Q>::cx, 0), where(Lattice<Q>::cy, 0), where(Lattice<Q>::cz, 0)};
-  static constexpr uint32_t cls1 = cls(1), cls2 = cls(2);
+// The tail of reverse tick k in one block (adjoint_tail_kernel): the
+// fixed-order sums of the near kernel's per-block partials, the particles'
+// adjoint state at the start of tick k (as AdjFinishArgs describes), and,
+// when tick k - 1 lies in the same segment, the reverse of tick k - 1's move
+// (as AdjMoveArgs describes).
+struct AdjTailArgs {
+  const double *partial;  // [block][np][kAdjSums] of the near kernel
+  int32_t nblocks;
+  int32_t do_move;
+  AdjFinishArgs fin;
+  AdjMoveArgs mv;         // (mv.mu unused: the state comes from fin, in LDS)
 };
-template <int Q>
-__device__ __forceinline__ int dir_c(int k, int q) {
-  using M = DirMasks<Q>;
-  return (int)((M::pos[k] >> q) & 1u) - (int)((M::neg[k] >> q) & 1u);
-}
-template <int Q>
-__device__ __forceinline__ double dir_w(int q) {
-  using M = DirMasks<Q>;
-  using L = Lattice<Q>;
-  return ((M::cls1 >> q) & 1u) ? L::wc(1) : (((M::cls2 >> q) & 1u) ? L::wc(2) : L::wc(3));
-}
-// the index of -c_q: the one velocity whose components are those of c_q negated
-template <int Q>
-__device__ __forceinline__ int dir_opp(int q) {
-  using M = DirMasks<Q>;
-  uint32_t m = ~0u;
+constexpr int kAdjThreads = 256;  // threads per block of the bulk and near kernels
+constexpr int kAdjTailThreads = 64 * kAdjParticles;  // one wave per particle's move
+constexpr int kNearSlots = 1024;  // link slots a near block stages in LDS per pass
+
+// The forward collision of a fluid node (own_idx: its row-blocked offset, n:
+// its Julia index): the gathered populations f, their density rho, the force
+// Fn and the post-collision populations P.
+template <int Q, int S, int COLL, int FM>
+__device__ __forceinline__ void adj_forward(const AdjArgs &a, int64_t own_idx, int64_t n, typename Store<S>::R (&f)[Q],
+                                           typename Store<S>::R (&P)[Q], typename Store<S>::R &rho,
+                                           typename Store<S>::R (&Fn)[3]) {
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  const BoxGeom &g = a.g;
+  const T *Sk = static_cast<const T *>(a.S);
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const int c = dir_c<Q>(k, q);
-    m &= c > 0 ? M::neg[k] : (c < 0 ? M::pos[k] : M::zero[k]);
+  for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(Sk[own_idx + (int64_t)q * g.nx], q);
+  if constexpr (FM == LBM_FORCE_UNIFORM) {
+    for (int k = 0; k < 3; k++) Fn[k] = (R)a.c.F[k];
+  } else if constexpr (FM == LBM_FORCE_FIELD) {
+    const R *Ff = static_cast<const R *>(a.Ffield);
+    for (int k = 0; k < 3; k++) Fn[k] = Ff[k * g.N + n];
   }
-  return __builtin_ctz(m);
+  if constexpr (S == kF64)
+    collide_ref<Q, COLL, FM>(f, Fn, a.c, P, rho);
+  else
+    collide_dev<Q, COLL, FM>(f, Fn, a.c, P, rho);
 }
 
-// One adjoint tick: dL/dS_k from dL/dS_{k+1}, with dL/dF +=.  One thread
-// per node.  NEAR = false: the bulk, every node outside the particles' near
-// boxes (x fastest over the lattice; no particle term reaches these nodes).
-// NEAR = true: the nodes of the near boxes (block b of particle p covers
-// nodes [j 256, (j + 1) 256) of p's box, j = b - near_first[p]; a node in
-// the boxes of several particles belongs to the first), with the particle
-// terms and the per-particle sums of the block.
-template <int Q, int S, int COLL, int FM, bool NEAR>
-__global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
+// dL/dP_k of a fluid node from dL/dS_{k+1}: lambda_q(x + c_q), except where
+// that node is a wall (bounced back into (x, q̄) instead) or the stream is
+// replaced by a particle link (drop bit q).
+template <int Q, class R>
+__device__ __forceinline__ void adj_gather(const AdjArgs &a, int x, int y, int z, int64_t own_idx, uint32_t drop,
+                                          R (&lp)[Q]) {
   using L = Lattice<Q>;
-  using T = typename Store<S>::T;
+  const BoxGeom &g = a.g;
+  const R *lin = static_cast<const R *>(a.lam_in);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int xn = wrapi(x + L::cx[q], g.nx), yn = wrapi(y + L::cy[q], g.ny), zn = wrapi(z + L::cz[q], g.nz);
+    const bool wn = is_wall_node(g, xn, yn, zn);
+    R v = 0;
+    if (!wn && !((drop >> q) & 1)) v = lin[((int64_t)(yn + g.ny * zn) * Q + q) * g.nx + xn];
+    if (wn) v += lin[own_idx + (int64_t)L::opp[q] * g.nx];
+    lp[q] = v;
+  }
+}
+
+// dL/dS_k of a fluid node = the collision's VJP of lp + the Ladd terms'
+// dL/drho (rho = sum of S_k); dL/dF += the Guo term's part.
+template <int Q, int S, int COLL, int FM>
+__device__ __forceinline__ void adj_store(const AdjArgs &a, int64_t own_idx, int64_t n,
+                                         const typename Store<S>::R (&f)[Q], typename Store<S>::R rho,
+                                         const typename Store<S>::R (&Fn)[3], const typename Store<S>::R (&lp)[Q],
+                                         typename Store<S>::R grho) {
+  using L = Lattice<Q>;
+  using R = typename Store<S>::R;
+  const BoxGeom &g = a.g;
+  R *lout = static_cast<R *>(a.lam_out);
+  R u[3] = {0, 0, 0};
+  {
+    R j[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      j[0] += (R)L::cx[q] * f[q];
+      j[1] += (R)L::cy[q] * f[q];
+      j[2] += (R)L::cz[q] * f[q];
+    }
+    if (FM != LBM_FORCE_NONE)
+      for (int k = 0; k < 3; k++) j[k] += (R)(0.5 * a.c.dt) * Fn[k];
+    const R inv = rho != (R)0 ? (R)1 / rho : (R)0;
+    for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
+  }
+  R ls_[Q], gFn[3] = {0, 0, 0};
+  collide_vjp<Q, COLL, FM, R>(rho, u, Fn, a.c, lp, ls_, gFn);
+#pragma unroll
+  for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], ls_[q] + grho);
+  if constexpr (FM == LBM_FORCE_FIELD)
+    for (int k = 0; k < 3; k++) a.gF[k * g.N + n] += (double)gFn[k];
+}
+
+// One adjoint tick on the bulk: dL/dS_k from dL/dS_{k+1}, with dL/dF +=, at
+// every node outside the particles' near boxes (one thread per node, x
+// fastest over the lattice; no particle term reaches these nodes).
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs a) {
   using R = typename Store<S>::R;
   static_assert(Q == 19, "the adjoint is built for D3Q19");
   const BoxGeom &g = a.g;
-  const int tid = threadIdx.x;
+  const int64_t nn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nn >= g.N) return;
+  const int x = (int)(nn % g.nx);
+  const int64_t r = nn / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  for (int p = 0; p < a.np; p++)
+    if (in_near_box(a.parts[p], g, x, y, z)) return;
+  const int64_t row = (int64_t)y + (int64_t)g.ny * z;
+  const int64_t n = row * g.nx + x, own_idx = row * Q * g.nx + x;
+  if (is_wall_node(g, x, y, z)) {
+    R *lout = static_cast<R *>(a.lam_out);
+#pragma unroll
+    for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], (R)0);
+    return;
+  }
+  R f[Q], P[Q], rho = 0, Fn[3] = {0, 0, 0}, lp[Q];
+  adj_forward<Q, S, COLL, FM>(a, own_idx, n, f, P, rho, Fn);
+  adj_gather<Q, R>(a, x, y, z, own_idx, 0u, lp);
+  adj_store<Q, S, COLL, FM>(a, own_idx, n, f, rho, Fn, lp, (R)0);
+}
+
+// The owners' Ladd chain of one link slot (x, q), reversed.  Forward:
+// S_q(x) = v_J with v_0 = fs = P_q̄(x) and v_i = v_{i-1} + lc rho(x) (u_b^i .
+// c_q ls) over the owners i in increasing particle order (om: their bits),
+// and the momentum exchange s_i = fs + v_i (+ the deviation shift) into each
+// owner's force and torque, whose adjoint is shat_i = -coeff ls [phi_i . c +
+// tau_i . ((X - R_i) x c)].  Returns dL/dfs = lam + 2 sum shat and the slot's
+// dL/drho(x) term sum_i (lam + sum_{j >= i} shat_j) lc (u_b^i . c ls); when
+// particle `want` is an owner, adds its dL/d(B1, B2, node-velocity snapshot,
+// R) to acc.  xs: the solid end x - c_q, xc: x.
+template <int Q, int S>
+__device__ __forceinline__ void slot_chain(const AdjArgs &a, int q, const double (&xs)[3], const double (&xc)[3],
+                                          double lam, double fs, double rho, uint32_t om, int want, double &dfs,
+                                          double &drho, double (&acc)[kAdjSums]) {
+  using L = Lattice<Q>;
+  const double cq[3] = {(double)L::cx[q], (double)L::cy[q], (double)L::cz[q]};
+  const double w = L::w(q);
+  const double lc = 2.0 * w * a.c.invC2s;
+  const double shift = S == kF64 ? 0.0 : 2.0 * w * a.c.rho0;
+  auto shat = [&](int p) {
+    const DevParticle &Pp = a.parts[p];
+    const double arm[3] = {xc[0] - Pp.R[0], xc[1] - Pp.R[1], xc[2] - Pp.R[2]};
+    const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
+                          arm[0] * cq[1] - arm[1] * cq[0]};
+    const double *pt = a.phitau + p * 6;
+    return -a.coeff * a.ls *
+           (pt[0] * cq[0] + pt[1] * cq[1] + pt[2] * cq[2] + pt[3] * cr[0] + pt[4] * cr[1] + pt[5] * cr[2]);
+  };
+  double shat_sum = 0.0;
+  for (int p = 0; p < a.np; p++)
+    if ((om >> p) & 1u) shat_sum += shat(p);
+  dfs = lam + 2.0 * shat_sum;
+  drho = 0.0;
+  double pre = 0.0, v = fs;
+  for (int p = 0; p < a.np; p++) {
+    if (!((om >> p) & 1u)) continue;
+    const DevParticle &Pp = a.parts[p];
+    double ubv[3];
+    node_velocity<3>(xs, Pp, LBM_SEMANTICS_GPU, 0.0, ubv);
+    const double udc = (ubv[0] * cq[0] + ubv[1] * cq[1] + ubv[2] * cq[2]) * a.ls;
+    const double aj = lam + shat_sum - pre;
+    drho += aj * lc * udc;
+    v = v + lc * rho * udc;
+    if (p == want) {
+      // u_b of p enters through lc rho (u_b . c ls)
+      const double ubar[3] = {aj * lc * rho * cq[0] * a.ls, aj * lc * rho * cq[1] * a.ls,
+                              aj * lc * rho * cq[2] * a.ls};
+      node_velocity_vjp(xs, Pp, ubar, acc);
+      // torque arm: d/dR of -coeff ls tau . ((X - R) x c) s = coeff ls s (c x tau)
+      const double s_p = fs + v + shift;
+      const double *tp = a.phitau + p * 6 + 3;
+      const double cxt[3] = {cq[1] * tp[2] - cq[2] * tp[1], cq[2] * tp[0] - cq[0] * tp[2],
+                             cq[0] * tp[1] - cq[1] * tp[0]};
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc[kAdjR + k] += a.coeff * a.ls * s_p * cxt[k];
+    }
+    pre += shat(p);
+  }
+}
+
+// Adds acc (per lane) over the block into tot, in a fixed order: the lanes of
+// each wave (DPP), then the waves in order.
+__device__ __forceinline__ void adj_block_sum(const double (&acc)[kAdjSums], DD (&red)[kAdjThreads / 64][kAdjSums],
+                                              DD *tot) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
+#pragma unroll
+  for (int k = 0; k < kAdjSums; k++) {
+    const DD s = wave_sum(DD{acc[k], 0.0});
+    if (lane == kWaveSumLane) red[wave][k] = s;
+  }
+  __syncthreads();
+  if (tid < kAdjSums) {
+    DD s = tot[tid];
+    for (int w = 0; w < kAdjThreads / 64; w++) s = dd_merge(s, red[w][tid]);
+    tot[tid] = s;
+  }
+  __syncthreads();
+}
+
+// One adjoint tick on the particles' near boxes (block b of particle p
+// covers box indices [j 256, (j + 1) 256) of p's near box, j = b -
+// near_first[p], x fastest; a node in the boxes of several particles belongs
+// to the first).  A lane per node gathers, finds the node's link slots and
+// the streams links replace, and stages its slots in LDS (node order, each
+// node's in increasing q); a lane per slot then runs the owners' Ladd chain
+// backwards (slot_chain), and the node lanes add the results in the order
+// they staged them before the collision's VJP.  The per-particle sums of the
+// block go to partial[b][p][...].
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
+  using L = Lattice<Q>;
+  using R = typename Store<S>::R;
+  static_assert(Q == 19, "the adjoint is built for D3Q19");
+  const BoxGeom &g = a.g;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
+  const int b = blockIdx.x;
+  int pb = 0;
+  while (pb + 1 < a.np && b >= a.near_first[pb + 1]) pb++;
+  int lo[3], hi[3];
+  near_box(a.parts[pb], g, lo, hi);
+  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+  const int64_t i0 = (int64_t)(b - a.near_first[pb]) * kAdjThreads;
+  auto node_of = [&](int t, int &x, int &y, int &z) {
+    const int64_t i = i0 + t;
+    x = lo[0] + (int)(i % ex);
+    y = lo[1] + (int)((i / ex) % ey);
+    z = lo[2] + (int)(i / ((int64_t)ex * ey));
+  };
   int x = 0, y = 0, z = 0;
-  bool live;
-  if constexpr (!NEAR) {
-    const int64_t nn = (int64_t)blockIdx.x * blockDim.x + tid;
-    live = nn < g.N;
-    if (live) {
-      x = (int)(nn % g.nx);
-      const int64_t r = nn / g.nx;
-      y = (int)(r % g.ny);
-      z = (int)(r / g.ny);
-      for (int p = 0; p < a.np; p++)
-        if (in_near_box(a.parts[p], g, x, y, z)) live = false;
-    }
-    if (!live) return;  // (no block-wide step follows in the bulk kernel)
-  } else {
-    const int b = blockIdx.x;
-    int pb = 0;
-    while (pb + 1 < a.np && b >= a.near_first[pb + 1]) pb++;
-    int lo[3], hi[3];
-    near_box(a.parts[pb], g, lo, hi);
-    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
-    const int64_t i = (int64_t)(b - a.near_first[pb]) * blockDim.x + tid;
-    live = i < (int64_t)ex * ey * ez;
-    if (live) {
-      x = lo[0] + (int)(i % ex);
-      y = lo[1] + (int)((i / ex) % ey);
-      z = lo[2] + (int)(i / ((int64_t)ex * ey));
-      for (int p = 0; p < pb; p++)
-        if (in_near_box(a.parts[p], g, x, y, z)) live = false;
-    }
+  bool live = i0 + tid < (int64_t)ex * ey * ez;
+  if (live) {
+    node_of(tid, x, y, z);
+    for (int p = 0; p < pb; p++)
+      if (in_near_box(a.parts[p], g, x, y, z)) live = false;
   }
   const int64_t row = (int64_t)y + (int64_t)g.ny * z;
-  const int64_t n = row * g.nx + x;
-  const T *Sk = static_cast<const T *>(a.S);
+  const int64_t n = row * g.nx + x, own_idx = row * Q * g.nx + x;
+  const bool fluid = live && !is_wall_node(g, x, y, z);
   const R *lin = static_cast<const R *>(a.lam_in);
-  R *lout = static_cast<R *>(a.lam_out);
-  const int64_t own_idx = row * Q * g.nx + x;
-  const bool wall = live && is_wall_node(g, x, y, z);
 
-  // neighbour x + c_q (periodic), its row offset and whether it is a wall node
-  auto nb = [&](int q, int &xn, int &yn, int &zn) {
-    xn = wrapi(x + L::cx[q], g.nx);
-    yn = wrapi(y + L::cy[q], g.ny);
-    zn = wrapi(z + L::cz[q], g.nz);
-  };
+  R f[Q], P[Q], rho = 0, Fn[3] = {0, 0, 0};
+  if (fluid) adj_forward<Q, S, COLL, FM>(a, own_idx, n, f, P, rho, Fn);
 
-  // forward collision of this node: rho, u and P_k (the link values need it)
-  R f[Q], P[Q], rho = 0;
-  R Fn[3] = {0, 0, 0};
-  if (live && !wall) {
+  // link geometry, the forward box blocks' predicates: (x, q) is a link slot
+  // of p when x lies in p's link box, r^2 <= d2(x) < (r + 1.5 dx)^2 and
+  // d2(x - c_q) < r^2.  own[p]: link slots (x, q) of p; drop: directions q
+  // whose stream x -> x + c_q a link of some particle replaces
+  uint32_t own[kAdjParticles];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(Sk[own_idx + (int64_t)q * g.nx], q);
-    if constexpr (FM == LBM_FORCE_UNIFORM) {
-      for (int k = 0; k < 3; k++) Fn[k] = (R)a.c.F[k];
-    } else if constexpr (FM == LBM_FORCE_FIELD) {
-      const R *Ff = static_cast<const R *>(a.Ffield);
-      for (int k = 0; k < 3; k++) Fn[k] = Ff[k * g.N + n];
-    }
-    if constexpr (S == kF64)
-      collide_ref<Q, COLL, FM>(f, Fn, a.c, P, rho);
-    else
-      collide_dev<Q, COLL, FM>(f, Fn, a.c, P, rho);
-  }
-
-  // the coordinates of the node and of its neighbours (x - 1 .. x + 1 along
-  // each axis, periodic), loaded together (near kernel)
-  double cxs[3] = {0.0, 0.0, 0.0}, cys[3] = {0.0, 0.0, 0.0}, czs[3] = {0.0, 0.0, 0.0};
-  if (NEAR && live) {
+  for (int p = 0; p < kAdjParticles; p++) own[p] = 0;
+  uint32_t own_any = 0, drop = 0;
+  if (fluid) {
+    double cxs[3], cys[3], czs[3];  // coordinates at x - 1 .. x + 1 along each axis (periodic)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       cxs[k] = a.coords[0][wrapi(x + k - 1, g.nx)];
       cys[k] = a.coords[1][wrapi(y + k - 1, g.ny)];
       czs[k] = a.coords[2][wrapi(z + k - 1, g.nz)];
     }
-  }
-  // (X + (-R))_1^2 + ... at the neighbour x + (ox, oy, oz), getSphere's order
-  auto d2_at = [&](const DevParticle &Pp, int ox, int oy, int oz) {
+    // (X + (-R))_1^2 + ... at the neighbour x + (ox, oy, oz), getSphere's order
+    auto d2_at = [&](const DevParticle &Pp, int ox, int oy, int oz) {
 #pragma clang fp contract(off)
-    const double u = cxs[ox + 1] + (-Pp.Rg[0]), v = cys[oy + 1] + (-Pp.Rg[1]), w = czs[oz + 1] + (-Pp.Rg[2]);
-    const double su = u * u, sv = v * v, sw = w * w;
-    return (su + sv) + sw;
-  };
-
-  // link geometry near the particles, the forward box blocks' predicates:
-  // (x, q) is a link slot of p when x lies in p's link box, r^2 <= d2(x) <
-  // (r + 1.5 dx)^2 and d2(x - c_q) < r^2.  own[p] = link slots (x, q) of p,
-  // drop = directions q whose stream x -> x + c_q a link of some particle replaces
-  uint32_t own[LBM_MAX_PARTICLES];
-  uint32_t own_any = 0, drop = 0;
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++) own[p] = 0;
-  if (NEAR && live && !wall) {
-    for (int p = 0; p < a.np; p++) {
+      const double u = cxs[ox + 1] + (-Pp.Rg[0]), v = cys[oy + 1] + (-Pp.Rg[1]), w = czs[oz + 1] + (-Pp.Rg[2]);
+      const double su = u * u, sv = v * v, sw = w * w;
+      return (su + sv) + sw;
+    };
+#pragma unroll
+    for (int p = 0; p < kAdjParticles; p++) {
+      if (p >= a.np) break;
       const DevParticle &Pp = a.parts[p];
       if (x < Pp.lo[0] - 1 || x > Pp.hi[0] + 1 || y < Pp.lo[1] - 1 || y > Pp.hi[1] + 1 || z < Pp.lo[2] - 1 ||
           z > Pp.hi[2] + 1)
@@ -373,226 +470,142 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs a) {
       uint32_t op = 0;
 #pragma unroll
       for (int q = 1; q < Q; q++) {
-        int xn, yn, zn;
-        nb(q, xn, yn, zn);
+        const int xn = wrapi(x + L::cx[q], g.nx), yn = wrapi(y + L::cy[q], g.ny), zn = wrapi(z + L::cz[q], g.nz);
         const double dn = d2_at(Pp, L::cx[q], L::cy[q], L::cz[q]);
         // (x, q̄) is a link slot: x - c_q̄ = x + c_q solid
         if (fl0 && dn < r2) op |= 1u << L::opp[q];
         // (x + c_q, q) is a link slot: x + c_q fluid within the shell, x solid
         if (d0 < r2 && in_box(Pp, xn, yn, zn) && dn >= r2 && dn < ro * ro) drop |= 1u << q;
       }
-      reg_set(own, p, op);
+      own[p] = op;
       own_any |= op;
     }
   }
-
-  // with a single particle, its per-particle sums (near kernel)
-  double acc1[kAdjSums];
-#pragma unroll
-  for (int k = 0; k < kAdjSums; k++) acc1[k] = 0.0;
-
-  // dL/dP_k at this node
   R lp[Q];
   R grho = 0;
-  const double coords_x[3] = {cxs[1], cys[1], czs[1]};
-  // dL/dS_{k+1} of this node's own populations, which its link slots read
-  R lown[Q];
+  if (fluid) adj_gather<Q, R>(a, x, y, z, own_idx, drop, lp);
+
+  // this node's first slot in the block's list: an exclusive scan of the
+  // slot counts over the lanes, then the waves
+  __shared__ int wtot[kAdjThreads / 64];
+  __shared__ uint32_t s_key[kNearSlots];  // node (bits 0-7), q (8-12), owners (16-23)
+  __shared__ R s_a[kNearSlots], s_b[kNearSlots];  // lam and fs in; dL/dfs and the dL/drho term out
+  __shared__ R s_rho[kAdjThreads];
+  __shared__ DD red[kAdjThreads / 64][kAdjSums];
+  __shared__ DD tot[kAdjParticles][kAdjSums];
+  const int cnt = __popc(own_any);
+  int incl = cnt;
 #pragma unroll
-  for (int q = 0; q < Q; q++) lown[q] = 0;
-  if (NEAR && own_any)
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  if (lane == 63) wtot[wave] = incl;
+  s_rho[tid] = rho;
+  if (tid < a.np * kAdjSums) tot[tid / kAdjSums][tid % kAdjSums] = DD{0.0, 0.0};
+  __syncthreads();
+  int first = incl - cnt, total = 0;
 #pragma unroll
-    for (int q = 1; q < Q; q++)
-      if ((own_any >> q) & 1) lown[q] = lin[own_idx + (int64_t)q * g.nx];
-  if (live && !wall) {
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      int xn, yn, zn;
-      nb(q, xn, yn, zn);
-      const bool wn = is_wall_node(g, xn, yn, zn);
-      R v = 0;
-      // streamed to x + c_q, unless that node is a wall or the slot is a link
-      if (!wn && !((drop >> q) & 1))
-        v = lin[((int64_t)(yn + g.ny * zn) * Q + q) * g.nx + xn];
-      // bounced back into (x, q̄) off a wall layer
-      if (wn) v += lin[own_idx + (int64_t)L::opp[q] * g.nx];
-      lp[q] = v;
-    }
-    // particle link slots (x, q'): S_q'(x) = P_q̄'(x) + the owners' Ladd
-    // terms; each lane walks its own slots in increasing q' (with a single
-    // particle, its per-particle sums are taken in the same walk)
-    uint32_t slots = NEAR ? own_any : 0u;
-    while (slots) {
-      const int qq = __builtin_ctz(slots);
-      slots &= slots - 1;
-      const int ob = dir_opp<Q>(qq);
-      const int c0 = dir_c<Q>(0, qq), c1 = dir_c<Q>(1, qq), c2 = dir_c<Q>(2, qq);
-      const double lam = (double)reg_get(lown, qq);
-      const double cq[3] = {(double)c0, (double)c1, (double)c2};
-      const double xs[3] = {reg_get(cxs, 1 - c0), reg_get(cys, 1 - c1), reg_get(czs, 1 - c2)};
-      const double lc = 2.0 * dir_w<Q>(qq) * a.c.invC2s;
-      // owners in particle order: u_b . c_q ls and shat_j of each
-      double udc_o[LBM_MAX_PARTICLES], sh_o[LBM_MAX_PARTICLES];
-      double shat_sum = 0.0;
-      for (int p = 0; p < a.np; p++) {
-        if (!((reg_get(own, p) >> qq) & 1)) continue;
-        const DevParticle &Pp = a.parts[p];
-        double ubv[3];
-        node_velocity<3>(xs, Pp, LBM_SEMANTICS_GPU, 0.0, ubv);
-        const double udc = (ubv[0] * cq[0] + ubv[1] * cq[1] + ubv[2] * cq[2]) * a.ls;
-        const double arm[3] = {coords_x[0] - Pp.R[0], coords_x[1] - Pp.R[1], coords_x[2] - Pp.R[2]};
-        const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
-                              arm[0] * cq[1] - arm[1] * cq[0]};
-        const double *pt = a.phitau + p * 6;
-        const double sh = -a.coeff * a.ls *
-                          (pt[0] * cq[0] + pt[1] * cq[1] + pt[2] * cq[2] + pt[3] * cr[0] + pt[4] * cr[1] + pt[5] * cr[2]);
-        reg_set(udc_o, p, udc);
-        reg_set(sh_o, p, sh);
-        shat_sum += sh;
-        if (a.np == 1) {
-          // the sums below for particle 0 (the only owner): v = fs + its
-          // Ladd term, s = fs + v (+ the deviation shift), dL/dv = lam + sh
-          const double fs = (double)reg_get(P, ob);
-          const double shift = S == kF64 ? 0.0 : 2.0 * dir_w<Q>(qq) * a.c.rho0;
-          const double v = fs + lc * (double)rho * udc;
-          const double s_p = fs + v + shift;
-          const double aj = lam + sh - 0.0;
-          const double ubar[3] = {aj * lc * (double)rho * cq[0] * a.ls, aj * lc * (double)rho * cq[1] * a.ls,
-                                  aj * lc * (double)rho * cq[2] * a.ls};
-          node_velocity_vjp(xs, Pp, ubar, acc1);
-          const double *tp = pt + 3;
-          const double cxt[3] = {cq[1] * tp[2] - cq[2] * tp[1], cq[2] * tp[0] - cq[0] * tp[2],
-                                 cq[0] * tp[1] - cq[1] * tp[0]};
-#pragma unroll
-          for (int k = 0; k < 3; k++) acc1[kAdjR + k] += a.coeff * a.ls * s_p * cxt[k];
-        }
-      }
-      // dL/dfs = lam + 2 sum shat; dL/dv_j = lam + sum_{i >= j} shat_i
-      reg_add(lp, ob, (R)(lam + 2.0 * shat_sum));
-      double pre = 0.0;
-      for (int p = 0; p < a.np; p++) {
-        if (!((reg_get(own, p) >> qq) & 1)) continue;
-        const double aj = lam + shat_sum - pre;
-        grho += (R)(aj * lc * reg_get(udc_o, p));
-        pre += reg_get(sh_o, p);
-      }
-    }
+  for (int w = 0; w < kAdjThreads / 64; w++) {
+    if (w < wave) first += wtot[w];
+    total += wtot[w];
   }
 
-  // dL/dS_k = collision VJP + the Ladd terms' rho (rho = sum of S_k)
+  for (int pass0 = 0; pass0 < total; pass0 += kNearSlots) {
+    if (own_any) {
+      int s = first - pass0;
+#pragma unroll
+      for (int q = 1; q < Q; q++) {
+        if (!((own_any >> q) & 1u)) continue;
+        if (s >= 0 && s < kNearSlots) {
+          uint32_t om = 0;
+#pragma unroll
+          for (int p = 0; p < kAdjParticles; p++) om |= ((own[p] >> q) & 1u) << p;
+          s_key[s] = (uint32_t)tid | (uint32_t)q << 8 | om << 16;
+          s_a[s] = lin[own_idx + (int64_t)q * g.nx];
+          s_b[s] = P[L::opp[q]];
+        }
+        s++;
+      }
+    }
+    __syncthreads();
+    const int ns = total - pass0 < kNearSlots ? total - pass0 : kNearSlots;
+    // a lane per slot; slot j's node and solid end
+    auto slot_at = [&](int j, int &q, uint32_t &om, double (&xs)[3], double (&xc)[3]) {
+      const uint32_t key = s_key[j];
+      q = (int)((key >> 8) & 31u);
+      om = key >> 16;
+      int sx, sy, sz;
+      node_of((int)(key & 255u), sx, sy, sz);
+      xc[0] = a.coords[0][sx];
+      xc[1] = a.coords[1][sy];
+      xc[2] = a.coords[2][sz];
+      xs[0] = a.coords[0][wrapi(sx - L::cx[q], g.nx)];
+      xs[1] = a.coords[1][wrapi(sy - L::cy[q], g.ny)];
+      xs[2] = a.coords[2][wrapi(sz - L::cz[q], g.nz)];
+      return (int)(key & 255u);
+    };
+    double acc[kAdjSums];
+    if (a.np > 1) {
+      // each particle's sums, from the slots it owns (inputs still in s_a, s_b)
+      for (int pp = 0; pp < a.np; pp++) {
+#pragma unroll
+        for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
+        for (int j = tid; j < ns; j += kAdjThreads) {
+          int q;
+          uint32_t om;
+          double xs[3], xc[3];
+          const int nd = slot_at(j, q, om, xs, xc);
+          if (!((om >> pp) & 1u)) continue;
+          double dfs, drho;
+          slot_chain<Q, S>(a, q, xs, xc, (double)s_a[j], (double)s_b[j], (double)s_rho[nd], om, pp, dfs, drho, acc);
+        }
+        adj_block_sum(acc, red, tot[pp]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
+    for (int j = tid; j < ns; j += kAdjThreads) {
+      int q;
+      uint32_t om;
+      double xs[3], xc[3];
+      const int nd = slot_at(j, q, om, xs, xc);
+      double dfs, drho;
+      slot_chain<Q, S>(a, q, xs, xc, (double)s_a[j], (double)s_b[j], (double)s_rho[nd], om, a.np == 1 ? 0 : -1,
+                       dfs, drho, acc);
+      s_a[j] = (R)dfs;
+      s_b[j] = (R)drho;
+    }
+    if (a.np == 1) adj_block_sum(acc, red, tot[0]);
+    else __syncthreads();
+    // the node lanes take their slots' results in the order they staged them
+    if (own_any) {
+      int s = first - pass0;
+#pragma unroll
+      for (int q = 1; q < Q; q++) {
+        if (!((own_any >> q) & 1u)) continue;
+        if (s >= 0 && s < kNearSlots) {
+          lp[L::opp[q]] += s_a[s];
+          grho += s_b[s];
+        }
+        s++;
+      }
+    }
+    __syncthreads();
+  }
+
   if (live) {
-    if (wall) {
+    if (!fluid) {
+      R *lout = static_cast<R *>(a.lam_out);
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], (R)0);
     } else {
-      R u[3] = {0, 0, 0};
-      {
-        R j[3] = {0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-          j[0] += (R)L::cx[q] * f[q];
-          j[1] += (R)L::cy[q] * f[q];
-          j[2] += (R)L::cz[q] * f[q];
-        }
-        if (FM != LBM_FORCE_NONE)
-          for (int k = 0; k < 3; k++) j[k] += (R)(0.5 * a.c.dt) * Fn[k];
-        const R inv = rho != (R)0 ? (R)1 / rho : (R)0;
-        for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
-      }
-      R ls_[Q], gFn[3] = {0, 0, 0};
-      collide_vjp<Q, COLL, FM, R>(rho, u, Fn, a.c, lp, ls_, gFn);
-#pragma unroll
-      for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], ls_[q] + grho);
-      if constexpr (FM == LBM_FORCE_FIELD)
-        for (int k = 0; k < 3; k++) a.gF[k * g.N + n] += (double)gFn[k];
+      adj_store<Q, S, COLL, FM>(a, own_idx, n, f, rho, Fn, lp, grho);
     }
   }
-
-  if constexpr (!NEAR) return;
-
-  // per-particle sums over the block (only blocks that hold link slots)
-  __shared__ int any_link;
-  __shared__ DD red[4][kAdjSums];
-  if (tid == 0) any_link = 0;
-  __syncthreads();
-  if (own_any) any_link = 1;
-  __syncthreads();
-  const int wave = tid >> 6, lane = __lane_id();
-  for (int p = 0; p < a.np; p++) {
-    double acc[kAdjSums];
-#pragma unroll
-    for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
-    const uint32_t own_p = reg_get(own, p);
-    if (a.np == 1) {
-#pragma unroll
-      for (int k = 0; k < kAdjSums; k++) acc[k] = acc1[k];
-    } else if (any_link && own_p != 0) {
-      const DevParticle &Pp = a.parts[p];
-      uint32_t slots = own_p;
-      while (slots) {
-        const int qq = __builtin_ctz(slots);
-        slots &= slots - 1;
-        const int ob = dir_opp<Q>(qq);
-        const int c0 = dir_c<Q>(0, qq), c1 = dir_c<Q>(1, qq), c2 = dir_c<Q>(2, qq);
-        const double lam = (double)reg_get(lown, qq);
-        const double cq[3] = {(double)c0, (double)c1, (double)c2};
-        const double xs[3] = {reg_get(cxs, 1 - c0), reg_get(cys, 1 - c1), reg_get(czs, 1 - c2)};
-        const double lc = 2.0 * dir_w<Q>(qq) * a.c.invC2s;
-        const double shift = S == kF64 ? 0.0 : 2.0 * dir_w<Q>(qq) * a.c.rho0;
-        const double fs = (double)reg_get(P, ob);
-        // the owner chain up to and after p
-        double v = fs, shat_sum = 0.0, pre = 0.0, s_p = 0.0;
-        for (int i = 0; i < a.np; i++) {
-          if (!((reg_get(own, i) >> qq) & 1)) continue;
-          const DevParticle &Pi = a.parts[i];
-          double ubv[3];
-          node_velocity<3>(xs, Pi, LBM_SEMANTICS_GPU, 0.0, ubv);
-          const double udc = (ubv[0] * cq[0] + ubv[1] * cq[1] + ubv[2] * cq[2]) * a.ls;
-          v = v + lc * (double)rho * udc;
-          const double arm[3] = {coords_x[0] - Pi.R[0], coords_x[1] - Pi.R[1], coords_x[2] - Pi.R[2]};
-          const double cr[3] = {arm[1] * cq[2] - arm[2] * cq[1], arm[2] * cq[0] - arm[0] * cq[2],
-                                arm[0] * cq[1] - arm[1] * cq[0]};
-          const double sh = -a.coeff * a.ls *
-                            (a.phitau[i * 6 + 0] * cq[0] + a.phitau[i * 6 + 1] * cq[1] + a.phitau[i * 6 + 2] * cq[2] + a.phitau[i * 6 + 3] * cr[0] +
-                             a.phitau[i * 6 + 4] * cr[1] + a.phitau[i * 6 + 5] * cr[2]);
-          if (i < p) pre += sh;
-          if (i == p) s_p = fs + v + shift;
-          shat_sum += sh;
-        }
-        const double aj = lam + shat_sum - pre;
-        // u_b of p enters through lc rho (u_b . c ls)
-        const double ubar[3] = {aj * lc * (double)rho * cq[0] * a.ls, aj * lc * (double)rho * cq[1] * a.ls,
-                                aj * lc * (double)rho * cq[2] * a.ls};
-        node_velocity_vjp(xs, Pp, ubar, acc);
-        // torque arm: d/dR of -coeff ls tau . ((X - R) x c) s = coeff ls s (c x tau)
-        const double *tp = a.phitau + p * 6 + 3;
-        const double cxt[3] = {cq[1] * tp[2] - cq[2] * tp[1], cq[2] * tp[0] - cq[0] * tp[2],
-                               cq[0] * tp[1] - cq[1] * tp[0]};
-        for (int k = 0; k < 3; k++) acc[kAdjR + k] += a.coeff * a.ls * s_p * cxt[k];
-      }
-    }
-    double *dst = a.partial + ((int64_t)blockIdx.x * a.np + p) * kAdjSums;
-    if (!any_link) {
-      if (tid < kAdjSums) dst[tid] = 0.0;
-      continue;
-    }
-    if (__any(own_p != 0)) {
-#pragma unroll
-      for (int k = 0; k < kAdjSums; k++) {
-        const DD s = wave_sum(DD{acc[k], 0.0});
-        if (lane == kWaveSumLane) red[wave][k] = s;
-      }
-    } else if (lane == kWaveSumLane) {
-      // (a wave without a link slot of p adds zeros)
-#pragma unroll
-      for (int k = 0; k < kAdjSums; k++) red[wave][k] = DD{0.0, 0.0};
-    }
-    __syncthreads();
-    if (tid < kAdjSums) {
-      DD s = red[0][tid];
-      for (int w = 1; w < (int)(blockDim.x >> 6); w++) s = dd_merge(s, red[w][tid]);
-      dst[tid] = s.hi + s.lo;
-    }
-    __syncthreads();
+  if (tid < a.np * kAdjSums) {
+    const DD t = tot[tid / kAdjSums][tid % kAdjSums];
+    a.partial[(int64_t)b * a.np * kAdjSums + tid] = t.hi + t.lo;
   }
 }
 

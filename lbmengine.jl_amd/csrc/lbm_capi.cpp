@@ -2223,8 +2223,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   if (h->cfg.semantics != LBM_SEMANTICS_GPU) return fail(LBM_EINVAL, "the adjoint follows the GPU semantics");
   if (!h->populated) return fail(LBM_ESTATE, "populations not initialised");
   if (!h->inter_h.empty()) return fail(LBM_EINVAL, "the adjoint does not differentiate particle interactions");
-  if (h->parts_h.size() > (size_t)kLocalParticles)
-    return fail(LBM_EINVAL, "the adjoint takes at most " + std::to_string(kLocalParticles) + " particles");
+  if (h->parts_h.size() > (size_t)kAdjParticles)
+    return fail(LBM_EINVAL, "the adjoint takes at most " + std::to_string(kAdjParticles) + " particles");
   const int np = (int)h->parts_h.size();
   const int64_t T = cfg->ticks, K = cfg->checkpoint_every, Wn = cfg->window;
   if (T < 1 || K < 1 || Wn < 1 || Wn > T) return fail(LBM_EINVAL, "need ticks >= 1, checkpoint_every >= 1, 1 <= window <= ticks");
@@ -2234,18 +2234,18 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   const size_t N = (size_t)h->N;
   const size_t pop_bytes = N * h->Q * elem_size(h->cfg.storage);
   const size_t rsize = h->cfg.storage == LBM_STORAGE_F64 ? 8 : 4;
-  const int threads = 256;
+  const int threads = kAdjThreads;
   const int64_t nblocks = ((int64_t)N + threads - 1) / threads;  // the bulk kernel's
   // the near kernel's blocks: particle p's near box (link box and one node)
   // spans at most 2 r / dx + 9 nodes an axis
-  int32_t near_first[LBM_MAX_PARTICLES + 1] = {0};
-  int32_t near_ext[LBM_MAX_PARTICLES] = {0};
+  int32_t near_first[kAdjParticles + 1] = {0};
+  int32_t near_ext[kAdjParticles] = {0};
   for (int p = 0; p < np; p++) {
     const int64_t e = (int64_t)std::ceil(2.0 * h->parts_h[p].radius / h->cfg.dx) + 10;
     near_ext[p] = (int32_t)e;
     near_first[p + 1] = near_first[p] + (int32_t)((e * e * e + threads - 1) / threads);
   }
-  for (int p = np; p < LBM_MAX_PARTICLES; p++) near_first[p + 1] = near_first[p];
+  for (int p = np; p < kAdjParticles; p++) near_first[p + 1] = near_first[p];
   const int64_t nblocks_near = near_first[np];
   const int64_t nck = (T + K - 1) / K, seg_len = std::min(K, T);
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
@@ -2253,7 +2253,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   AdjBuffers buf;
   char *ck = nullptr, *seg = nullptr, *fin = nullptr;
   void *lam[2] = {nullptr, nullptr};
-  double *gF = nullptr, *partial = nullptr, *sums_d = nullptr;
+  double *gF = nullptr, *partial = nullptr;
   DevParticle *recs_seg = nullptr;  // the records of every tick of a segment (two sets)
   DevParticle *ck_recs_d = nullptr;  // the checkpoints' records
   if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess ||
@@ -2261,7 +2261,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
       buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
       buf.alloc(&partial, (size_t)std::max<int64_t>(nblocks_near, 1) * np * kAdjSums * sizeof(double)) != hipSuccess ||
-      buf.alloc(&sums_d, (size_t)np * kAdjSums * sizeof(double)) != hipSuccess ||
       buf.alloc(&recs_seg, 2 * (size_t)(seg_len + 1) * np * sizeof(DevParticle)) != hipSuccess ||
       buf.alloc(&ck_recs_d, (size_t)nck * np * sizeof(DevParticle)) != hipSuccess)
     return fail(LBM_ENOMEM, "hipMalloc adjoint buffers (checkpoints, segment states, adjoint fields)");
@@ -2400,15 +2399,14 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.ls = h->cfg.dx / h->cfg.dt;
   aa.shell = 1.5 * h->cfg.dx;
   aa.phitau = pt_d;
-  for (int p = 0; p <= LBM_MAX_PARTICLES; p++) aa.near_first[p] = near_first[p];
-  for (int p = 0; p < LBM_MAX_PARTICLES; p++) aa.near_ext[p] = near_ext[p];
+  for (int p = 0; p <= kAdjParticles; p++) aa.near_first[p] = near_first[p];
+  for (int p = 0; p < kAdjParticles; p++) aa.near_ext[p] = near_ext[p];
   AdjMoveArgs ma{};
   ma.mu = mu_d;
   ma.mu_in = muin_d;
   ma.phitau = pt_d;
   ma.dt = h->cfg.dt;
   AdjFinishArgs fa2{};
-  fa2.sums = sums_d;
   fa2.mu_in = muin_d;
   fa2.mu = mu_d;
   fa2.gB = gB_d;
@@ -2420,37 +2418,35 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   // the two share only the read-only checkpoints.  The recomputation uses
   // the lattice's own working state (populations' spare buffer, records,
   // partials, rho), which the reverse sweep does not touch.
-  // Within a reverse tick the near kernel (a few latency-bound blocks) runs
-  // on a third stream beside the bulk kernel: both start after the move's
-  // reverse and write disjoint nodes; the reduction waits for both.
-  hipStream_t s_rev = h->stream, s_rec = nullptr, s_near = nullptr;
+  // A reverse tick is the near kernel, the bulk kernel and the tail, in
+  // that order on the handle's stream: a further stream for the near kernel
+  // can share a hardware queue with the recomputation's and then waits
+  // behind its kernels.
+  hipStream_t s_rev = h->stream, s_rec = nullptr;
   hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
-  constexpr int kEvents = 6;
+  constexpr int kEvents = 4;
   struct Streams {
     lbm_handle *h;
-    hipStream_t main, *extra[2];
+    hipStream_t main, *extra;
     hipEvent_t *evs;
     ~Streams() {
       h->stream = main;
-      for (hipStream_t *s : extra)
-        if (*s) {
-          hipStreamSynchronize(*s);
-          hipStreamDestroy(*s);
-        }
+      if (*extra) {
+        hipStreamSynchronize(*extra);
+        hipStreamDestroy(*extra);
+      }
       for (int i = 0; i < kEvents; i++)
         if (evs[i]) hipEventDestroy(evs[i]);
     }
   };
-  hipEvent_t evs[kEvents] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  Streams guard{h, s_rev, {&s_rec, &s_near}, evs};
+  hipEvent_t evs[kEvents] = {nullptr, nullptr, nullptr, nullptr};
+  Streams guard{h, s_rev, &s_rec, evs};
   HIP_TRY(hipStreamCreate(&s_rec), "create recomputation stream");
-  HIP_TRY(hipStreamCreate(&s_near), "create near-kernel stream");
   for (int i = 0; i < kEvents; i++) HIP_TRY(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming), "create event");
   ev_rec[0] = evs[0];
   ev_rec[1] = evs[1];
   ev_rev[0] = evs[2];
   ev_rev[1] = evs[3];
-  hipEvent_t ev_moved = evs[4], ev_near = evs[5];
   HIP_TRY(hipEventRecord(ev_rev[0], s_rev), "event");  // (both sets free)
   HIP_TRY(hipEventRecord(ev_rev[1], s_rev), "event");
   // S_k of segment c's ticks: the checkpoint itself, then slots 1.. of its
@@ -2490,7 +2486,11 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     return LBM_OK;
   };
   const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
-  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_near};
+  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_rev};
+  AdjTailArgs ta{};
+  ta.partial = partial;
+  ta.nblocks = (int32_t)nblocks_near;
+  ta.mv = ma;
   if (const int rc = recompute(nck - 1)) return rc;
   for (int64_t c = nck - 1; c >= 0; c--) {
     const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
@@ -2498,25 +2498,31 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     if (c > 0)
       if (const int rc = recompute(c - 1)) return rc;
     for (int64_t k = t1 - 1; k >= t0; k--) {
-      // the move's reverse (dL/dF, dL/dT of tick k), the adjoint tick, then
-      // the particles' adjoint state at the start of tick k
-      ma.recs_k = recs_of(c, k - t0);
-      ma.recs_k1 = recs_of(c, k - t0 + 1);
-      hipError_t e = launch_adjoint_move(ma, np, s_rev);
-      aa.parts = ma.recs_k;
+      // the move's reverse (dL/dF, dL/dT of tick k: on its own at the
+      // segment's last tick, else in the previous tail), the adjoint tick,
+      // then the tail: the particles' adjoint state at the start of tick k
+      // and the reverse of tick k - 1's move when that is in the segment
+      hipError_t e = hipSuccess;
+      if (k == t1 - 1) {
+        ma.recs_k = recs_of(c, k - t0);
+        ma.recs_k1 = recs_of(c, k - t0 + 1);
+        e = launch_adjoint_move(ma, np, s_rev);
+      }
+      aa.parts = recs_of(c, k - t0);
       aa.S = Sk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
-      if (e == hipSuccess) e = hipEventRecord(ev_moved, s_rev);
-      if (e == hipSuccess) e = hipStreamWaitEvent(s_near, ev_moved, 0);
       if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, true, aa, ln);
-      if (e == hipSuccess) e = hipEventRecord(ev_near, s_near);
       if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
-      if (e == hipSuccess) e = hipStreamWaitEvent(s_rev, ev_near, 0);
-      if (e == hipSuccess) e = launch_adjoint_reduce(partial, nblocks_near, np, sums_d, s_rev);
-      fa2.recs_k = ma.recs_k;
-      fa2.direct = k > ws;  // V_k is one of the averaged speeds
-      if (e == hipSuccess) e = launch_adjoint_finish(fa2, s_rev);
+      ta.fin = fa2;
+      ta.fin.recs_k = recs_of(c, k - t0);
+      ta.fin.direct = k > ws;  // V_k is one of the averaged speeds
+      ta.do_move = k > t0;
+      if (k > t0) {
+        ta.mv.recs_k = recs_of(c, k - 1 - t0);
+        ta.mv.recs_k1 = recs_of(c, k - t0);
+      }
+      if (e == hipSuccess) e = launch_adjoint_tail(ta, s_rev);
       if (e != hipSuccess) return hip_fail(e, "adjoint launch");
       a ^= 1;
     }

@@ -42,10 +42,9 @@ struct AdjArgs;
 // near = false: the bulk kernel (nodes outside the particles' near boxes);
 // true: the near kernel (their nodes, with the particle terms and sums)
 hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArgs &a, const Launch &l);
-hipError_t launch_adjoint_reduce(const double *partial, int64_t nblocks, int np, double *out, hipStream_t stream);
 struct AdjMoveArgs;
-struct AdjFinishArgs;
+struct AdjTailArgs;
 hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream);
-hipError_t launch_adjoint_finish(const AdjFinishArgs &a, hipStream_t stream);
+hipError_t launch_adjoint_tail(const AdjTailArgs &a, hipStream_t stream);
 
 }  // namespace lbm

@@ -117,6 +117,51 @@ def test_adjoint_against_the_dual_number_oracle(gpu, oracle):
         assert abs(ad - dl) <= 1e-8 * abs(dl), (a, b, ad, dl)
 
 
+@pytest.mark.parametrize("sep", [7.5, 6.4], ids=["gap", "overlap"])
+def test_two_particles_against_the_dual_number_oracle(gpu, oracle, sep):
+    """Two squirmers (radii 3.6 and 3.4) whose centres lie `sep` apart: 0.5
+    node apart at their closest (nodes in the gap hold link slots of both)
+    or overlapping by 0.6 (slots with two owners: the Ladd terms' owner chain
+    runs backwards through both particles).  The second swimmer's speed is
+    the loss; FP64 adjoint against the dual-number oracle along each
+    particle's B1 and B2, a force-field direction and a mixed one."""
+    from lbmengine_hip import adjoint
+    from lbmengine_hip.model import resolve_particle
+    n, T, W, target, pen = 20, 14, 6, 1e-4, 1e3
+    rng = np.random.default_rng(31)
+    F = rng.normal(0.0, 1e-6, (n, n, n, 3))
+    ax = np.array([0.99, -0.1, 0.05])
+    ax /= np.linalg.norm(ax)
+    c0 = np.array([-0.2, 0.1, 0.15])
+    specs = [dict(radius=3.6, position=c0 - 0.5 * sep * ax, swimmingSpeed=2e-3 / 3, beta=-2.0,
+                  swimmingDirection=[0.6, 0.8, 0.0]),
+             dict(radius=3.4, position=c0 + 0.5 * sep * ax, swimmingSpeed=1.5e-3 / 3, beta=3.0,
+                  swimmingDirection=[-0.48, 0.6, 0.64])]
+    m = gpu.modelInit(**KW3, xlims=(-n // 2, n // 2 - 1), storage="f64", forceDensity=np.asfortranarray(F))
+    for sp in specs:
+        gpu.addSquirmer(m, **sp, coupleForces=True, coupleTorques=True)
+    f0 = m.distributions
+    ps = [resolve_particle(m.resolved, kind="squirmer", **sp, coupleForces=True, coupleTorques=True)
+          for sp in specs]
+    g = adjoint.swimming_loss_gradient(m, ticks=T, checkpoint_every=5, window=W, particle=1, target_speed=target,
+                                       force_penalty=pen)
+    assert g.dB1.shape == (2,) and g.dB2.shape == (2,)
+    Fj = np.asfortranarray(F).reshape(-1, order="F")
+    gF = np.asfortranarray(g.dF).reshape(-1, order="F")
+    dFd = rng.standard_normal(Fj.shape) * 1e-6
+    B1 = [p.B1 for p in ps]
+    B2 = [p.B2 for p in ps]
+    dirs = ([B1[0], 0.0], [0.0, 0.0], None), ([0.0, B1[1]], [0.0, 0.0], None), ([0.0, 0.0], [B2[0], 0.0], None), \
+           ([0.0, 0.0], [0.0, B2[1]], None), ([0.0, 0.0], [0.0, 0.0], dFd), \
+           ([0.3 * B1[0], -0.5 * B1[1]], [0.2 * B2[0], 0.7 * B2[1]], dFd)
+    for a, b, dF in dirs:
+        L, dl, _ = oracle.dual_loss_jvp(m.resolved, ps, f0, ticks=T, window=W, F=Fj, dB1=a, dB2=b, dF=dF,
+                                        swimmer=1, target=target, penalty=pen)
+        ad = float(np.dot(g.dB1, a) + np.dot(g.dB2, b)) + (0.0 if dF is None else float(np.dot(gF, dF)))
+        assert abs(L - g.loss) <= 1e-12 * L
+        assert abs(ad - dl) <= 1e-8 * abs(dl), (a, b, ad, dl)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(gpu):
     from lbmengine_hip import adjoint
     F, pos, e = setup(20, 4.5, 9)
