@@ -122,9 +122,17 @@ def batch_bench(lh, args):
     nodes = sum(int(np.prod(m.resolved.n[:3])) for m in boxes)
     b = lh.LBMbatch(boxes)
     if args.batch_block:
-        v = [int(x) for x in args.batch_block.split(",")]
-        block = tuple(v[:3])
-        b.set_block(*block)
+        parts = args.batch_block.split(",")
+        if parts[0].startswith("r"):
+            # row tiles: rTHREADS[,tile order[,low occupancy]]
+            v = [int(parts[0][1:])] + [int(x) for x in parts[1:]]
+            block = (v[0], 0, 0)
+            b.set_row_tiles(v[0])
+            v = v[:1] + [0, 0] + v[1:]
+        else:
+            v = [int(x) for x in parts]
+            block = tuple(v[:3])
+            b.set_block(*block)
         if len(v) > 3:
             b.set_tile_order(v[3])
         if len(v) > 4:
@@ -148,6 +156,7 @@ def batch_bench(lh, args):
            "boxes": len(boxes), "nodes": nodes, "steps": args.batch_steps, "warmup": args.batch_warmup,
            "MLUPS": round(nodes * args.batch_steps / wall / 1e6, 1),
            "ms_per_step": round(wall / args.batch_steps * 1e3, 5), "block": list(block),
+           "row_tiles": b.get_row_tiles(),
            "tile_order": b.get_tile_order(), "low_occupancy": b.get_occupancy(),
            "kernel_ms": round(kms, 5), "step_ms_device": round(sms, 5),
            "hbm_frac": round(bytes_per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -442,7 +451,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-warmup", type=int, default=100)
     ap.add_argument("--batch-accuracy", type=int, default=1, help="0: skip the per-box FP64 comparison")
-    ap.add_argument("--batch-block", default="", help="bx,by,bz[,tile order[,low occupancy]] of the batched kernel instead of "
+    ap.add_argument("--batch-block", default="", help="bx,by,bz[,tile order[,low occupancy]] of the batched kernel, or "
+                                                      "rTHREADS[,tile order[,low occupancy]] for row tiles, instead of "
                                                       "autotuning")
     ap.add_argument("--no-configs", action="store_true", help="skip the D2Q9 and channel configurations")
     ap.add_argument("--no-adjoint", action="store_true")

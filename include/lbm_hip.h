@@ -388,13 +388,23 @@ int lbm_batch_sync(lbm_batch *b);
  * order of the tiles (the rules of lbm_set_tile_order). */
 int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz);
 int lbm_batch_get_block(lbm_batch *b, int32_t *bx, int32_t *by, int32_t *bz);
+/* Row tiles instead of one shared shape: blocks of `threads` threads (64..512,
+ * a multiple of 64), and every box its own tile shape of that many threads
+ * whose x extent holds a whole row of the box (the least divisor of
+ * `threads` of at least 16 that does, or `threads`), so that a row's x
+ * neighbours lie in lines its own tile reads.  get returns 0 while the
+ * boxes share one shape (lbm_batch_set_block). */
+int lbm_batch_set_row_tiles(lbm_batch *b, int32_t threads);
+int lbm_batch_get_row_tiles(lbm_batch *b, int32_t *threads);
 int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap);
 int lbm_batch_get_tile_order(lbm_batch *b, int32_t *xcd_remap);
 int lbm_batch_set_occupancy(lbm_batch *b, int32_t low);  /* as lbm_set_occupancy */
 int lbm_batch_get_occupancy(lbm_batch *b, int32_t *low);
 
 /* Autotuner of the batched fused kernel over the block shapes that fit every
- * box and both tile orders (the boxes' state is untouched). */
+ * box, row tiles of 64..512 threads, both tile orders and both occupancy
+ * builds (the boxes' state is untouched).  best: the block shape, or
+ * {threads, 0, 0} when row tiles are fastest. */
 int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best_ms);
 
 /* Ticks per captured graph (0 or even; default 32), timing and the averages

@@ -157,7 +157,9 @@ Tiling make_tiling(const lbm_handle *h, int bx, int by, int bz) {
   t.mty = div_magic((uint32_t)t.ty, (uint64_t)t.ntiles / (uint64_t)t.tx + 1);
   t.remap = 1;
   t.occ = 0;
-  t.reserved = 0;
+  // flat thread index t < bx by bz: t / bx, then (t / bx) / by
+  t.mbx = div_magic((uint32_t)bx, (uint64_t)bx * by * bz);
+  t.mby = div_magic((uint32_t)by, (uint64_t)by * bz);
   return t;
 }
 
@@ -1654,7 +1656,8 @@ enum BatchSection { kSecCollide = 0, kSecStep0 = 1, kSecStep1 = 2, kSecFinal0 = 
 struct lbm_batch {
   std::vector<lbm_handle *> boxes;
   int Q = 0, storage = 0, coll = 0, force = 0;
-  Tiling shape{};  // block shape and tile order of every box (bx, by, bz, remap)
+  Tiling shape{};  // block shape of every box (bx, by, bz) unless row tiles, tile order, occupancy
+  int32_t rows = 0;  // > 0: every box takes row tiles (row_tile) of this many threads
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   // argument tables, one per slot (box 0's population buffer at run start)
@@ -1679,7 +1682,35 @@ void drop_batch_graphs(lbm_batch *b) {
   }
 }
 
-// Give box h the batch's block shape (and box-block plan for it).
+// Tile of `threads` threads for box h whose x extent spans whole rows: bx
+// the least divisor of `threads` of at least 16 that holds a row (the whole
+// block when none does), by the largest divisor of the rest that fits the
+// box's y extent, bz what remains (tiles overhanging the box leave threads
+// idle).  A row of the box then lies in lines its own tile reads and
+// writes, as in the one-lattice kernel's default tiles, whatever the other
+// boxes' sizes.
+void row_tile(const lbm_handle *h, int32_t threads, int32_t &bx, int32_t &by, int32_t &bz) {
+  bx = threads;
+  for (int32_t d = 16; d <= threads; d++)
+    if (threads % d == 0 && d >= h->n[0]) {
+      bx = d;
+      break;
+    }
+  const int32_t rest = threads / bx;
+  by = 1;
+  for (int32_t d = 1; d <= rest; d++)
+    if (rest % d == 0 && d <= h->n[1]) by = d;
+  bz = rest / by;
+}
+
+// Box h's tile shape in batch b, with the batch's tile order and occupancy.
+Tiling lattice_shape(const lbm_batch *b, const lbm_handle *h) {
+  Tiling t = b->shape;
+  if (b->rows > 0) row_tile(h, b->rows, t.bx, t.by, t.bz);
+  return t;
+}
+
+// Give box h the block shape `shape` (and box-block plan for it).
 int adopt_shape(lbm_handle *h, const Tiling &shape) {
   if (h->tiling.bx == shape.bx && h->tiling.by == shape.by && h->tiling.bz == shape.bz &&
       h->tiling.remap == shape.remap)
@@ -1765,7 +1796,8 @@ Launch batch_launch(lbm_batch *b, const StepArgs *host, const StepArgs *dev, int
     }
     if (i >= 0) push(i, 1, host[i].t.ntiles);
   }
-  Launch l{dim3((unsigned)ba.start[ba.n]), dim3(b->shape.bx, b->shape.by, b->shape.bz), b->stream};
+  const int threads = b->rows > 0 ? b->rows : b->shape.bx * b->shape.by * b->shape.bz;
+  Launch l{dim3((unsigned)ba.start[ba.n]), dim3((unsigned)threads), b->stream};
   l.shmem = shmem;
   return l;
 }
@@ -1818,7 +1850,7 @@ int prepare_boxes(lbm_batch *b) {
     if (h->parts_h.size() > (size_t)kLocalParticles)
       return fail(LBM_EINVAL, "a box of a batch takes at most " + std::to_string(kLocalParticles) +
                                   " particles; run a box with more alone");
-    if (const int rc = adopt_shape(h, b->shape)) return rc;
+    if (const int rc = adopt_shape(h, lattice_shape(b, h))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
   }
   return LBM_OK;
@@ -1880,7 +1912,7 @@ int lbm_batch_create(lbm_handle *const *boxes, int32_t n, lbm_batch **out) {
     if (t.bx > boxes[i]->n[0] || t.by > boxes[i]->n[1] || t.bz > boxes[i]->n[2]) t = Tiling{64, 1, 1, 0, 0, 0, 0, 1};
   b->shape = t;
   for (lbm_handle *h : b->boxes)
-    if (const int rc = adopt_shape(h, b->shape)) return bail(rc);
+    if (const int rc = adopt_shape(h, lattice_shape(b, h))) return bail(rc);
   *out = b;
   return LBM_OK;
 }
@@ -1975,9 +2007,28 @@ int lbm_batch_set_block(lbm_batch *b, int32_t bx, int32_t by, int32_t bz) {
   t.remap = 1;
   t.occ = b->shape.occ;  // (the occupancy variant stays)
   b->shape = t;
+  b->rows = 0;
   drop_batch_graphs(b);
   for (lbm_handle *h : b->boxes)
-    if (const int rc = adopt_shape(h, b->shape)) return rc;
+    if (const int rc = adopt_shape(h, lattice_shape(b, h))) return rc;
+  return LBM_OK;
+}
+
+int lbm_batch_set_row_tiles(lbm_batch *b, int32_t threads) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (threads < 64 || threads > 512 || threads % 64)
+    return fail(LBM_EINVAL, "row tiles take 64..512 threads, a multiple of 64");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  b->rows = threads;
+  drop_batch_graphs(b);
+  for (lbm_handle *h : b->boxes)
+    if (const int rc = adopt_shape(h, lattice_shape(b, h))) return rc;
+  return LBM_OK;
+}
+
+int lbm_batch_get_row_tiles(lbm_batch *b, int32_t *threads) {
+  if (!b || !threads) return fail(LBM_EINVAL, "null argument");
+  *threads = b->rows;
   return LBM_OK;
 }
 
@@ -1987,7 +2038,7 @@ int lbm_batch_set_tile_order(lbm_batch *b, int32_t xcd_remap) {
   b->shape.remap = xcd_remap != 0;
   drop_batch_graphs(b);
   for (lbm_handle *h : b->boxes)
-    if (const int rc = adopt_shape(h, b->shape)) return rc;
+    if (const int rc = adopt_shape(h, lattice_shape(b, h))) return rc;
   return LBM_OK;
 }
 
@@ -2069,34 +2120,48 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   bxs.erase(std::unique(bxs.begin(), bxs.end()), bxs.end());
   // shapes whose idle threads (tiles overhanging the boxes) stay within 25%
   // of the nodes, or within 15% of the least any shape achieves
+  // one shape for every box (rows = 0), or row tiles of `rows` threads
   struct Shape {
-    int32_t bx, by, bz;
+    int32_t bx, by, bz, rows;
     double waste;
   };
   std::vector<Shape> shapes;
   double least = 1e300;
+  auto add_shape = [&](int32_t bx, int32_t by, int32_t bz, int32_t rows) {
+    int64_t covered = 0;
+    for (const lbm_handle *h : b->boxes) {
+      int32_t x = bx, y = by, z = bz;
+      if (rows) row_tile(h, rows, x, y, z);
+      covered += (int64_t)make_tiling(h, x, y, z).ntiles * x * y * z;
+    }
+    const double waste = (double)covered / (double)nodes - 1.0;
+    shapes.push_back({bx, by, bz, rows, waste});
+    least = std::min(least, waste);
+  };
   for (int32_t bx : bxs)
     for (int32_t by : {1, 2, 4, 8, 16})
       for (int32_t bz : {1, 2, 4}) {
         const int64_t t = (int64_t)bx * by * bz;
         if (t < 64 || t > 512 || t % 64 || bx > mx[0] || by > mx[1] || bz > mx[2]) continue;
-        int64_t covered = 0;
-        for (const lbm_handle *h : b->boxes) covered += (int64_t)make_tiling(h, bx, by, bz).ntiles * t;
-        const double waste = (double)covered / (double)nodes - 1.0;
-        shapes.push_back({bx, by, bz, waste});
-        least = std::min(least, waste);
+        add_shape(bx, by, bz, 0);
       }
-  std::vector<Tiling> cands;
+  for (int32_t rows = 64; rows <= 512; rows += 64) add_shape(0, 0, 0, rows);
+  struct Cand {
+    Tiling t;
+    int32_t rows;
+  };
+  std::vector<Cand> cands;
   for (const Shape &sh : shapes) {
     if (sh.waste > std::max(0.25, least + 0.15)) continue;
     for (int occ = 0; occ < 2; occ++)
       for (int remap : {1, 0}) {
-        Tiling c{};
-        c.bx = sh.bx;
-        c.by = sh.by;
-        c.bz = sh.bz;
-        c.remap = remap;
-        c.occ = occ;
+        Cand c{};
+        c.t.bx = sh.rows ? b->shape.bx : sh.bx;
+        c.t.by = sh.rows ? b->shape.by : sh.by;
+        c.t.bz = sh.rows ? b->shape.bz : sh.bz;
+        c.t.remap = remap;
+        c.t.occ = occ;
+        c.rows = sh.rows;
         cands.push_back(c);
       }
   }
@@ -2122,17 +2187,18 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  const Tiling saved = b->shape;
+  const Cand saved{b->shape, b->rows};
   double best_t = 1e300;
-  Tiling best_c = cands[0];
+  Cand best_c = cands[0];
   int rc = LBM_OK;
   hipError_t e = hipSuccess;
   std::vector<StepArgs> trial(2 * (size_t)n);
-  for (const Tiling &c : cands) {
+  for (const Cand &c : cands) {
+    b->shape = c.t;
+    b->rows = c.rows;
     for (lbm_handle *h : b->boxes)
-      if ((rc = adopt_shape(h, c))) break;
+      if ((rc = adopt_shape(h, lattice_shape(b, h)))) break;
     if (rc) break;
-    b->shape = c;
     for (int i = 0; i < n; i++) {
       lbm_handle *h = b->boxes[i];
       char *s0 = scratch + off[i], *s1 = s0 + scratch_bytes;
@@ -2166,17 +2232,20 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
   hipFree(trial_d);
   hipFree(scratch);
   if (!rc && e != hipSuccess) rc = hip_fail(e, "batch autotune");
-  b->shape = rc ? saved : best_c;
+  const Cand &keep = rc ? saved : best_c;
+  b->shape = keep.t;
+  b->rows = keep.rows;
   drop_batch_graphs(b);
   for (lbm_handle *h : b->boxes) {
-    const int rc2 = adopt_shape(h, b->shape);
+    const int rc2 = adopt_shape(h, lattice_shape(b, h));
     if (!rc) rc = rc2;
   }
   if (rc) return rc;
   if (best) {
-    best[0] = best_c.bx;
-    best[1] = best_c.by;
-    best[2] = best_c.bz;
+    // (row tiles: the thread count, then zeros)
+    best[0] = best_c.rows ? best_c.rows : best_c.t.bx;
+    best[1] = best_c.rows ? 0 : best_c.t.by;
+    best[2] = best_c.rows ? 0 : best_c.t.bz;
   }
   if (best_ms) *best_ms = best_t;
   return LBM_OK;

@@ -170,7 +170,8 @@ struct Tiling {
   uint32_t mtx, mty;    // tile_div multipliers for tx and ty (0: divide)
   int32_t occ;          // 1: the gathering kernels built for half the waves per
                         // SIMD and twice the registers (no spills in the box path)
-  int32_t reserved;
+  uint32_t mbx, mby;    // tile_div multipliers for bx and by over a flat thread
+                        // index (the batched kernel's one-dimensional blocks)
 };
 
 // Multiplier m with n / d == umulhi(n, m) for every n < nmax, or 0 when
@@ -184,8 +185,8 @@ inline uint32_t div_magic(uint32_t d, uint64_t nmax) {
   return (uint32_t)m;
 }
 
-// n / d for a tile index (uniform over the block): a scalar multiply-high
-// instead of an integer division
+// n / d for a tile or thread index: a multiply-high (scalar when n is
+// uniform over the block) instead of an integer division
 __device__ __forceinline__ uint32_t tile_div(uint32_t n, int32_t d, uint32_t m) {
   return m ? __umulhi(n, m) : (d == 1 ? n : n / (uint32_t)d);
 }

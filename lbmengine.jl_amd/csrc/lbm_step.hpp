@@ -1354,8 +1354,12 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
 // -------------------------------------------------------------- step
 // Block b of one lattice's part of a step kernel (see StepMode): blocks
 // [0, nbox_pad) work on the particles' link boxes (gathering modes only),
-// the others each do one tile outside them.
-template <int Q, int S, int COLL, int FM, int MODE>
+// the others each do one tile outside them.  FLAT: the block is
+// one-dimensional (the batched kernel, whose lattices have tiles of their
+// own shapes with the block's thread count) and thread t takes node
+// (t mod bx, (t / bx) mod by, t / (bx by)) of the tile, the node a
+// three-dimensional block's thread of the same linear index takes.
+template <int Q, int S, int COLL, int FM, int MODE, bool FLAT>
 __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   using L = Lattice<Q>;
   using T = typename Store<S>::T;
@@ -1374,9 +1378,22 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   const int tpx = tile - (int)tq * a.t.tx;
   const int tpy = (int)tq - (int)tqq * a.t.ty;
   const int tpz = (int)tqq;
-  const int x = tpx * a.t.bx + threadIdx.x;
-  const int y = tpy * a.t.by + threadIdx.y;
-  const int z = tpz * a.t.bz + threadIdx.z;
+  int ix, iy, iz;
+  if constexpr (FLAT) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t r = tile_div(t, a.t.bx, a.t.mbx);   // t / bx
+    const uint32_t zz = tile_div(r, a.t.by, a.t.mby);  // t / (bx by)
+    ix = (int)(t - r * (uint32_t)a.t.bx);
+    iy = (int)(r - zz * (uint32_t)a.t.by);
+    iz = (int)zz;
+  } else {
+    ix = threadIdx.x;
+    iy = threadIdx.y;
+    iz = threadIdx.z;
+  }
+  const int x = tpx * a.t.bx + ix;
+  const int y = tpy * a.t.by + iy;
+  const int z = tpz * a.t.bz + iz;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
   // whether this tile overlaps particle P's owned region: the six bounds
   // in one go, compared without short-circuits (which would make six
@@ -1548,11 +1565,13 @@ constexpr int kWavesPerEU = OCC ? (S == kF64 ? 2 : 4) : (S == kF64 ? 4 : 8);
 template <int Q, int S, int COLL, int FM, int MODE, int OCC>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<S, OCC>)))
 step_kernel(StepArgs a) {
-  step_block<Q, S, COLL, FM, MODE>(a, blockIdx.x);
+  step_block<Q, S, COLL, FM, MODE, false>(a, blockIdx.x);
 }
 
 // One step kernel over a batch of independent lattices (the same lattice,
-// storage, collision and force kinds; any sizes, parameters and particles).
+// storage, collision and force kinds; any sizes, parameters and particles),
+// in one-dimensional blocks: each lattice has tiles of its own shape with
+// the block's thread count.
 // The grid is a list of segments, each a multiple of 8 blocks: segment s
 // covers blocks [start[s], start[s + 1]) and holds either the box blocks
 // (gathering modes) or the tiles, followed by idle blocks, of lattice
@@ -1583,7 +1602,7 @@ batch_step_kernel(BatchArgs ba) {
   const int nbox = MODE != kCollide ? a.sp.nbox_pad : 0;
   const int local = b - ba.start[lo] + ((ba.seg[lo] & 1) ? nbox : 0);
   if (local >= nbox + a.t.ntiles) return;
-  step_block<Q, S, COLL, FM, MODE>(a, local);
+  step_block<Q, S, COLL, FM, MODE, true>(a, local);
 }
 
 // -------------------------------------------------------- aux kernels

@@ -126,6 +126,8 @@ SIGNATURES = {
     "lbm_batch_sync": [_vp],
     "lbm_batch_set_block": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "lbm_batch_get_block": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "lbm_batch_set_row_tiles": [_vp, C.c_int32],
+    "lbm_batch_get_row_tiles": [_vp, C.POINTER(C.c_int32)],
     "lbm_batch_autotune": [_vp, C.c_int32, C.POINTER(C.c_int32), _dp],
     "lbm_batch_set_graph": [_vp, C.c_int32],
     "lbm_batch_set_timing": [_vp, C.c_int],

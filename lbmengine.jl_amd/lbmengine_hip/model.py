@@ -720,6 +720,17 @@ class LBMbatch:
         _capi.check(self._fn["lbm_batch_get_block"](self._b, *[C.byref(x) for x in b]))
         return tuple(x.value for x in b)
 
+    def set_row_tiles(self, threads: int) -> None:
+        """Blocks of `threads` threads; every box a tile shape of that many
+        threads spanning whole rows of the box."""
+        _capi.check(self._fn["lbm_batch_set_row_tiles"](self._b, int(threads)))
+
+    def get_row_tiles(self) -> int:
+        """Threads of the row tiles, 0 while the boxes share one block shape."""
+        v = C.c_int32()
+        _capi.check(self._fn["lbm_batch_get_row_tiles"](self._b, C.byref(v)))
+        return v.value
+
     def set_tile_order(self, xcd_remap: int) -> None:
         _capi.check(self._fn["lbm_batch_set_tile_order"](self._b, int(xcd_remap)))
 
@@ -737,7 +748,8 @@ class LBMbatch:
         return v.value
 
     def autotune(self, reps: int = 10):
-        """Pick the fastest block shape of the batched stream-collide kernel."""
+        """Pick the fastest block shape of the batched stream-collide kernel:
+        returns (shape, ms) with shape (threads, 0, 0) for row tiles."""
         best = (C.c_int32 * 3)()
         ms = C.c_double()
         _capi.check(self._fn["lbm_batch_autotune"](self._b, int(reps), best, C.byref(ms)))

@@ -130,6 +130,31 @@ def test_batch_block_shapes(gpu, block):
         same(a, b)
 
 
+@pytest.mark.parametrize("threads", [64, 192, 320])
+def test_batch_row_tiles(gpu, threads):
+    """Every box its own tile shape of `threads` threads spanning its rows
+    (x extents 16..26 nodes: tiles of 16 to 64 threads wide, some with
+    several z layers or overhanging the box), then back to one shared shape."""
+    pairs = uneven_boxes(gpu)
+    batch = gpu.LBMbatch([a for a, _ in pairs])
+    batch.set_graph(4)
+    batch.set_row_tiles(threads)
+    assert batch.get_row_tiles() == threads
+    for steps in (7, 8):
+        gpu.LBMpropagate(batch, ticks=steps)
+        for _, b in pairs:
+            gpu.LBMpropagate(b, ticks=steps)
+        for a, b in pairs:
+            same(a, b)
+    batch.set_block(64, 1, 1)
+    assert batch.get_row_tiles() == 0 and batch.get_block() == (64, 1, 1)
+    gpu.LBMpropagate(batch, ticks=5)
+    for _, b in pairs:
+        gpu.LBMpropagate(b, ticks=5)
+    for a, b in pairs:
+        same(a, b)
+
+
 def test_batch_autotune_keeps_state(gpu):
     pairs = uneven_boxes(gpu, "f32")
     batch = gpu.LBMbatch([a for a, _ in pairs])
@@ -138,7 +163,10 @@ def test_batch_autotune_keeps_state(gpu):
         gpu.LBMpropagate(b, ticks=3)
     f0 = [a.distributions for a, _ in pairs]
     shape, ms = batch.autotune(2)
-    assert ms > 0 and np.prod(shape) >= 64 and batch.get_block() == tuple(shape)
+    if shape[1] == 0:  # row tiles
+        assert ms > 0 and shape[0] >= 64 and batch.get_row_tiles() == shape[0]
+    else:
+        assert ms > 0 and np.prod(shape) >= 64 and batch.get_block() == tuple(shape)
     for (a, _), f in zip(pairs, f0):
         assert np.array_equal(a.distributions, f)
     gpu.LBMpropagate(batch, ticks=8)
