@@ -132,7 +132,8 @@ __host__ __device__ __forceinline__ void node_velocity_vjp(const double (&xc)[3]
 }
 
 struct AdjArgs {
-  const void *S;        // S_k, storage type, row-blocked (as the step's buffers)
+  const void *P;        // P_k = C(S_k), storage type, row-blocked (as the step's buffers)
+  const void *H;        // the moments of S_k (rho, u; compute type, row-blocked by 4, as the keeping modes write them)
   const void *lam_in;   // dL/dS_{k+1}, compute type, row-blocked
   void *lam_out;        // dL/dS_k
   double *gF;           // (N, 3) += dL/dF (Julia layout k N + n)
@@ -146,10 +147,12 @@ struct AdjArgs {
   double coeff, ls;     // momentum-exchange coefficient and lattice speed
   double shell;         // 1.5 dx: link nodes lie within r + shell of a centre
   const double *phitau;  // [particle][6]: dL/dF_p (0..2) and dL/dT_p (3..5) of the tick's move
-  // the near kernel's blocks: particle p's are [near_first[p], near_first[p + 1]),
-  // enough for near_ext[p]^3 nodes of its near box
+  // the near blocks (the first of the grid): particle p's are
+  // [near_first[p], near_first[p + 1]), enough for near_ext[p]^3 nodes of
+  // its near box; the bulk blocks follow
   int32_t near_first[kAdjParticles + 1];
   int32_t near_ext[kAdjParticles];
+  int32_t nblocks_near;
 };
 
 // Near box of a particle: its link box grown by one node (clamped to the
@@ -201,33 +204,34 @@ struct AdjTailArgs {
   AdjFinishArgs fin;
   AdjMoveArgs mv;         // (mv.mu unused: the state comes from fin, in LDS)
 };
-constexpr int kAdjThreads = 256;  // threads per block of the bulk and near kernels
+constexpr int kAdjThreads = 256;  // threads per block of the adjoint kernel
+// Waves per SIMD the adjoint kernel is built for: its near blocks' slot
+// phase then fits in 128 registers without spilling, and the bulk blocks,
+// which need 85, keep four waves per SIMD.
+constexpr int kAdjWaves = 4;
 constexpr int kAdjTailThreads = 64 * kAdjParticles;  // one wave per particle's move
 constexpr int kNearSlots = 1024;  // link slots a near block stages in LDS per pass
 
-// The forward collision of a fluid node (own_idx: its row-blocked offset, n:
-// its Julia index): the gathered populations f, their density rho, the force
-// Fn and the post-collision populations P.
-template <int Q, int S, int COLL, int FM>
-__device__ __forceinline__ void adj_forward(const AdjArgs &a, int64_t own_idx, int64_t n, typename Store<S>::R (&f)[Q],
-                                           typename Store<S>::R (&P)[Q], typename Store<S>::R &rho,
+// What the reverse collision of a fluid node needs of the forward one (x,
+// row: its position; n: its Julia index): the density and the equilibrium
+// velocity of S_k, kept by the recomputation, and the force.
+template <int Q, int S, int FM>
+__device__ __forceinline__ void adj_moments(const AdjArgs &a, int x, int64_t row, int64_t n,
+                                           typename Store<S>::R &rho, typename Store<S>::R (&u)[3],
                                            typename Store<S>::R (&Fn)[3]) {
-  using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   const BoxGeom &g = a.g;
-  const T *Sk = static_cast<const T *>(a.S);
+  const R *H = static_cast<const R *>(a.H) + row * 4 * g.nx + x;
+  rho = H[0];
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(Sk[own_idx + (int64_t)q * g.nx], q);
+  for (int k = 0; k < 3; k++) u[k] = H[(int64_t)(k + 1) * g.nx];
+  Fn[0] = Fn[1] = Fn[2] = (R)0;
   if constexpr (FM == LBM_FORCE_UNIFORM) {
     for (int k = 0; k < 3; k++) Fn[k] = (R)a.c.F[k];
   } else if constexpr (FM == LBM_FORCE_FIELD) {
     const R *Ff = static_cast<const R *>(a.Ffield);
     for (int k = 0; k < 3; k++) Fn[k] = Ff[k * g.N + n];
   }
-  if constexpr (S == kF64)
-    collide_ref<Q, COLL, FM>(f, Fn, a.c, P, rho);
-  else
-    collide_dev<Q, COLL, FM>(f, Fn, a.c, P, rho);
 }
 
 // dL/dP_k of a fluid node from dL/dS_{k+1}: lambda_q(x + c_q), except where
@@ -253,28 +257,12 @@ __device__ __forceinline__ void adj_gather(const AdjArgs &a, int x, int y, int z
 // dL/dS_k of a fluid node = the collision's VJP of lp + the Ladd terms'
 // dL/drho (rho = sum of S_k); dL/dF += the Guo term's part.
 template <int Q, int S, int COLL, int FM>
-__device__ __forceinline__ void adj_store(const AdjArgs &a, int64_t own_idx, int64_t n,
-                                         const typename Store<S>::R (&f)[Q], typename Store<S>::R rho,
-                                         const typename Store<S>::R (&Fn)[3], const typename Store<S>::R (&lp)[Q],
-                                         typename Store<S>::R grho) {
-  using L = Lattice<Q>;
+__device__ __forceinline__ void adj_store(const AdjArgs &a, int64_t own_idx, int64_t n, typename Store<S>::R rho,
+                                         const typename Store<S>::R (&u)[3], const typename Store<S>::R (&Fn)[3],
+                                         const typename Store<S>::R (&lp)[Q], typename Store<S>::R grho) {
   using R = typename Store<S>::R;
   const BoxGeom &g = a.g;
   R *lout = static_cast<R *>(a.lam_out);
-  R u[3] = {0, 0, 0};
-  {
-    R j[3] = {0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      j[0] += (R)L::cx[q] * f[q];
-      j[1] += (R)L::cy[q] * f[q];
-      j[2] += (R)L::cz[q] * f[q];
-    }
-    if (FM != LBM_FORCE_NONE)
-      for (int k = 0; k < 3; k++) j[k] += (R)(0.5 * a.c.dt) * Fn[k];
-    const R inv = rho != (R)0 ? (R)1 / rho : (R)0;
-    for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
-  }
   R ls_[Q], gFn[3] = {0, 0, 0};
   collide_vjp<Q, COLL, FM, R>(rho, u, Fn, a.c, lp, ls_, gFn);
 #pragma unroll
@@ -284,14 +272,14 @@ __device__ __forceinline__ void adj_store(const AdjArgs &a, int64_t own_idx, int
 }
 
 // One adjoint tick on the bulk: dL/dS_k from dL/dS_{k+1}, with dL/dF +=, at
-// every node outside the particles' near boxes (one thread per node, x
-// fastest over the lattice; no particle term reaches these nodes).
+// every node outside the particles' near boxes (block b of the bulk, one
+// thread per node, x fastest over the lattice; no particle term reaches
+// these nodes).
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs a) {
+__device__ __forceinline__ void adjoint_bulk_block(const AdjArgs &a, int b) {
   using R = typename Store<S>::R;
-  static_assert(Q == 19, "the adjoint is built for D3Q19");
   const BoxGeom &g = a.g;
-  const int64_t nn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = (int64_t)b * kAdjThreads + threadIdx.x;
   if (nn >= g.N) return;
   const int x = (int)(nn % g.nx);
   const int64_t r = nn / g.nx;
@@ -306,10 +294,10 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs a) {
     for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], (R)0);
     return;
   }
-  R f[Q], P[Q], rho = 0, Fn[3] = {0, 0, 0}, lp[Q];
-  adj_forward<Q, S, COLL, FM>(a, own_idx, n, f, P, rho, Fn);
+  R rho, u[3], Fn[3], lp[Q];
+  adj_moments<Q, S, FM>(a, x, row, n, rho, u, Fn);
   adj_gather<Q, R>(a, x, y, z, own_idx, 0u, lp);
-  adj_store<Q, S, COLL, FM>(a, own_idx, n, f, rho, Fn, lp, (R)0);
+  adj_store<Q, S, COLL, FM>(a, own_idx, n, rho, u, Fn, lp, (R)0);
 }
 
 // The owners' Ladd chain of one link slot (x, q), reversed.  Forward:
@@ -390,23 +378,23 @@ __device__ __forceinline__ void adj_block_sum(const double (&acc)[kAdjSums], DD 
   __syncthreads();
 }
 
-// One adjoint tick on the particles' near boxes (block b of particle p
+// One adjoint tick on the particles' near boxes (near block b of particle p
 // covers box indices [j 256, (j + 1) 256) of p's near box, j = b -
 // near_first[p], x fastest; a node in the boxes of several particles belongs
-// to the first).  A lane per node gathers, finds the node's link slots and
-// the streams links replace, and stages its slots in LDS (node order, each
+// to the first).  A lane per node finds the node's link slots and the
+// streams links replace, and stages its slots in LDS (node order, each
 // node's in increasing q); a lane per slot then runs the owners' Ladd chain
-// backwards (slot_chain), and the node lanes add the results in the order
-// they staged them before the collision's VJP.  The per-particle sums of the
-// block go to partial[b][p][...].
+// backwards (slot_chain); the node lanes take the results into LDS rows of
+// their own in the order they staged them, then gather, add them and run the
+// collision's VJP.  The per-particle sums of the block go to
+// partial[b][p][...].
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
+__device__ __forceinline__ void adjoint_near_block(const AdjArgs &a, int b) {
   using L = Lattice<Q>;
+  using T = typename Store<S>::T;
   using R = typename Store<S>::R;
-  static_assert(Q == 19, "the adjoint is built for D3Q19");
   const BoxGeom &g = a.g;
   const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
-  const int b = blockIdx.x;
   int pb = 0;
   while (pb + 1 < a.np && b >= a.near_first[pb + 1]) pb++;
   int lo[3], hi[3];
@@ -430,9 +418,10 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
   const int64_t n = row * g.nx + x, own_idx = row * Q * g.nx + x;
   const bool fluid = live && !is_wall_node(g, x, y, z);
   const R *lin = static_cast<const R *>(a.lam_in);
+  const T *Pk = static_cast<const T *>(a.P);
 
-  R f[Q], P[Q], rho = 0, Fn[3] = {0, 0, 0};
-  if (fluid) adj_forward<Q, S, COLL, FM>(a, own_idx, n, f, P, rho, Fn);
+  R rho = 0, u[3] = {0, 0, 0}, Fn[3] = {0, 0, 0};
+  if (fluid) adj_moments<Q, S, FM>(a, x, row, n, rho, u, Fn);
 
   // link geometry, the forward box blocks' predicates: (x, q) is a link slot
   // of p when x lies in p's link box, r^2 <= d2(x) < (r + 1.5 dx)^2 and
@@ -481,9 +470,7 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
       own_any |= op;
     }
   }
-  R lp[Q];
   R grho = 0;
-  if (fluid) adj_gather<Q, R>(a, x, y, z, own_idx, drop, lp);
 
   // this node's first slot in the block's list: an exclusive scan of the
   // slot counts over the lanes, then the waves
@@ -493,6 +480,7 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
   __shared__ R s_rho[kAdjThreads];
   __shared__ DD red[kAdjThreads / 64][kAdjSums];
   __shared__ DD tot[kAdjParticles][kAdjSums];
+  __shared__ R s_add[Q][kAdjThreads];  // row tid: dL/dfs of node tid's slots, by the population they add to
   const int cnt = __popc(own_any);
   int incl = cnt;
 #pragma unroll
@@ -523,7 +511,7 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
           for (int p = 0; p < kAdjParticles; p++) om |= ((own[p] >> q) & 1u) << p;
           s_key[s] = (uint32_t)tid | (uint32_t)q << 8 | om << 16;
           s_a[s] = lin[own_idx + (int64_t)q * g.nx];
-          s_b[s] = P[L::opp[q]];
+          s_b[s] = decode<S, Q>(Pk[own_idx + (int64_t)L::opp[q] * g.nx], L::opp[q]);
         }
         s++;
       }
@@ -585,7 +573,7 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
       for (int q = 1; q < Q; q++) {
         if (!((own_any >> q) & 1u)) continue;
         if (s >= 0 && s < kNearSlots) {
-          lp[L::opp[q]] += s_a[s];
+          s_add[L::opp[q]][tid] = s_a[s];
           grho += s_b[s];
         }
         s++;
@@ -600,13 +588,33 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], (R)0);
     } else {
-      adj_store<Q, S, COLL, FM>(a, own_idx, n, f, rho, Fn, lp, grho);
+      R lp[Q];
+      adj_gather<Q, R>(a, x, y, z, own_idx, drop, lp);
+#pragma unroll
+      for (int q = 1; q < Q; q++)
+        if ((own_any >> q) & 1u) lp[L::opp[q]] += s_add[L::opp[q]][tid];
+      adj_store<Q, S, COLL, FM>(a, own_idx, n, rho, u, Fn, lp, grho);
     }
   }
   if (tid < a.np * kAdjSums) {
     const DD t = tot[tid / kAdjSums][tid % kAdjSums];
     a.partial[(int64_t)b * a.np * kAdjSums + tid] = t.hi + t.lo;
   }
+}
+
+// One adjoint tick (dL/dS_k from dL/dS_{k+1}, with dL/dF +=) in one launch:
+// the near blocks (the first nblocks_near of the grid, dispatched first)
+// beside the bulk blocks, so the near blocks' chain of dependent steps runs
+// while the bulk streams.
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(kAdjWaves)))
+adjoint_kernel(AdjArgs a) {
+  static_assert(Q == 19, "the adjoint is built for D3Q19");
+  const int b = blockIdx.x;
+  if (b < a.nblocks_near)
+    adjoint_near_block<Q, S, COLL, FM>(a, b);
+  else
+    adjoint_bulk_block<Q, S, COLL, FM>(a, b - a.nblocks_near);
 }
 
 }  // namespace lbm

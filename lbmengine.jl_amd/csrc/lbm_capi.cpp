@@ -2320,13 +2320,15 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   HIP_TRY(hipStreamSynchronize(h->stream), "sync");
 
   AdjBuffers buf;
-  char *ck = nullptr, *seg = nullptr, *fin = nullptr;
+  char *ck = nullptr, *seg = nullptr, *hyd = nullptr, *fin = nullptr;
+  const size_t hyd_bytes = N * 4 * rsize;  // one tick's moments (rho, u)
   void *lam[2] = {nullptr, nullptr};
   double *gF = nullptr, *partial = nullptr;
   DevParticle *recs_seg = nullptr;  // the records of every tick of a segment (two sets)
   DevParticle *ck_recs_d = nullptr;  // the checkpoints' records
   if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess ||
       buf.alloc(&seg, 2 * (size_t)seg_len * pop_bytes) != hipSuccess ||
+      buf.alloc(&hyd, 2 * (size_t)seg_len * hyd_bytes) != hipSuccess ||
       buf.alloc(&fin, pop_bytes) != hipSuccess || buf.alloc(&lam[0], N * h->Q * rsize) != hipSuccess ||
       buf.alloc(&lam[1], N * h->Q * rsize) != hipSuccess || buf.alloc(&gF, N * 3 * sizeof(double)) != hipSuccess ||
       buf.alloc(&partial, (size_t)std::max<int64_t>(nblocks_near, 1) * np * kAdjSums * sizeof(double)) != hipSuccess ||
@@ -2470,6 +2472,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.phitau = pt_d;
   for (int p = 0; p <= kAdjParticles; p++) aa.near_first[p] = near_first[p];
   for (int p = 0; p < kAdjParticles; p++) aa.near_ext[p] = near_ext[p];
+  aa.nblocks_near = (int32_t)nblocks_near;
   AdjMoveArgs ma{};
   ma.mu = mu_d;
   ma.mu_in = muin_d;
@@ -2487,10 +2490,10 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   // the two share only the read-only checkpoints.  The recomputation uses
   // the lattice's own working state (populations' spare buffer, records,
   // partials, rho), which the reverse sweep does not touch.
-  // A reverse tick is the near kernel, the bulk kernel and the tail, in
-  // that order on the handle's stream: a further stream for the near kernel
-  // can share a hardware queue with the recomputation's and then waits
-  // behind its kernels.
+  // A reverse tick is the adjoint kernel (near and bulk blocks in one
+  // launch) and the tail, on the handle's stream: a further stream can
+  // share a hardware queue with the recomputation's and then waits behind
+  // its kernels.
   hipStream_t s_rev = h->stream, s_rec = nullptr;
   hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
   constexpr int kEvents = 4;
@@ -2518,44 +2521,73 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   ev_rev[1] = evs[3];
   HIP_TRY(hipEventRecord(ev_rev[0], s_rev), "event");  // (both sets free)
   HIP_TRY(hipEventRecord(ev_rev[1], s_rev), "event");
-  // S_k of segment c's ticks: the checkpoint itself, then slots 1.. of its
-  // set; slot 0 takes the last tick's output, which no tick reads
-  auto Sk = [&](int64_t c, int64_t j) -> void * {
-    return j == 0 ? static_cast<void *>(ck + (size_t)c * pop_bytes)
-                  : static_cast<void *>(seg + ((size_t)(c & 1) * seg_len + (size_t)(j % seg_len)) * pop_bytes);
+  // segment c's tick j (0 <= j < its length): P_k = C(S_k) and the
+  // moments of S_k in slot j of the segment's set, the records at its start
+  // in recs_of(c, j) (and after the segment's last move in recs_of(c, len))
+  auto Pk = [&](int64_t c, int64_t j) -> void * {
+    return seg + ((size_t)(c & 1) * seg_len + (size_t)j) * pop_bytes;
   };
+  auto Hk = [&](int64_t c, int64_t j) -> void * { return hyd + ((size_t)(c & 1) * seg_len + (size_t)j) * hyd_bytes; };
   auto recs_of = [&](int64_t c, int64_t j) { return recs_seg + ((size_t)(c & 1) * (seg_len + 1) + (size_t)j) * np; };
-  // recompute segment c tick by tick (collide, gather, move), each tick's
-  // output in its own slot and each tick's records on the device
+  // recompute segment c as a run of its ticks: collide (keeping P_{t0} and
+  // the moments), fused kernels (keeping each P_k and its moments), the last
+  // tick's gather and its move; each gathering kernel writes the records it
+  // streamed with straight into the segment's record slots
   auto recompute = [&](int64_t c) -> int {
     const int64_t t0 = c * K, len = std::min(t0 + K, T) - t0;
     h->stream = s_rec;
     HIP_TRY(hipStreamWaitEvent(s_rec, ev_rev[c & 1], 0), "wait for the set");
-    HIP_TRY(hipMemcpyAsync(parts_of(h, h->pcur), ck_recs_d + (size_t)c * np, np * sizeof(DevParticle),
+    int pp = h->pcur;  // the parity of the partials the next gathering kernel reads
+    HIP_TRY(hipMemcpyAsync(parts_of(h, pp), ck_recs_d + (size_t)c * np, np * sizeof(DevParticle),
                            hipMemcpyDeviceToDevice, s_rec),
             "restore checkpoint records");
-    HIP_TRY(hipMemcpyAsync(recs_of(c, 0), parts_of(h, h->pcur), np * sizeof(DevParticle), hipMemcpyDeviceToDevice,
-                           s_rec),
-            "keep segment records");
-    for (int64_t j = 0; j < len; j++) {
-      hipError_t e = launch_mode(h, kCollide, Sk(c, j), h->f[h->cur ^ 1], h->pcur);
-      if (e == hipSuccess) e = launch_mode(h, kGather, h->f[h->cur ^ 1], Sk(c, j + 1), h->pcur);
-      if (e != hipSuccess) return hip_fail(e, "segment step launch");
-      h->pcur ^= 1;
-      StepArgs fa = step_args(h, nullptr, nullptr, h->pcur);
-      fa.sp.parts_out = parts_of(h, h->pcur);  // in place
-      e = launch_finalize(fa.sp, s_rec);
-      if (e != hipSuccess) return hip_fail(e, "finalize launch");
-      HIP_TRY(hipMemcpyAsync(recs_of(c, j + 1), parts_of(h, h->pcur), np * sizeof(DevParticle),
-                             hipMemcpyDeviceToDevice, s_rec),
-              "keep segment records");
+    auto keep = [&](int mode, const void *fin, void *fout, void *moments, const DevParticle *pin, DevParticle *pout) {
+      StepArgs a = step_args(h, fin, fout, pp);
+      a.hydro = moments;
+      a.sp.parts_in = pin;
+      a.sp.parts_out = pout;
+      const Launch lk = step_launch(h, a, base_mode(mode));
+      switch (h->cfg.storage) {
+        case LBM_STORAGE_F64: return launch_step_keep<kF64>(h->cfg.collision, h->cfg.force, mode, a, lk);
+        case LBM_STORAGE_F32: return launch_step_keep<kF32>(h->cfg.collision, h->cfg.force, mode, a, lk);
+        case LBM_STORAGE_F16C: return launch_step_keep<kF16C>(h->cfg.collision, h->cfg.force, mode, a, lk);
+        default: return launch_step_keep<kF16>(h->cfg.collision, h->cfg.force, mode, a, lk);
+      }
+    };
+    // gathering kernel m (1..len) streams tick t0 + m - 1 with the records
+    // recs_of(c, m - 1), which it writes (the checkpoint's, for m = 1)
+    auto rin = [&](int64_t m) -> const DevParticle * { return m == 1 ? parts_of(h, pp) : recs_of(c, m - 2); };
+    // (the collide-only kernel has no box blocks and writes no records)
+    hipError_t e = keep(kCollideKeep, ck + (size_t)c * pop_bytes, Pk(c, 0), Hk(c, 0), parts_of(h, pp),
+                        parts_of(h, pp ^ 1));
+    for (int64_t m = 1; m < len && e == hipSuccess; m++) {
+      e = keep(kFusedKeep, Pk(c, m - 1), Pk(c, m), Hk(c, m), rin(m), recs_of(c, m - 1));
+      pp ^= 1;
     }
+    if (e == hipSuccess) {
+      StepArgs a = step_args(h, Pk(c, len - 1), h->f[h->cur ^ 1], pp);
+      a.sp.parts_in = rin(len);
+      a.sp.parts_out = recs_of(c, len - 1);
+      const Launch lg = step_launch(h, a, kGather);
+      switch (h->cfg.storage) {
+        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
+        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
+        case LBM_STORAGE_F16C: e = launch_step<kF16C>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
+        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
+      }
+      pp ^= 1;
+    }
+    if (e != hipSuccess) return hip_fail(e, "segment step launch");
+    StepArgs fa = step_args(h, nullptr, nullptr, pp);
+    fa.sp.parts_in = recs_of(c, len - 1);
+    fa.sp.parts_out = recs_of(c, len);
+    e = launch_finalize(fa.sp, s_rec);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
     HIP_TRY(hipEventRecord(ev_rec[c & 1], s_rec), "event");
     h->stream = s_rev;
     return LBM_OK;
   };
-  const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
-  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_rev};
+  const Launch l{dim3((unsigned)(nblocks_near + nblocks)), dim3(threads), s_rev};
   AdjTailArgs ta{};
   ta.partial = partial;
   ta.nblocks = (int32_t)nblocks_near;
@@ -2578,11 +2610,11 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
         e = launch_adjoint_move(ma, np, s_rev);
       }
       aa.parts = recs_of(c, k - t0);
-      aa.S = Sk(c, k - t0);
+      aa.P = Pk(c, k - t0);
+      aa.H = Hk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
-      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, true, aa, ln);
-      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
       ta.fin = fa2;
       ta.fin.recs_k = recs_of(c, k - t0);
       ta.fin.direct = k > ws;  // V_k is one of the averaged speeds

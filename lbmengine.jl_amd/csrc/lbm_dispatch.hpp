@@ -19,6 +19,9 @@ template <int S>
 hipError_t launch_step_q15_q27(int Q, int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_batch_step(int Q, int coll, int fm, int mode, int occ, const BatchArgs &a, const Launch &l);
+// kCollideKeep or kFusedKeep on a D3Q19 lattice (the adjoint's recomputation)
+template <int S>
+hipError_t launch_step_keep(int coll, int fm, int mode, const StepArgs &a, const Launch &l);
 template <int S>
 hipError_t launch_init_uniform(int Q, const AuxArgs &a, const Launch &l);
 template <int S>
@@ -39,9 +42,8 @@ hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t strea
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
 struct AdjArgs;
-// near = false: the bulk kernel (nodes outside the particles' near boxes);
-// true: the near kernel (their nodes, with the particle terms and sums)
-hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArgs &a, const Launch &l);
+// one reverse tick: the particles' near boxes and the bulk in one launch
+hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l);
 struct AdjMoveArgs;
 struct AdjTailArgs;
 hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream);

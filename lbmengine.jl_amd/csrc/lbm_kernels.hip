@@ -3,14 +3,15 @@
 // 1 (FP32), 2 (FP16) or 3 (custom 16-bit), and per part: -DLBM_PART=0 builds
 // the one-lattice step of D2Q9/D3Q19 and the auxiliary kernels of every
 // lattice, -DLBM_PART=1 the batched step (D2Q9/D3Q19), -DLBM_PART=2 the
-// one-lattice step of D3Q15/D3Q27 (separate objects compile in parallel).
+// one-lattice step of D3Q15/D3Q27, -DLBM_PART=3 the D3Q19 kernels that keep
+// the moments (separate objects compile in parallel).
 #include "lbm_dispatch.hpp"
 
 #ifndef LBM_STORE
 #error "LBM_STORE must be 0 (FP64), 1 (FP32), 2 (FP16) or 3 (custom 16-bit)"
 #endif
 #ifndef LBM_PART
-#error "LBM_PART must be 0 (one lattice and auxiliary kernels), 1 (batched step) or 2 (D3Q15/D3Q27 step)"
+#error "LBM_PART must be 0 (one lattice and auxiliary kernels), 1 (batched step), 2 (D3Q15/D3Q27 step) or 3 (keeping modes)"
 #endif
 
 namespace lbm {
@@ -60,6 +61,40 @@ static hipError_t launch_batch_qcf(int mode, int occ, const BatchArgs &a, const 
 template <>
 hipError_t launch_batch_step<S>(int Q, int coll, int fm, int mode, int occ, const BatchArgs &a, const Launch &l) {
   LBM_KEY_SWITCH(9, 19, launch_batch_qcf, mode, occ, a, l)
+}
+
+#elif LBM_PART == 3
+
+template <int COLL, int FM>
+static hipError_t launch_keep_cf(int mode, const StepArgs &a, const Launch &l) {
+  constexpr int Q = 19;
+  switch (mode * 2 + (a.t.occ ? 1 : 0)) {
+    case 2 * kCollideKeep:
+    case 2 * kCollideKeep + 1:
+      hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kCollideKeep, 0>), l.grid, l.block, l.shmem, l.stream, a);
+      break;
+    case 2 * kFusedKeep:
+      hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kFusedKeep, 0>), l.grid, l.block, l.shmem, l.stream, a);
+      break;
+    case 2 * kFusedKeep + 1:
+      hipLaunchKernelGGL((step_kernel<Q, S, COLL, FM, kFusedKeep, 1>), l.grid, l.block, l.shmem, l.stream, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_step_keep<S>(int coll, int fm, int mode, const StepArgs &a, const Launch &l) {
+  switch (coll * 10 + fm) {
+    case 0: return launch_keep_cf<0, 0>(mode, a, l);
+    case 1: return launch_keep_cf<0, 1>(mode, a, l);
+    case 2: return launch_keep_cf<0, 2>(mode, a, l);
+    case 10: return launch_keep_cf<1, 0>(mode, a, l);
+    case 11: return launch_keep_cf<1, 1>(mode, a, l);
+    case 12: return launch_keep_cf<1, 2>(mode, a, l);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 #elif LBM_PART == 2

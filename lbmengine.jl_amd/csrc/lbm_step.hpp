@@ -118,6 +118,7 @@ struct StepArgs {
   void *fout;
   const void *Ffield;  // (N, d): double for FP64 storage, float otherwise
   void *rho_side;      // (N): rho of the last collision at link-box nodes (compute type)
+  void *hydro;         // the moments the keeping modes write (see StepMode), else null
   BoxGeom g;
   CollParams c;
   Tiling t;
@@ -131,7 +132,7 @@ struct StepArgs {
 template <int Q, int COLL, int FM>
 __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *Fn,
                                             const CollParams &c, double (&fs)[Q],
-                                            double &rho_out) {
+                                            double &rho_out, double *u_out = nullptr) {
 #pragma clang fp contract(off)
   using L = Lattice<Q>;
   constexpr int D = L::D;
@@ -166,6 +167,8 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
 #pragma unroll
   for (int k = 0; k < D; k++) u[k] = rho != 0.0 ? mom[k] / rho : 0.0;
   rho_out = rho;
+  if (u_out)
+    for (int k = 0; k < 3; k++) u_out[k] = k < D ? u[k] : 0.0;
   // equilibriumDistribution (gpu/fluids.jl:39-52)
   double uu = 0.0;
 #pragma unroll
@@ -244,7 +247,7 @@ __host__ __device__ inline void collide_ref(const double (&f)[Q], const double *
 template <int Q, int COLL, int FM>
 __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn,
                                             const CollParams &c, float (&ds)[Q],
-                                            float &rho_out) {
+                                            float &rho_out, float *u_out = nullptr) {
   using L = Lattice<Q>;
   constexpr int D = L::D;
   const float ls = c.f_ls;
@@ -274,6 +277,8 @@ __device__ __forceinline__ void collide_dev(const float (&d)[Q], const float *Fn
 #pragma unroll
   for (int k = 0; k < 3; k++) u[k] = j[k] * inv;
   rho_out = rho;
+  if (u_out)
+    for (int k = 0; k < 3; k++) u_out[k] = k < D ? u[k] : 0.0f;
   // density multiplying the velocity terms of the equilibrium
   const float rv = c.incompressible ? c.f_rho0 : rho;
   float uu = 0.0f;
@@ -354,7 +359,7 @@ constexpr bool lattice_has_class(int k) {
 
 template <int Q, int COLL, int FM, int SK = kF16>
 __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q], const float *Fn, const CollParams &c,
-                                            typename Store<SK>::T (&hs)[Q], float &rho_out) {
+                                            typename Store<SK>::T (&hs)[Q], float &rho_out, float *u_out = nullptr) {
 #pragma clang fp contract(off)
   using L = Lattice<Q>;
   constexpr int D = L::D, NC = L::NC;
@@ -412,6 +417,8 @@ __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q],
 #pragma unroll
   for (int k = 0; k < D; k++) u[k] = j[k] * inv;
   rho_out = rho;
+  if (u_out)
+    for (int k = 0; k < 3; k++) u_out[k] = u[k];
   // density multiplying the velocity terms of the equilibrium
   const float rv = c.incompressible ? c.f_rho0 : rho;
   float uu = 0.0f;
@@ -471,22 +478,23 @@ __device__ __forceinline__ void collide_h16(const typename Store<SK>::T (&h)[Q],
   }
 }
 
-// Collision from stored values to stored values (the tiles' path).
+// Collision from stored values to stored values (the tiles' path); u_out,
+// when given, receives the velocity the equilibrium was taken at.
 template <int Q, int S, int COLL, int FM>
 __device__ __forceinline__ void collide_stored(const typename Store<S>::T (&raw)[Q], const typename Store<S>::R *Fn,
                                                const CollParams &c, typename Store<S>::T (&out)[Q],
-                                               typename Store<S>::R &rho) {
+                                               typename Store<S>::R &rho, typename Store<S>::R *u_out = nullptr) {
   using R = typename Store<S>::R;
   if constexpr (S == kF16 || S == kF16C) {
-    collide_h16<Q, COLL, FM, S>(raw, Fn, c, out, rho);
+    collide_h16<Q, COLL, FM, S>(raw, Fn, c, out, rho, u_out);
   } else {
     R f[Q], fs[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = decode<S, Q>(raw[q], q);
     if constexpr (S == kF64)
-      collide_ref<Q, COLL, FM>(f, Fn, c, fs, rho);
+      collide_ref<Q, COLL, FM>(f, Fn, c, fs, rho, u_out);
     else
-      collide_dev<Q, COLL, FM>(f, Fn, c, fs, rho);
+      collide_dev<Q, COLL, FM>(f, Fn, c, fs, rho, u_out);
 #pragma unroll
     for (int q = 0; q < Q; q++) out[q] = encode<S, Q>(fs[q], q);
   }
@@ -811,8 +819,13 @@ __device__ __forceinline__ GatherRows gather_rows(const BoxGeom &g, int Q, int y
 // post-collision populations): kCollide P_0 = C(S_0); kFused S_k = G(P_{k-1})
 // then P_k = C(S_k); kGather S_n = G(P_{n-1}).  G is streaming with halfway
 // bounce-back, the Ladd term and the momentum exchange (tick n-1's, which
-// also moves the particles), C the collision.
-enum StepMode { kCollide = 0, kFused = 1, kGather = 2 };
+// also moves the particles), C the collision.  kCollideKeep and kFusedKeep
+// are kCollide and kFused that also keep each fluid node's moments (rho and
+// the velocity of its equilibrium) in StepArgs::hydro: the adjoint's
+// recomputation, whose reverse collision needs nothing else of S_k.
+enum StepMode { kCollide = 0, kFused = 1, kGather = 2, kCollideKeep = 3, kFusedKeep = 4 };
+constexpr int base_mode(int m) { return m == kCollideKeep ? kCollide : (m == kFusedKeep ? kFused : m); }
+constexpr bool keeps_moments(int m) { return m == kCollideKeep || m == kFusedKeep; }
 
 // Collision of a node's populations f (S), into fs (P), with rho.
 // The Guo force at node n (zero without forcing).
@@ -834,14 +847,27 @@ __device__ __forceinline__ void node_force(const StepArgs &a, int64_t n, typenam
 template <int Q, int S, int COLL, int FM>
 __device__ __forceinline__ void collide_node(const StepArgs &a, int64_t n,
                                              const typename Store<S>::R (&f)[Q],
-                                             typename Store<S>::R (&fs)[Q], typename Store<S>::R &rho) {
+                                             typename Store<S>::R (&fs)[Q], typename Store<S>::R &rho,
+                                             typename Store<S>::R *u_out = nullptr) {
   using R = typename Store<S>::R;
   R Fn[3];
   node_force<Q, S, FM>(a, n, Fn);
   if constexpr (S == kF64)
-    collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    collide_ref<Q, COLL, FM>(f, Fn, a.c, fs, rho, u_out);
   else
-    collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho);
+    collide_dev<Q, COLL, FM>(f, Fn, a.c, fs, rho, u_out);
+}
+
+// The moments of node (x, row) into StepArgs::hydro (row-blocked like the
+// populations: plane k of row r is rho, u_x, u_y, u_z for k = 0..3).
+template <int S>
+__device__ __forceinline__ void keep_node_moments(const StepArgs &a, int x, int row, typename Store<S>::R rho,
+                                                  const typename Store<S>::R (&u)[3]) {
+  using R = typename Store<S>::R;
+  R *H = static_cast<R *>(a.hydro) + (int64_t)row * 4 * a.g.nx + x;
+  store_pop(&H[0], rho);
+#pragma unroll
+  for (int k = 0; k < 3; k++) store_pop(&H[(int64_t)(k + 1) * a.g.nx], u[k]);
 }
 
 // The stored value of a population that is zero (wall layers).
@@ -1338,15 +1364,17 @@ __device__ __forceinline__ void box_work(const StepArgs &a, int bid) {
         fg[q] = bbv[q];
       }
     }
-    if constexpr (MODE == kGather) {
+    if constexpr (base_mode(MODE) == kGather) {
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&fout[pop_index(g, Q, q, x, row)], encode<S, Q>(fg[q], q));
     } else {
-      R fs[Q], rho;
-      collide_node<Q, S, COLL, FM>(a, (int64_t)x + g.nx * (int64_t)row, fg, fs, rho);
+      R fs[Q], rho, u[3];
+      collide_node<Q, S, COLL, FM>(a, (int64_t)x + g.nx * (int64_t)row, fg, fs, rho,
+                                   keeps_moments(MODE) ? u : nullptr);
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&fout[pop_index(g, Q, q, x, row)], encode<S, Q>(fs[q], q));
       rho_side[(int64_t)x + g.nx * (int64_t)row] = rho;
+      if constexpr (keeps_moments(MODE)) keep_node_moments<S>(a, x, row, rho, u);
     }
   }
 }
@@ -1365,7 +1393,9 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
   const BoxGeom &g = a.g;
-  if constexpr (MODE != kCollide) {
+  constexpr int M = base_mode(MODE);
+  constexpr bool KEEP = keeps_moments(MODE);
+  if constexpr (M != kCollide) {
     if (b < a.sp.nbox_pad) {
       if (b < a.sp.nbox) box_work<Q, S, COLL, FM, MODE>(a, b);
       return;
@@ -1430,7 +1460,7 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
       if ((g.walled >> k) & 1) ok = ok & (lo[k] >= 2) & (lo[k] + ext[k] - 1 <= nn[k] - 3);
     return ok;
   };
-  if constexpr (MODE == kFused) {
+  if constexpr (M == kFused) {
     if (clear_of_walls()) {
       // The run's repeated kernel away from walls: the node's Q gathers go
       // out first and the particles' regions (scalar, one load each) are
@@ -1461,13 +1491,14 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
         bool near = false;
         for (int p = 0; p < a.sp.np; p++) near = near | tile_touches(a.sp.parts_in[p]);
         if (near) return false;
-        R Fn[3], rho;
+        R Fn[3], rho, u[3];
         node_force<Q, S, FM>(a, (int64_t)x + g.nx * (int64_t)(y + g.ny * z), Fn);
         T out[Q];
-        collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
+        collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho, KEEP ? u : nullptr);
         const I own = rb[1][1] + xb[1];
 #pragma unroll
         for (int q = 0; q < Q; q++) store_pop(reinterpret_cast<T *>(fout + (own + (I)q * nxb)), out[q]);
+        if constexpr (KEEP) keep_node_moments<S>(a, x, y + g.ny * z, rho, u);
         return true;
       };
       if (g.off32 ? fast(uint32_t(0)) : fast(int64_t(0))) return;
@@ -1475,7 +1506,7 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   }
   bool boxed = false;  // owned by the box blocks
   boxed = band_node_of_any();
-  if (MODE != kCollide && boxed) return;  // the box blocks' node
+  if (M != kCollide && boxed) return;  // the box blocks' node
   const int row = y + g.ny * z;
   const int64_t own = (int64_t)row * Q * g.nx + x;
   const int64_t n = (int64_t)x + g.nx * (int64_t)row;
@@ -1491,7 +1522,7 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   }
 
   T raw[Q];
-  if constexpr (MODE == kCollide) {
+  if constexpr (M == kCollide) {
 #pragma unroll
     for (int q = 0; q < Q; q++) raw[q] = fin[own + (int64_t)q * g.nx];
   } else {
@@ -1510,7 +1541,7 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
   }
 
   R *rho_side = static_cast<R *>(a.rho_side);
-  if (MODE != kCollide && (wb & kMovingLinkBit)) {
+  if (M != kCollide && (wb & kMovingLinkBit)) {
     // moving walls: the bounced populations also take the wall's momentum
     // (gpu/fluids.jl:168-174), with rho(x) of the node's last collision
     const R rx = rho_side[n];
@@ -1520,32 +1551,34 @@ __device__ __forceinline__ void step_block(const StepArgs &a, int b) {
       f[q] = decode<S, Q>(raw[q], q);
       if ((wb >> q) & 1u) f[q] = f[q] + moving_wall_term<Q, R>(g, a.c, q, x, y, z, rx);
     }
-    if constexpr (MODE == kGather) {
+    if constexpr (M == kGather) {
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], encode<S, Q>(f[q], q));
     } else {
-      R fs[Q], rho;
-      collide_node<Q, S, COLL, FM>(a, n, f, fs, rho);
+      R fs[Q], rho, u[3];
+      collide_node<Q, S, COLL, FM>(a, n, f, fs, rho, KEEP ? u : nullptr);
 #pragma unroll
       for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], encode<S, Q>(fs[q], q));
       rho_side[n] = rho;
+      if constexpr (KEEP) keep_node_moments<S>(a, x, row, rho, u);
     }
     return;
   }
 
-  if constexpr (MODE == kGather) {
+  if constexpr (M == kGather) {
 #pragma unroll
     for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], raw[q]);
   } else {
-    R Fn[3], rho;
+    R Fn[3], rho, u[3];
     node_force<Q, S, FM>(a, n, Fn);
     T out[Q];
-    collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho);
+    collide_stored<Q, S, COLL, FM>(raw, Fn, a.c, out, rho, KEEP ? u : nullptr);
 #pragma unroll
     for (int q = 0; q < Q; q++) store_pop(&fout[own + (int64_t)q * g.nx], out[q]);
     // rho of the next gather's links: particle links (only the collide-only
     // kernel reaches box nodes) and moving walls
-    if ((MODE == kCollide && boxed) || (wb & kMovingLinkBit)) rho_side[n] = rho;
+    if ((M == kCollide && boxed) || (wb & kMovingLinkBit)) rho_side[n] = rho;
+    if constexpr (KEEP) keep_node_moments<S>(a, x, row, rho, u);
   }
 }
 
@@ -1599,7 +1632,7 @@ batch_step_kernel(BatchArgs ba) {
       hi = mid - 1;
   }
   const StepArgs &a = ba.boxes[ba.seg[lo] >> 1];
-  const int nbox = MODE != kCollide ? a.sp.nbox_pad : 0;
+  const int nbox = base_mode(MODE) != kCollide ? a.sp.nbox_pad : 0;
   const int local = b - ba.start[lo] + ((ba.seg[lo] & 1) ? nbox : 0);
   if (local >= nbox + a.t.ntiles) return;
   step_block<Q, S, COLL, FM, MODE, true>(a, local);

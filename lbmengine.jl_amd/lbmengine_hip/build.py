@@ -2,7 +2,8 @@
 
 The library is compiled from lbmengine.jl_amd/csrc: the step kernels once
 per population storage (FP64, FP32, FP16, custom 16-bit) and part (one
-lattice of D2Q9/D3Q19, batch, one lattice of D3Q15/D3Q27), the
+lattice of D2Q9/D3Q19, batch, one lattice of D3Q15/D3Q27, the D3Q19
+kernels that keep the moments for the adjoint), the
 particle kernels, the adjoint kernels, and the host runtime that implements
 include/lbm_hip.h.
 """
@@ -43,7 +44,7 @@ def build(verbose: bool = False) -> Path:
     OBJ_DIR.mkdir(exist_ok=True)
     jobs = []
     for store, tag in ((0, "f64"), (1, "f32"), (2, "f16"), (3, "f16c")):
-        for part, ptag in ((0, ""), (1, "_batch"), (2, "_q15q27")):
+        for part, ptag in ((0, ""), (1, "_batch"), (2, "_q15q27"), (3, "_keep")):
             jobs.append([HIPCC, *COMMON, "-c", f"-DLBM_STORE={store}", f"-DLBM_PART={part}",
                          str(CSRC / "lbm_kernels.hip"), "-o", str(OBJ_DIR / f"lbm_kernels_{tag}{ptag}.o")])
     jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "lbm_particles.hip"), "-o", str(OBJ_DIR / "lbm_particles.o")])
