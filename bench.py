@@ -50,9 +50,6 @@ sys.path.insert(0, str(ROOT / "lbmengine.jl_amd"))
 
 METRIC = "MLUPS + % HBM peak, D3Q19 TRT squirmer 192^3; FP16-store vs FP64 at err≤1e-3"
 HBM_PEAK_GBS = 8000.0
-# the short run the profiles are taken on (no variants, batch, adjoint or CPU leg)
-PROFILE_ARGS = ("--steps 1000 --warmup 1000 --no-cpu --no-variants --no-configs --block 192,1,1 --batch-steps 200 "
-                "--batch-warmup 100 --batch-accuracy 0 --batch-block 96,2,1 --adjoint-ticks 400 --adjoint-dirs 0")
 BYTES = {"f64": 8, "f32": 4, "f16": 2, "f16c": 2}
 ACC_BOUND = 1e-3  # velocity-field error against FP64 the headline is quoted at
 
@@ -250,11 +247,13 @@ def kernel_times(m, steps):
 
 # Bytes one launch of each kernel moves by the algorithm (read + write), for
 # the workloads this script runs: the step kernel reads and writes every
-# population once; the batched kernel does so for every box; the adjoint
-# kernel reads the tick's populations, reads and writes the adjoint
-# populations (compute type), reads the force field and updates dL/dF (FP64):
-# 19*4*3 + 12 + 48 = 288 B per node in FP32.
-ADJ_N, ADJ_BYTES_PER_NODE, ADJ_R = 96, 288, 10.0
+# population once; the batched kernel does so for every box; the adjoint's
+# bulk and near kernels read each node's moments of the tick (rho and u,
+# 16 B in FP32), read and write the adjoint populations (compute type), read
+# the force field and update dL/dF (FP64): 16 + 19*4*2 + 12 + 48 = 228 B per
+# node in FP32 (the post-collision populations the near kernel reads at
+# particle links add well under 0.1 %).
+ADJ_N, ADJ_BYTES_PER_NODE, ADJ_R = 96, 228, 10.0
 
 
 def adjoint_workload(seed):

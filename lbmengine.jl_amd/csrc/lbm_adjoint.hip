@@ -1,6 +1,6 @@
 // lbm_adjoint.hip — launchers of the adjoint tick's kernels (D3Q19; every
-// storage, collision and force kind): the tick's kernel (near and bulk
-// blocks), the move's reverse and the tick's tail.
+// storage, collision and force kind): the near and bulk kernels, the move's
+// reverse and the tick's tail.
 #include "lbm_adjoint.hpp"
 #include "lbm_adjoint_move.hpp"
 #include "lbm_dispatch.hpp"
@@ -106,30 +106,41 @@ __global__ void __launch_bounds__(kAdjTailThreads) adjoint_tail_kernel(AdjTailAr
     m.phitau[p * 6 + (j - kPState)] = g;
 }
 
-template <int S, int COLL>
+template <int S, int COLL, bool NEAR>
 static hipError_t launch_adjoint_sc(int fm, const AdjArgs &a, const Launch &l) {
+#define LBM_ADJ_LAUNCH(FM)                                                                       \
+  if (NEAR)                                                                                      \
+    hipLaunchKernelGGL((adjoint_near_kernel<19, S, COLL, FM>), l.grid, l.block, 0, l.stream, a); \
+  else                                                                                           \
+    hipLaunchKernelGGL((adjoint_kernel<19, S, COLL, FM>), l.grid, l.block, 0, l.stream, a);
   switch (fm) {
-    case LBM_FORCE_NONE: hipLaunchKernelGGL((adjoint_kernel<19, S, COLL, LBM_FORCE_NONE>), l.grid, l.block, 0, l.stream, a); break;
-    case LBM_FORCE_UNIFORM: hipLaunchKernelGGL((adjoint_kernel<19, S, COLL, LBM_FORCE_UNIFORM>), l.grid, l.block, 0, l.stream, a); break;
-    case LBM_FORCE_FIELD: hipLaunchKernelGGL((adjoint_kernel<19, S, COLL, LBM_FORCE_FIELD>), l.grid, l.block, 0, l.stream, a); break;
+    case LBM_FORCE_NONE: LBM_ADJ_LAUNCH(LBM_FORCE_NONE) break;
+    case LBM_FORCE_UNIFORM: LBM_ADJ_LAUNCH(LBM_FORCE_UNIFORM) break;
+    case LBM_FORCE_FIELD: LBM_ADJ_LAUNCH(LBM_FORCE_FIELD) break;
     default: return hipErrorInvalidValue;
   }
+#undef LBM_ADJ_LAUNCH
   return hipGetLastError();
 }
 
-template <int S>
+template <int S, bool NEAR>
 static hipError_t launch_adjoint_s(int coll, int fm, const AdjArgs &a, const Launch &l) {
-  return coll == LBM_COLLISION_TRT ? launch_adjoint_sc<S, LBM_COLLISION_TRT>(fm, a, l)
-                                   : launch_adjoint_sc<S, LBM_COLLISION_BGK>(fm, a, l);
+  return coll == LBM_COLLISION_TRT ? launch_adjoint_sc<S, LBM_COLLISION_TRT, NEAR>(fm, a, l)
+                                   : launch_adjoint_sc<S, LBM_COLLISION_BGK, NEAR>(fm, a, l);
 }
 
-hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l) {
+template <bool NEAR>
+static hipError_t launch_adjoint_part(int storage, int coll, int fm, const AdjArgs &a, const Launch &l) {
   switch (storage) {
-    case LBM_STORAGE_F64: return launch_adjoint_s<kF64>(coll, fm, a, l);
-    case LBM_STORAGE_F32: return launch_adjoint_s<kF32>(coll, fm, a, l);
-    case LBM_STORAGE_F16C: return launch_adjoint_s<kF16C>(coll, fm, a, l);
-    default: return launch_adjoint_s<kF16>(coll, fm, a, l);
+    case LBM_STORAGE_F64: return launch_adjoint_s<kF64, NEAR>(coll, fm, a, l);
+    case LBM_STORAGE_F32: return launch_adjoint_s<kF32, NEAR>(coll, fm, a, l);
+    case LBM_STORAGE_F16C: return launch_adjoint_s<kF16C, NEAR>(coll, fm, a, l);
+    default: return launch_adjoint_s<kF16, NEAR>(coll, fm, a, l);
   }
+}
+
+hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArgs &a, const Launch &l) {
+  return near ? launch_adjoint_part<true>(storage, coll, fm, a, l) : launch_adjoint_part<false>(storage, coll, fm, a, l);
 }
 
 hipError_t launch_adjoint_tail(const AdjTailArgs &a, hipStream_t stream) {

@@ -147,12 +147,10 @@ struct AdjArgs {
   double coeff, ls;     // momentum-exchange coefficient and lattice speed
   double shell;         // 1.5 dx: link nodes lie within r + shell of a centre
   const double *phitau;  // [particle][6]: dL/dF_p (0..2) and dL/dT_p (3..5) of the tick's move
-  // the near blocks (the first of the grid): particle p's are
-  // [near_first[p], near_first[p + 1]), enough for near_ext[p]^3 nodes of
-  // its near box; the bulk blocks follow
+  // the near kernel's blocks: particle p's are [near_first[p],
+  // near_first[p + 1]), enough for near_ext[p]^3 nodes of its near box
   int32_t near_first[kAdjParticles + 1];
   int32_t near_ext[kAdjParticles];
-  int32_t nblocks_near;
 };
 
 // Near box of a particle: its link box grown by one node (clamped to the
@@ -204,11 +202,7 @@ struct AdjTailArgs {
   AdjFinishArgs fin;
   AdjMoveArgs mv;         // (mv.mu unused: the state comes from fin, in LDS)
 };
-constexpr int kAdjThreads = 256;  // threads per block of the adjoint kernel
-// Waves per SIMD the adjoint kernel is built for: its near blocks' slot
-// phase then fits in 128 registers without spilling, and the bulk blocks,
-// which need 85, keep four waves per SIMD.
-constexpr int kAdjWaves = 4;
+constexpr int kAdjThreads = 256;  // threads per block of the adjoint kernels
 constexpr int kAdjTailThreads = 64 * kAdjParticles;  // one wave per particle's move
 constexpr int kNearSlots = 1024;  // link slots a near block stages in LDS per pass
 
@@ -378,7 +372,7 @@ __device__ __forceinline__ void adj_block_sum(const double (&acc)[kAdjSums], DD 
   __syncthreads();
 }
 
-// One adjoint tick on the particles' near boxes (near block b of particle p
+// One adjoint tick on the particles' near boxes (block b of particle p
 // covers box indices [j 256, (j + 1) 256) of p's near box, j = b -
 // near_first[p], x fastest; a node in the boxes of several particles belongs
 // to the first).  A lane per node finds the node's link slots and the
@@ -602,19 +596,21 @@ __device__ __forceinline__ void adjoint_near_block(const AdjArgs &a, int b) {
   }
 }
 
-// One adjoint tick (dL/dS_k from dL/dS_{k+1}, with dL/dF +=) in one launch:
-// the near blocks (the first nblocks_near of the grid, dispatched first)
-// beside the bulk blocks, so the near blocks' chain of dependent steps runs
-// while the bulk streams.
+// One adjoint tick (dL/dS_k from dL/dS_{k+1}, with dL/dF +=) is two kernels
+// on one stream: the near blocks, then the bulk.  Their register needs
+// differ (about 270 against 85): one build of both would leave the bulk at
+// two waves per SIMD or, capped at 128 registers, spill the near blocks'
+// slot phase (550 B a lane) and make them the launch's critical path.
 template <int Q, int S, int COLL, int FM>
-__global__ void __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(kAdjWaves)))
-adjoint_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs a) {
   static_assert(Q == 19, "the adjoint is built for D3Q19");
-  const int b = blockIdx.x;
-  if (b < a.nblocks_near)
-    adjoint_near_block<Q, S, COLL, FM>(a, b);
-  else
-    adjoint_bulk_block<Q, S, COLL, FM>(a, b - a.nblocks_near);
+  adjoint_bulk_block<Q, S, COLL, FM>(a, blockIdx.x);
+}
+
+template <int Q, int S, int COLL, int FM>
+__global__ void __launch_bounds__(kAdjThreads) adjoint_near_kernel(AdjArgs a) {
+  static_assert(Q == 19, "the adjoint is built for D3Q19");
+  adjoint_near_block<Q, S, COLL, FM>(a, blockIdx.x);
 }
 
 }  // namespace lbm

@@ -2472,7 +2472,6 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.phitau = pt_d;
   for (int p = 0; p <= kAdjParticles; p++) aa.near_first[p] = near_first[p];
   for (int p = 0; p < kAdjParticles; p++) aa.near_ext[p] = near_ext[p];
-  aa.nblocks_near = (int32_t)nblocks_near;
   AdjMoveArgs ma{};
   ma.mu = mu_d;
   ma.mu_in = muin_d;
@@ -2490,10 +2489,10 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   // the two share only the read-only checkpoints.  The recomputation uses
   // the lattice's own working state (populations' spare buffer, records,
   // partials, rho), which the reverse sweep does not touch.
-  // A reverse tick is the adjoint kernel (near and bulk blocks in one
-  // launch) and the tail, on the handle's stream: a further stream can
-  // share a hardware queue with the recomputation's and then waits behind
-  // its kernels.
+  // A reverse tick is the near kernel, the bulk kernel and the tail, in
+  // that order on the handle's stream: a further stream can share a
+  // hardware queue with the recomputation's and then waits behind its
+  // kernels.
   hipStream_t s_rev = h->stream, s_rec = nullptr;
   hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
   constexpr int kEvents = 4;
@@ -2587,7 +2586,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
     h->stream = s_rev;
     return LBM_OK;
   };
-  const Launch l{dim3((unsigned)(nblocks_near + nblocks)), dim3(threads), s_rev};
+  const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
+  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_rev};
   AdjTailArgs ta{};
   ta.partial = partial;
   ta.nblocks = (int32_t)nblocks_near;
@@ -2614,7 +2614,8 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       aa.H = Hk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
-      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, aa, l);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, true, aa, ln);
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
       ta.fin = fa2;
       ta.fin.recs_k = recs_of(c, k - t0);
       ta.fin.direct = k > ws;  // V_k is one of the averaged speeds

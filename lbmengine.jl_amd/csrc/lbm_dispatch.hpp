@@ -42,8 +42,9 @@ hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t strea
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
 
 struct AdjArgs;
-// one reverse tick: the particles' near boxes and the bulk in one launch
-hipError_t launch_adjoint(int storage, int coll, int fm, const AdjArgs &a, const Launch &l);
+// near = false: the bulk kernel (nodes outside the particles' near boxes);
+// true: the near kernel (their nodes, with the particle terms and sums)
+hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArgs &a, const Launch &l);
 struct AdjMoveArgs;
 struct AdjTailArgs;
 hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream);
