@@ -250,10 +250,11 @@ def kernel_times(m, steps):
 # population once; the batched kernel does so for every box; the adjoint's
 # bulk and near kernels read each node's moments of the tick (rho and u,
 # 16 B in FP32), read and write the adjoint populations (compute type), read
-# the force field and update dL/dF (FP64): 16 + 19*4*2 + 12 + 48 = 228 B per
-# node in FP32 (the post-collision populations the near kernel reads at
-# particle links add well under 0.1 %).
-ADJ_N, ADJ_BYTES_PER_NODE, ADJ_R = 96, 228, 10.0
+# the force field and update the segment's dL/dF (compute type): 16 + 19*4*2
+# + 12 + 24 = 204 B per node in FP32 (the post-collision populations the near
+# kernel reads at particle links add well under 0.1 %, the FP64 fold of
+# dL/dF once per segment of 100 ticks 0.2 %).
+ADJ_N, ADJ_BYTES_PER_NODE, ADJ_R = 96, 204, 10.0
 
 
 def adjoint_workload(seed):

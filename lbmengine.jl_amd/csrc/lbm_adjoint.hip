@@ -143,6 +143,20 @@ hipError_t launch_adjoint(int storage, int coll, int fm, bool near, const AdjArg
   return near ? launch_adjoint_part<true>(storage, coll, fm, a, l) : launch_adjoint_part<false>(storage, coll, fm, a, l);
 }
 
+// dst += src, src = 0 (n values): a segment's FP32 sum of dL/dF folded into
+// the FP64 total.
+__global__ void __launch_bounds__(256) adjoint_fold_kernel(float *src, double *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[i] += (double)src[i];
+  src[i] = 0.0f;
+}
+
+hipError_t launch_adjoint_fold(float *src, double *dst, int64_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(adjoint_fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_adjoint_tail(const AdjTailArgs &a, hipStream_t stream) {
   hipLaunchKernelGGL(adjoint_tail_kernel, dim3(1), dim3(kAdjTailThreads), 0, stream, a);
   return hipGetLastError();

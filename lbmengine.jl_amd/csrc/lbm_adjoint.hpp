@@ -136,7 +136,8 @@ struct AdjArgs {
   const void *H;        // the moments of S_k (rho, u; compute type, row-blocked by 4, as the keeping modes write them)
   const void *lam_in;   // dL/dS_{k+1}, compute type, row-blocked
   void *lam_out;        // dL/dS_k
-  double *gF;           // (N, 3) += dL/dF (Julia layout k N + n)
+  void *gF;             // (N, 3) += dL/dF (Julia layout k N + n), compute type: the
+                        // segment's sum for reduced storage, folded into FP64 after it
   const void *Ffield;   // (N, 3) force field (compute type)
   BoxGeom g;
   CollParams c;
@@ -151,6 +152,10 @@ struct AdjArgs {
   // near_first[p + 1]), enough for near_ext[p]^3 nodes of its near box
   int32_t near_first[kAdjParticles + 1];
   int32_t near_ext[kAdjParticles];
+  // the bulk kernel's fast path (no walls, 32-bit offsets) and its
+  // tile_div multipliers for nx and ny
+  int32_t fast;
+  uint32_t mnx, mny;
 };
 
 // Near box of a particle: its link box grown by one node (clamped to the
@@ -262,7 +267,7 @@ __device__ __forceinline__ void adj_store(const AdjArgs &a, int64_t own_idx, int
 #pragma unroll
   for (int q = 0; q < Q; q++) store_pop(&lout[own_idx + (int64_t)q * g.nx], ls_[q] + grho);
   if constexpr (FM == LBM_FORCE_FIELD)
-    for (int k = 0; k < 3; k++) a.gF[k * g.N + n] += (double)gFn[k];
+    for (int k = 0; k < 3; k++) static_cast<R *>(a.gF)[k * g.N + n] += gFn[k];
 }
 
 // One adjoint tick on the bulk: dL/dS_k from dL/dS_{k+1}, with dL/dF +=, at
@@ -273,6 +278,39 @@ template <int Q, int S, int COLL, int FM>
 __device__ __forceinline__ void adjoint_bulk_block(const AdjArgs &a, int b) {
   using R = typename Store<S>::R;
   const BoxGeom &g = a.g;
+  if (a.fast) {
+    // Lattices without walls whose population offsets fit in 32 bits: node
+    // coordinates by multiply-high divisions, the neighbours' rows from nine
+    // row bases (y and z, each -1, 0, +1, periodic), no wall tests.
+    using L = Lattice<Q>;
+    const uint32_t nn = (uint32_t)b * kAdjThreads + threadIdx.x;
+    if (nn >= (uint32_t)g.N) return;
+    const uint32_t r = tile_div(nn, g.nx, a.mnx), zz = tile_div(r, g.ny, a.mny);
+    const int x = (int)(nn - r * (uint32_t)g.nx), y = (int)(r - zz * (uint32_t)g.ny), z = (int)zz;
+    for (int p = 0; p < a.np; p++)
+      if (in_near_box(a.parts[p], g, x, y, z)) return;
+    const uint32_t Qnx = (uint32_t)Q * (uint32_t)g.nx;
+    const uint32_t ya[3] = {(uint32_t)(y == g.ny - 1 ? 0 : y + 1), (uint32_t)y, (uint32_t)(y == 0 ? g.ny - 1 : y - 1)};
+    const uint32_t za[3] = {(uint32_t)(z == g.nz - 1 ? 0 : z + 1), (uint32_t)z, (uint32_t)(z == 0 ? g.nz - 1 : z - 1)};
+    const uint32_t xa[3] = {(uint32_t)(x == g.nx - 1 ? 0 : x + 1), (uint32_t)x, (uint32_t)(x == 0 ? g.nx - 1 : x - 1)};
+    uint32_t rb[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) rb[i][j] = (ya[i] + (uint32_t)g.ny * za[j]) * Qnx;
+    const R *lin = static_cast<const R *>(a.lam_in);
+    R lp[Q];
+    // lambda_S(x + c_q, q): index 1 - c of each axis
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      lp[q] = lin[rb[1 - L::cy[q]][1 - L::cz[q]] + (uint32_t)q * (uint32_t)g.nx + xa[1 - L::cx[q]]];
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    R rho, u[3], Fn[3];
+    adj_moments<Q, S, FM>(a, x, row, (int64_t)(row * (uint32_t)g.nx + (uint32_t)x), rho, u, Fn);
+    adj_store<Q, S, COLL, FM>(a, (int64_t)(row * Qnx + (uint32_t)x), (int64_t)(row * (uint32_t)g.nx + (uint32_t)x), rho,
+                              u, Fn, lp, (R)0);
+    return;
+  }
   const int64_t nn = (int64_t)b * kAdjThreads + threadIdx.x;
   if (nn >= g.N) return;
   const int x = (int)(nn % g.nx);

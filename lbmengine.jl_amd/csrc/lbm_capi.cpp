@@ -1762,12 +1762,13 @@ int upload_table(lbm_batch *b, int s) {
   return LBM_OK;
 }
 
-// Lattices by which a box segment leads its tiles in a batched grid: an
-// eighth of the batch (64 boxes of 48-96 nodes a side with one squirmer
-// each: 0.897 ms per fused kernel with the box blocks right before their
-// tiles, 0.850 ms leading by 4 lattices, 0.846 ms by 8, 0.865 ms by 12,
-// 0.888 ms with every box segment first).
-int batch_lead(int n) { return n / 8; }
+// Lattices by which a box segment leads its tiles in a batched grid: a
+// sixteenth of the batch (64 boxes of 48-96 nodes a side with one squirmer
+// each, row tiles of 192 threads: 0.7505 ms per fused kernel leading by 32
+// lattices, 0.7429 by 16, 0.7364 by 10, 0.7327 by 8, 0.7288 by 5, 0.7272
+// by 4, 0.7436 with each box segment right before its tiles; with one
+// shared 32 x 1 x 2 shape the best lead had been 8).
+int batch_lead(int n) { return n / 16; }
 
 // Grid of a batched kernel over the boxes' arguments `host` (one section):
 // the box blocks of lattice i + lead, then the tiles of lattice i, for i =
@@ -2324,6 +2325,11 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   const size_t hyd_bytes = N * 4 * rsize;  // one tick's moments (rho, u)
   void *lam[2] = {nullptr, nullptr};
   double *gF = nullptr, *partial = nullptr;
+  // reduced storage with a force field: each segment's dL/dF summed in FP32
+  // (half the bytes of an FP64 update per node and tick), folded into gF
+  // after the segment
+  float *gF32 = nullptr;
+  const bool fold = h->cfg.storage != LBM_STORAGE_F64 && h->cfg.force == LBM_FORCE_FIELD;
   DevParticle *recs_seg = nullptr;  // the records of every tick of a segment (two sets)
   DevParticle *ck_recs_d = nullptr;  // the checkpoints' records
   if (buf.alloc(&ck, (size_t)nck * pop_bytes) != hipSuccess ||
@@ -2453,10 +2459,14 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   int a = 0;
   HIP_TRY(hipMemsetAsync(lam[a], 0, N * h->Q * rsize, h->stream), "clear adjoint");
   HIP_TRY(hipMemsetAsync(gF, 0, N * 3 * sizeof(double), h->stream), "clear force gradient");
+  if (fold) {
+    if (buf.alloc(&gF32, N * 3 * sizeof(float)) != hipSuccess) return fail(LBM_ENOMEM, "hipMalloc segment force gradient");
+    HIP_TRY(hipMemsetAsync(gF32, 0, N * 3 * sizeof(float), h->stream), "clear segment force gradient");
+  }
   HIP_TRY(hipMemsetAsync(gB_d, 0, (size_t)np * 2 * sizeof(double), h->stream), "clear B gradients");
   AdjArgs aa;
   std::memset(&aa, 0, sizeof aa);
-  aa.gF = gF;
+  aa.gF = fold ? static_cast<void *>(gF32) : static_cast<void *>(gF);
   aa.Ffield = h->F;
   aa.g = geom(h);
   aa.c = coll(h);
@@ -2472,6 +2482,9 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   aa.phitau = pt_d;
   for (int p = 0; p <= kAdjParticles; p++) aa.near_first[p] = near_first[p];
   for (int p = 0; p < kAdjParticles; p++) aa.near_ext[p] = near_ext[p];
+  aa.fast = aa.g.walled == 0 && (uint64_t)N * h->Q < (uint64_t(1) << 31);
+  aa.mnx = div_magic((uint32_t)h->n[0], (uint64_t)N);
+  aa.mny = div_magic((uint32_t)h->n[1], (uint64_t)N / (uint64_t)h->n[0] + 1);
   AdjMoveArgs ma{};
   ma.mu = mu_d;
   ma.mu_in = muin_d;
@@ -2628,6 +2641,7 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
       if (e != hipSuccess) return hip_fail(e, "adjoint launch");
       a ^= 1;
     }
+    if (fold) HIP_TRY(launch_adjoint_fold(gF32, gF, (int64_t)N * 3, s_rev), "fold force gradient");
     HIP_TRY(hipEventRecord(ev_rev[c & 1], s_rev), "event");
   }
   std::vector<double> gB((size_t)np * 2);

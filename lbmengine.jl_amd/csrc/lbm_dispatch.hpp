@@ -49,5 +49,6 @@ struct AdjMoveArgs;
 struct AdjTailArgs;
 hipError_t launch_adjoint_move(const AdjMoveArgs &a, int np, hipStream_t stream);
 hipError_t launch_adjoint_tail(const AdjTailArgs &a, hipStream_t stream);
+hipError_t launch_adjoint_fold(float *src, double *dst, int64_t n, hipStream_t stream);
 
 }  // namespace lbm

@@ -250,14 +250,14 @@ def test_batch_with_interactions_matches_boxes_run_alone(gpu):
 
 
 def test_box_segments_ahead_of_their_tiles(gpu):
-    """Seventeen boxes: each box's particle blocks are launched two boxes
-    ahead of its tiles (an eighth of the batch), and a box without particles
-    has no such blocks; every box still ends bit-identical to the box
-    advanced alone, for FP32 and FP16 storage and two block shapes."""
+    """Thirty-three boxes: each box's particle blocks are launched two boxes
+    ahead of its tiles (a sixteenth of the batch), and a box without
+    particles has no such blocks; every box still ends bit-identical to the
+    box advanced alone, for FP32 and FP16 storage and two block shapes."""
     rng = np.random.default_rng(7)
     for storage, block in (("f32", (32, 2, 1)), ("f16", (16, 4, 2))):
         pairs = []
-        for i in range(17):
+        for i in range(33):
             n = int(rng.choice([16, 20, 24]))
             lims = dict(xlims=(-n // 2, n // 2 - 1), ylims=(-10, 9), zlims=(-9, 10))
             parts = [] if i == 5 else [sq(float(rng.uniform(2.5, 3.5)), list(rng.uniform(-1, 1, 3)),
