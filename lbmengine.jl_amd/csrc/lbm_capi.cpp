@@ -2502,13 +2502,15 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   // the two share only the read-only checkpoints.  The recomputation uses
   // the lattice's own working state (populations' spare buffer, records,
   // partials, rho), which the reverse sweep does not touch.
-  // A reverse tick is the near kernel, the bulk kernel and the tail, in
-  // that order on the handle's stream: a further stream can share a
-  // hardware queue with the recomputation's and then waits behind its
-  // kernels.
-  hipStream_t s_rev = h->stream, s_rec = nullptr;
-  hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
-  constexpr int kEvents = 4;
+  // Two streams.  The handle's (s_bulk) takes each reverse tick's bulk
+  // kernel and, one kernel per tick behind it, the recomputation of the
+  // next segment; the other (s_near) takes each tick's near kernel and
+  // tail.  Tick k's near kernel waits for tick k + 1's bulk and tick k's
+  // bulk for tick k + 1's near kernel (the adjoint populations and dL/dF
+  // they read and update; events of the tick's parity).
+  hipStream_t s_bulk = h->stream, s_near = nullptr;
+  hipEvent_t ev_rec[2] = {}, ev_b[2] = {}, ev_n[2] = {}, ev_fold = nullptr, ev_segdone = nullptr;
+  constexpr int kEvents = 8;
   struct Streams {
     lbm_handle *h;
     hipStream_t main, *extra;
@@ -2523,16 +2525,22 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
         if (evs[i]) hipEventDestroy(evs[i]);
     }
   };
-  hipEvent_t evs[kEvents] = {nullptr, nullptr, nullptr, nullptr};
-  Streams guard{h, s_rev, &s_rec, evs};
-  HIP_TRY(hipStreamCreate(&s_rec), "create recomputation stream");
+  hipEvent_t evs[kEvents] = {};
+  Streams guard{h, s_bulk, &s_near, evs};
+  HIP_TRY(hipStreamCreate(&s_near), "create near-kernel stream");
   for (int i = 0; i < kEvents; i++) HIP_TRY(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming), "create event");
   ev_rec[0] = evs[0];
   ev_rec[1] = evs[1];
-  ev_rev[0] = evs[2];
-  ev_rev[1] = evs[3];
-  HIP_TRY(hipEventRecord(ev_rev[0], s_rev), "event");  // (both sets free)
-  HIP_TRY(hipEventRecord(ev_rev[1], s_rev), "event");
+  ev_b[0] = evs[2];
+  ev_b[1] = evs[3];
+  ev_n[0] = evs[4];
+  ev_n[1] = evs[5];
+  ev_fold = evs[6];
+  ev_segdone = evs[7];
+  // (every event starts recorded after the set-up above on the handle's stream)
+  for (hipEvent_t ev : {ev_b[0], ev_b[1], ev_n[0], ev_n[1], ev_fold, ev_segdone})
+    HIP_TRY(hipEventRecord(ev, s_bulk), "event");
+  HIP_TRY(hipStreamWaitEvent(s_near, ev_fold, 0), "wait for the set-up");
   // segment c's tick j (0 <= j < its length): P_k = C(S_k) and the
   // moments of S_k in slot j of the segment's set, the records at its start
   // in recs_of(c, j) (and after the segment's last move in recs_of(c, len))
@@ -2541,94 +2549,121 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
   };
   auto Hk = [&](int64_t c, int64_t j) -> void * { return hyd + ((size_t)(c & 1) * seg_len + (size_t)j) * hyd_bytes; };
   auto recs_of = [&](int64_t c, int64_t j) { return recs_seg + ((size_t)(c & 1) * (seg_len + 1) + (size_t)j) * np; };
-  // recompute segment c as a run of its ticks: collide (keeping P_{t0} and
-  // the moments), fused kernels (keeping each P_k and its moments), the last
-  // tick's gather and its move; each gathering kernel writes the records it
-  // streamed with straight into the segment's record slots
-  auto recompute = [&](int64_t c) -> int {
-    const int64_t t0 = c * K, len = std::min(t0 + K, T) - t0;
-    h->stream = s_rec;
-    HIP_TRY(hipStreamWaitEvent(s_rec, ev_rev[c & 1], 0), "wait for the set");
-    int pp = h->pcur;  // the parity of the partials the next gathering kernel reads
-    HIP_TRY(hipMemcpyAsync(parts_of(h, pp), ck_recs_d + (size_t)c * np, np * sizeof(DevParticle),
-                           hipMemcpyDeviceToDevice, s_rec),
-            "restore checkpoint records");
-    auto keep = [&](int mode, const void *fin, void *fout, void *moments, const DevParticle *pin, DevParticle *pout) {
-      StepArgs a = step_args(h, fin, fout, pp);
-      a.hydro = moments;
-      a.sp.parts_in = pin;
-      a.sp.parts_out = pout;
-      const Launch lk = step_launch(h, a, base_mode(mode));
-      switch (h->cfg.storage) {
-        case LBM_STORAGE_F64: return launch_step_keep<kF64>(h->cfg.collision, h->cfg.force, mode, a, lk);
-        case LBM_STORAGE_F32: return launch_step_keep<kF32>(h->cfg.collision, h->cfg.force, mode, a, lk);
-        case LBM_STORAGE_F16C: return launch_step_keep<kF16C>(h->cfg.collision, h->cfg.force, mode, a, lk);
-        default: return launch_step_keep<kF16>(h->cfg.collision, h->cfg.force, mode, a, lk);
-      }
-    };
+  // The recomputation of segment c as a run of its ticks, issued piece by
+  // piece on s_bulk: piece 0 restores the checkpoint's records and collides
+  // (keeping P_{t0} and the moments), pieces 1 .. len - 1 are fused kernels
+  // (keeping each P_k and its moments), piece len the last tick's gather,
+  // piece len + 1 its move, after which ev_rec of the segment's set is
+  // recorded.  Each gathering kernel writes the records it streamed with
+  // straight into the segment's record slots.
+  struct RecState {
+    int64_t c = -1, len = 0, m = 0;
+    int pp = 0;  // the parity of the partials the next gathering kernel reads
+  } rs;
+  auto keep = [&](int mode, const void *fin, void *fout, void *moments, const DevParticle *pin, DevParticle *pout) {
+    StepArgs sa = step_args(h, fin, fout, rs.pp);
+    sa.hydro = moments;
+    sa.sp.parts_in = pin;
+    sa.sp.parts_out = pout;
+    const Launch lk = step_launch(h, sa, base_mode(mode));
+    switch (h->cfg.storage) {
+      case LBM_STORAGE_F64: return launch_step_keep<kF64>(h->cfg.collision, h->cfg.force, mode, sa, lk);
+      case LBM_STORAGE_F32: return launch_step_keep<kF32>(h->cfg.collision, h->cfg.force, mode, sa, lk);
+      case LBM_STORAGE_F16C: return launch_step_keep<kF16C>(h->cfg.collision, h->cfg.force, mode, sa, lk);
+      default: return launch_step_keep<kF16>(h->cfg.collision, h->cfg.force, mode, sa, lk);
+    }
+  };
+  auto rec_piece = [&]() -> int {
+    if (rs.c < 0) return LBM_OK;
+    const int64_t c = rs.c, len = rs.len, m = rs.m;
     // gathering kernel m (1..len) streams tick t0 + m - 1 with the records
     // recs_of(c, m - 1), which it writes (the checkpoint's, for m = 1)
-    auto rin = [&](int64_t m) -> const DevParticle * { return m == 1 ? parts_of(h, pp) : recs_of(c, m - 2); };
-    // (the collide-only kernel has no box blocks and writes no records)
-    hipError_t e = keep(kCollideKeep, ck + (size_t)c * pop_bytes, Pk(c, 0), Hk(c, 0), parts_of(h, pp),
-                        parts_of(h, pp ^ 1));
-    for (int64_t m = 1; m < len && e == hipSuccess; m++) {
+    auto rin = [&](int64_t mm) -> const DevParticle * { return mm == 1 ? parts_of(h, h->pcur) : recs_of(c, mm - 2); };
+    hipError_t e = hipSuccess;
+    if (m == 0) {
+      e = hipMemcpyAsync(parts_of(h, h->pcur), ck_recs_d + (size_t)c * np, np * sizeof(DevParticle),
+                         hipMemcpyDeviceToDevice, s_bulk);
+      // (the collide-only kernel has no box blocks and writes no records)
+      if (e == hipSuccess)
+        e = keep(kCollideKeep, ck + (size_t)c * pop_bytes, Pk(c, 0), Hk(c, 0), parts_of(h, h->pcur),
+                 parts_of(h, h->pcur ^ 1));
+    } else if (m < len) {
       e = keep(kFusedKeep, Pk(c, m - 1), Pk(c, m), Hk(c, m), rin(m), recs_of(c, m - 1));
-      pp ^= 1;
-    }
-    if (e == hipSuccess) {
-      StepArgs a = step_args(h, Pk(c, len - 1), h->f[h->cur ^ 1], pp);
-      a.sp.parts_in = rin(len);
-      a.sp.parts_out = recs_of(c, len - 1);
-      const Launch lg = step_launch(h, a, kGather);
+      rs.pp ^= 1;
+    } else if (m == len) {
+      StepArgs sa = step_args(h, Pk(c, len - 1), h->f[h->cur ^ 1], rs.pp);
+      sa.sp.parts_in = rin(len);
+      sa.sp.parts_out = recs_of(c, len - 1);
+      const Launch lg = step_launch(h, sa, kGather);
       switch (h->cfg.storage) {
-        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
-        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
-        case LBM_STORAGE_F16C: e = launch_step<kF16C>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
-        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, kGather, a, lg); break;
+        case LBM_STORAGE_F64: e = launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, kGather, sa, lg); break;
+        case LBM_STORAGE_F32: e = launch_step<kF32>(h->Q, h->cfg.collision, h->cfg.force, kGather, sa, lg); break;
+        case LBM_STORAGE_F16C: e = launch_step<kF16C>(h->Q, h->cfg.collision, h->cfg.force, kGather, sa, lg); break;
+        default: e = launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, kGather, sa, lg); break;
       }
-      pp ^= 1;
+      rs.pp ^= 1;
+    } else {
+      StepArgs fa = step_args(h, nullptr, nullptr, rs.pp);
+      fa.sp.parts_in = recs_of(c, len - 1);
+      fa.sp.parts_out = recs_of(c, len);
+      e = launch_finalize(fa.sp, s_bulk);
+      if (e == hipSuccess) e = hipEventRecord(ev_rec[c & 1], s_bulk);
+      rs.c = -1;
     }
-    if (e != hipSuccess) return hip_fail(e, "segment step launch");
-    StepArgs fa = step_args(h, nullptr, nullptr, pp);
-    fa.sp.parts_in = recs_of(c, len - 1);
-    fa.sp.parts_out = recs_of(c, len);
-    e = launch_finalize(fa.sp, s_rec);
-    if (e != hipSuccess) return hip_fail(e, "finalize launch");
-    HIP_TRY(hipEventRecord(ev_rec[c & 1], s_rec), "event");
-    h->stream = s_rev;
+    if (e != hipSuccess) return hip_fail(e, "segment recomputation launch");
+    rs.m++;
     return LBM_OK;
   };
-  const Launch l{dim3((unsigned)nblocks), dim3(threads), s_rev};
-  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_rev};
+  auto rec_start = [&](int64_t c) {
+    rs.c = c;
+    rs.len = std::min(c * K + K, T) - c * K;
+    rs.m = 0;
+    rs.pp = h->pcur;
+  };
+  auto rec_finish = [&]() -> int {
+    while (rs.c >= 0)
+      if (const int rc = rec_piece()) return rc;
+    return LBM_OK;
+  };
+  const Launch l{dim3((unsigned)nblocks), dim3(threads), s_bulk};
+  const Launch ln{dim3((unsigned)std::max<int64_t>(nblocks_near, 1)), dim3(threads), s_near};
   AdjTailArgs ta{};
   ta.partial = partial;
   ta.nblocks = (int32_t)nblocks_near;
   ta.mv = ma;
-  if (const int rc = recompute(nck - 1)) return rc;
+  rec_start(nck - 1);
+  if (const int rc = rec_finish()) return rc;
   for (int64_t c = nck - 1; c >= 0; c--) {
     const int64_t t0 = c * K, t1 = std::min(t0 + K, T);
-    HIP_TRY(hipStreamWaitEvent(s_rev, ev_rec[c & 1], 0), "wait for the recomputation");
-    if (c > 0)
-      if (const int rc = recompute(c - 1)) return rc;
+    // segment c's slots (written on s_bulk) and dL/dF folded
+    HIP_TRY(hipStreamWaitEvent(s_near, ev_rec[c & 1], 0), "wait for the recomputation");
+    HIP_TRY(hipStreamWaitEvent(s_near, ev_fold, 0), "wait for the fold");
+    if (c > 0) {
+      // the next recomputation writes the other set, which the near stream
+      // read until segment c + 1's last tail
+      HIP_TRY(hipStreamWaitEvent(s_bulk, ev_segdone, 0), "wait for the near stream");
+      rec_start(c - 1);
+    }
     for (int64_t k = t1 - 1; k >= t0; k--) {
-      // the move's reverse (dL/dF, dL/dT of tick k: on its own at the
-      // segment's last tick, else in the previous tail), the adjoint tick,
-      // then the tail: the particles' adjoint state at the start of tick k
-      // and the reverse of tick k - 1's move when that is in the segment
-      hipError_t e = hipSuccess;
-      if (k == t1 - 1) {
-        ma.recs_k = recs_of(c, k - t0);
-        ma.recs_k1 = recs_of(c, k - t0 + 1);
-        e = launch_adjoint_move(ma, np, s_rev);
-      }
+      // tick k + 1's kernels, recorded in the events of its parity
+      hipError_t e = hipStreamWaitEvent(s_near, ev_b[(k + 1) & 1], 0);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s_bulk, ev_n[(k + 1) & 1], 0);
       aa.parts = recs_of(c, k - t0);
       aa.P = Pk(c, k - t0);
       aa.H = Hk(c, k - t0);
       aa.lam_in = lam[a];
       aa.lam_out = lam[a ^ 1];
+      // near stream: the move's reverse (dL/dF, dL/dT of tick k: on its own
+      // at the segment's last tick, else in the previous tail), the near
+      // kernel, then the tail: the particles' adjoint state at the start of
+      // tick k and the reverse of tick k - 1's move when that is in the segment
+      if (e == hipSuccess && k == t1 - 1) {
+        ma.recs_k = recs_of(c, k - t0);
+        ma.recs_k1 = recs_of(c, k - t0 + 1);
+        e = launch_adjoint_move(ma, np, s_near);
+      }
       if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, true, aa, ln);
-      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
+      if (e == hipSuccess) e = hipEventRecord(ev_n[k & 1], s_near);
       ta.fin = fa2;
       ta.fin.recs_k = recs_of(c, k - t0);
       ta.fin.direct = k > ws;  // V_k is one of the averaged speeds
@@ -2637,13 +2672,23 @@ int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, 
         ta.mv.recs_k = recs_of(c, k - 1 - t0);
         ta.mv.recs_k1 = recs_of(c, k - t0);
       }
-      if (e == hipSuccess) e = launch_adjoint_tail(ta, s_rev);
+      if (e == hipSuccess) e = launch_adjoint_tail(ta, s_near);
+      // bulk stream: the bulk kernel, then a piece of the next recomputation
+      if (e == hipSuccess) e = launch_adjoint(h->cfg.storage, h->cfg.collision, h->cfg.force, false, aa, l);
+      if (e == hipSuccess) e = hipEventRecord(ev_b[k & 1], s_bulk);
       if (e != hipSuccess) return hip_fail(e, "adjoint launch");
+      if (const int rc = rec_piece()) return rc;
       a ^= 1;
     }
-    if (fold) HIP_TRY(launch_adjoint_fold(gF32, gF, (int64_t)N * 3, s_rev), "fold force gradient");
-    HIP_TRY(hipEventRecord(ev_rev[c & 1], s_rev), "event");
+    if (const int rc = rec_finish()) return rc;
+    HIP_TRY(hipEventRecord(ev_segdone, s_near), "event");
+    // the segment's dL/dF, once its last near kernel has added to it
+    HIP_TRY(hipStreamWaitEvent(s_bulk, ev_n[t0 & 1], 0), "wait for the near stream");
+    if (fold) HIP_TRY(launch_adjoint_fold(gF32, gF, (int64_t)N * 3, s_bulk), "fold force gradient");
+    HIP_TRY(hipEventRecord(ev_fold, s_bulk), "event");
   }
+  // the last tail (the B gradients and the particles' adjoint state)
+  HIP_TRY(hipStreamWaitEvent(s_bulk, ev_segdone, 0), "wait for the near stream");
   std::vector<double> gB((size_t)np * 2);
   HIP_TRY(copy_sync(h, gB.data(), gB_d, gB.size() * sizeof(double), hipMemcpyDeviceToHost), "read B gradients");
   std::vector<double> gB1(np), gB2(np);
