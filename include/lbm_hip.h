@@ -420,6 +420,8 @@ int lbm_batch_last_run_times(lbm_batch *b, double *kernel_ms, double *step_ms);
  *   L = (mean over ticks T-window+1..T of |V_swimmer| - target_speed)^2
  *       + force_penalty * sum F^2
  *
+ * with F the model's Guo force field (zero at wall nodes, where the step
+ * zeroes it: gpu/initMethods.jl:636-654; dL/dF is 0 there)
  * from the handle's current state over `ticks` ticks, with respect to every
  * particle's B1 and B2 and the Guo force field.  The forward run saves the
  * populations and particle records every `checkpoint_every` ticks; the
@@ -441,7 +443,8 @@ typedef struct lbm_adjoint_config {
 /* Runs the forward ticks (the handle ends ticks ahead, as after lbm_run) and
  * the reverse sweep.  Outputs (each may be NULL): *loss; dB1, dB2 (one per
  * particle, in id order); dF ((N, 3), Julia layout; requires a force field).
- * Device memory: about (ticks/K + K + 2) population buffers. */
+ * Device memory: about ticks/K + 2.5 K + 4 population buffers (two segment
+ * sets of every tick's post-collision populations and moments). */
 int lbm_adjoint_run(lbm_handle *h, const lbm_adjoint_config *cfg, double *loss, double *dB1, double *dB2,
                     double *dF);
 

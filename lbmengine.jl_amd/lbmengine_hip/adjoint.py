@@ -69,7 +69,8 @@ def swimming_loss(model, *, ticks: int, window: int | None = None, particle: int
                   trajectory: list | None = None) -> float:
     """The same loss by a plain forward run (for finite differences).  With
     `trajectory`, every particle's position before each tick is appended to
-    it (the geometry the tick used)."""
+    it (the geometry the tick used).  `force_field` is the field the penalty
+    is taken on: the model's own, which is zero at wall nodes."""
     w = ticks if window is None else window
     speeds = []
     if trajectory is None:

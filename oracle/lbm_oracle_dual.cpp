@@ -422,7 +422,8 @@ extern "C" {
 
 /* Loss and its derivative along (dB1, dB2, dF) from the populations f0
  * (Q x N, Julia layout) over `ticks` ticks:
- *   L = (mean |V_swimmer| over the last `window` ticks - target)^2 + penalty sum F^2.
+ *   L = (mean |V_swimmer| over the last `window` ticks - target)^2 + penalty sum F^2,
+ * F the model's force field (zero at wall nodes).
  * dB1, dB2: one per particle; F, dF: (d x N) force field and its direction
  * (NULL: none).  speeds (optional, `window` values): the swimmer's speeds. */
 int lbmd_loss_jvp(const lbm_config *cfg, const double *f0, const double *F, int32_t np, const lbm_particle *parts,
@@ -474,8 +475,9 @@ int lbmd_loss_jvp(const lbm_config *cfg, const double *f0, const double *F, int3
         else if (F)
           m.F[i] = m.wall[x] ? D(0.0) : D(F[i], dF ? dF[i] : 0.0);
       }
+    // the penalty is on the model's field, which is zero at wall nodes
     if (cfg->force == LBM_FORCE_FIELD && F)
-      for (size_t i = 0; i < m.F.size(); i++) Fsq = Fsq + D(F[i], dF ? dF[i] : 0.0) * D(F[i], dF ? dF[i] : 0.0);
+      for (size_t i = 0; i < m.F.size(); i++) Fsq = Fsq + m.F[i] * m.F[i];
   }
   hydro_update(m);
   for (int p = 0; p < np; p++) {

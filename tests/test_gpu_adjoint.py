@@ -162,6 +162,55 @@ def test_two_particles_against_the_dual_number_oracle(gpu, oracle, sep):
         assert abs(ad - dl) <= 1e-8 * abs(dl), (a, b, ad, dl)
 
 
+def test_walled_lattice_against_the_dual_number_oracle(gpu, oracle):
+    """A lattice with wall layers at z = 0 and z = n - 1 (the bulk kernel's
+    general path: wall tests, bounce-back of the adjoint populations): FP64
+    adjoint against the dual-number oracle along B1, B2, a force-field
+    direction and a mixed one; FP32 storage (dL/dF summed per segment in
+    FP32, folded into FP64) against the FP64 adjoint's directional
+    derivatives within 1e-3."""
+    from lbmengine_hip import adjoint
+    from lbmengine_hip.model import resolve_particle
+    n, r, T, W, target, pen = 18, 3.4, 14, 6, 1e-4, 1e3
+    rng = np.random.default_rng(41)
+    F = rng.normal(0.0, 1e-6, (n, n, n, 3))
+    pos, e = [0.3, -0.4, 0.2], [0.36, 0.48, 0.8]
+    B1, B2 = 1e-3, -1.5e-3
+
+    def walled(storage):
+        m = gpu.modelInit(**KW3, xlims=(-n // 2, n // 2 - 1), walledDimensions=[3], storage=storage,
+                          forceDensity=np.asfortranarray(F))
+        gpu.addSquirmer(m, radius=r, position=pos, swimmingSpeed=2 * B1 / 3, beta=B2 / B1, swimmingDirection=e,
+                        coupleForces=True, coupleTorques=True)
+        return m
+
+    m = walled("f64")
+    f0 = m.distributions
+    p = resolve_particle(m.resolved, kind="squirmer", radius=r, position=pos, swimmingSpeed=2 * B1 / 3,
+                         beta=B2 / B1, swimmingDirection=e, coupleForces=True, coupleTorques=True)
+    g = adjoint.swimming_loss_gradient(m, ticks=T, checkpoint_every=5, window=W, target_speed=target,
+                                       force_penalty=pen)
+    Fj = np.asfortranarray(F).reshape(-1, order="F")
+    gF = np.asfortranarray(g.dF).reshape(-1, order="F")
+    dFd = rng.standard_normal(Fj.shape) * 1e-6
+    dirs = ((B1, 0.0, None), (0.0, B2, None), (0.0, 0.0, dFd), (0.4 * B1, -0.3 * B2, dFd))
+    for a, b, dF in dirs:
+        L, dl, _ = oracle.dual_loss_jvp(m.resolved, [p], f0, ticks=T, window=W, F=Fj, dB1=[a], dB2=[b], dF=dF,
+                                        target=target, penalty=pen)
+        ad = g.dB1[0] * a + g.dB2[0] * b + (0.0 if dF is None else float(np.dot(gF, dF)))
+        assert abs(L - g.loss) <= 1e-12 * L
+        assert abs(ad - dl) <= 1e-8 * abs(dl), (a, b, ad, dl)
+    m32 = walled("f32")
+    m32.distributions = f0
+    g32 = adjoint.swimming_loss_gradient(m32, ticks=T, checkpoint_every=5, window=W, target_speed=target,
+                                         force_penalty=pen)
+    gF32 = np.asfortranarray(g32.dF).reshape(-1, order="F")
+    for a, b, dF in dirs:
+        ad = g.dB1[0] * a + g.dB2[0] * b + (0.0 if dF is None else float(np.dot(gF, dF)))
+        ad32 = g32.dB1[0] * a + g32.dB2[0] * b + (0.0 if dF is None else float(np.dot(gF32, dF)))
+        assert abs(ad32 - ad) <= 1e-3 * abs(ad), (a, b, ad32, ad)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(gpu):
     from lbmengine_hip import adjoint
     F, pos, e = setup(20, 4.5, 9)
