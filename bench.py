@@ -507,6 +507,9 @@ def main():
     # achievable ceiling: a plain device copy moving the same bytes per launch
     copy_ms = lh._capi.measure_copy_ms(bytes_per_launch // 2, 10)
     copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
+    # the ceiling of the step's access pattern: its gathers and stores alone
+    pull_ms = lh._capi.measure_pull_ms(lh._capi.STORAGE[args.storage], args.n, args.n, args.n, 10)
+    pull_gbs = bytes_per_launch / (pull_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
@@ -525,7 +528,9 @@ def main():
                      "bytes_per_launch": bytes_per_launch,
                      "frac_wall_clock": round(bytes_per_launch / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                      "copy_ceiling_GBs": round(copy_gbs, 1), "copy_ms": round(copy_ms, 5),
-                     "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
+                     "pull_ceiling_GBs": round(pull_gbs, 1), "pull_ms": round(pull_ms, 5),
+                     "frac_of_pull": round(achieved / pull_gbs, 4)},
     }
     del m
 
@@ -534,15 +539,18 @@ def main():
         for st in ("f64", "f32", "f16", "f16c"):
             if st == args.storage:
                 var[st] = {"MLUPS": out["value"], "kernel_ms": out["roofline"]["kernel_ms"],
-                           "hbm_frac": out["roofline"]["frac"]}
+                           "hbm_frac": out["roofline"]["frac"], "pull_ms": out["roofline"]["pull_ms"],
+                           "frac_of_pull": out["roofline"]["frac_of_pull"]}
                 continue
             mv = build_model(lh, st, args)
             mv.autotune(10)
             ks = max(args.steps // 4, 100)
             w = run_wall(mv, ks, args.warmup // 4)
             k, _ = kernel_times(mv, min(ks, 500))
+            pm = lh._capi.measure_pull_ms(lh._capi.STORAGE[st], args.n, args.n, args.n, 10)
             var[st] = {"MLUPS": round(nodes * ks / w / 1e6, 1), "kernel_ms": round(k, 5),
                        "hbm_frac": round(2 * Q * BYTES[st] * nodes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "pull_ms": round(pm, 5), "frac_of_pull": round(pm / k, 4),
                        "block": list(mv.get_block()), "tile_order": mv.get_tile_order(),
                        "low_occupancy": mv.get_occupancy()}
             del mv

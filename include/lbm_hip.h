@@ -358,6 +358,14 @@ int lbm_last_run_times(lbm_handle *h, double *kernel_ms, double *step_ms);
  * profiler byte counters.  No counterpart in the reference. */
 int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms);
 
+/* Measurement: average time (ms) of the D3Q19 step's data movement alone on
+ * a periodic nx x ny x nz lattice of the given storage, over `reps` launches:
+ * every node pulls its 19 populations f_q(x - c_q) from the row-blocked
+ * layout and stores them unchanged, one node per lane, as the step kernels'
+ * tiles do, with no collision — the ceiling of the step's access pattern.
+ * No counterpart in the reference. */
+int lbm_measure_pull(int32_t storage, int32_t nx, int32_t ny, int32_t nz, int32_t reps, double *ms);
+
 /* ------------------------------------------------------------ batches
  * A batch advances up to 256 independent boxes (handles made by lbm_create,
  * of one lattice, storage, collision and force kind; any sizes, parameters,

@@ -1601,6 +1601,40 @@ int lbm_measure_copy(int64_t bytes, int32_t reps, double *ms) {
   return LBM_OK;
 }
 
+int lbm_measure_pull(int32_t storage, int32_t nx, int32_t ny, int32_t nz, int32_t reps, double *ms) {
+  if (!ms || reps < 1 || nx < 1 || ny < 1 || nz < 1) return fail(LBM_EINVAL, "bad lattice or repetitions");
+  if (storage < LBM_STORAGE_F64 || storage > LBM_STORAGE_F16C) return fail(LBM_EINVAL, "unknown storage");
+  const int eb = (int)elem_size(storage);
+  const int64_t bytes = (int64_t)nx * ny * nz * 19 * eb;
+  if ((int64_t)ny * nz * ((nx + 255) / 256) > INT32_MAX) return fail(LBM_EINVAL, "lattice too large");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_pull(eb, a, b, nx, ny, nz, s);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  // back and forth, as the step's populations go
+  for (int r = 0; r < reps && e == hipSuccess; r++)
+    e = (r & 1) ? launch_pull(eb, b, a, nx, ny, nz, s) : launch_pull(eb, a, b, nx, ny, nz, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  if (e != hipSuccess) return hip_fail(e, "pull measurement");
+  *ms = t / reps;
+  return LBM_OK;
+}
+
 // Current block shape of the step kernel.
 int lbm_get_block(lbm_handle *h, int32_t *bx, int32_t *by, int32_t *bz) {
   if (!h) return fail(LBM_EINVAL, "null handle");

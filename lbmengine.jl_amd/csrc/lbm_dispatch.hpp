@@ -40,6 +40,8 @@ hipError_t launch_finalize(const StepParticles &sp, hipStream_t stream);
 hipError_t launch_move(const StepParticles &sp, hipStream_t stream);
 hipError_t launch_finalize_batch(const StepArgs *boxes, int n, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t stream);
+// the D3Q19 step's gathers and stores alone (elements of 2, 4 or 8 bytes)
+hipError_t launch_pull(int elem_bytes, const void *src, void *dst, int nx, int ny, int nz, hipStream_t stream);
 
 struct AdjArgs;
 // near = false: the bulk kernel (nodes outside the particles' near boxes);

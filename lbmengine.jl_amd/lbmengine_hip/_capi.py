@@ -118,6 +118,7 @@ SIGNATURES = {
     "lbm_last_run_ms": [_vp, _dp],
     "lbm_last_run_times": [_vp, _dp, _dp],
     "lbm_measure_copy": [C.c_int64, C.c_int32, _dp],
+    "lbm_measure_pull": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp],
     "lbm_autotune": [_vp, C.c_int32, C.POINTER(C.c_int32), _dp],
     "lbm_set_graph": [_vp, C.c_int32],
     "lbm_batch_create": [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)],
@@ -188,6 +189,14 @@ def measure_copy_ms(nbytes: int, reps: int = 10) -> float:
     """Average ms of a plain device copy of nbytes (read nbytes, write nbytes)."""
     ms = C.c_double(0.0)
     check(load()["lbm_measure_copy"](int(nbytes) // 16 * 16, int(reps), C.byref(ms)))
+    return ms.value
+
+
+def measure_pull_ms(storage: int, nx: int, ny: int, nz: int, reps: int = 10) -> float:
+    """Average ms of the D3Q19 step's gathers and stores alone (no collision)
+    on a periodic nx x ny x nz lattice of the given storage."""
+    ms = C.c_double(0.0)
+    check(load()["lbm_measure_pull"](int(storage), int(nx), int(ny), int(nz), int(reps), C.byref(ms)))
     return ms.value
 
 

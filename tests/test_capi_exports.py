@@ -51,3 +51,15 @@ def test_bad_configuration_is_rejected_without_device(hiplib):
     cfg.flags = 4  # no such flag
     assert fn["lbm_create"](ctypes.byref(cfg), ctypes.byref(h)) == _capi.LBM_EINVAL
     assert b"flags" in fn["lbm_last_error"]()
+
+
+def test_measurements_reject_bad_arguments(hiplib):
+    """The copy and pull measurements check their arguments before any
+    device call."""
+    from lbmengine_hip import _capi
+    fn = _capi.load()
+    ms = ctypes.c_double()
+    assert fn["lbm_measure_copy"](17, 1, ctypes.byref(ms)) == _capi.LBM_EINVAL
+    assert fn["lbm_measure_pull"](7, 8, 8, 8, 1, ctypes.byref(ms)) == _capi.LBM_EINVAL
+    assert fn["lbm_measure_pull"](3, 0, 8, 8, 1, ctypes.byref(ms)) == _capi.LBM_EINVAL
+    assert fn["lbm_measure_pull"](3, 8, 8, 8, 0, ctypes.byref(ms)) == _capi.LBM_EINVAL
