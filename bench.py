@@ -285,7 +285,7 @@ def adjoint_near_nodes(seed):
 def _kernel_family(name):
     """Short key of a traced kernel: the fused step kernel of a storage (either
     occupancy build), the fused batched kernel, the adjoint's bulk and near
-    kernels or the copy kernel."""
+    kernels, the copy kernel or the pull kernel of an element type."""
     n = name.replace(" ", "")
     if n.startswith("void"):
         n = n[4:]
@@ -301,6 +301,8 @@ def _kernel_family(name):
         return "adjoint_near_kernel"
     if n.startswith("lbm::adjoint_kernel<"):
         return "adjoint_kernel"
+    if n.startswith("lbm::pull_kernel<") and targs:
+        return "pull_kernel/" + targs[0]
     return None
 
 
@@ -336,6 +338,9 @@ def traffic_summary(fetch_dir, write_dir, args, argv):
     # (only the headline storage's step kernel: the others in a run belong
     # to other workloads, such as the adjoint's 96^3 forward steps)
     algo[f"step_kernel/{args.storage}"] = 2 * 19 * BYTES[args.storage] * args.n ** 3
+    # the headline's pull (the step's gathers and stores alone, same bytes)
+    pull_t = {2: "unsignedshort", 4: "unsignedint", 8: "unsignedlong"}[BYTES[args.storage]]
+    algo[f"pull_kernel/{pull_t}"] = 2 * 19 * BYTES[args.storage] * args.n ** 3
     made_by = [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o pmc --output-format csv -- python3 bench.py {argv}",
                f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o pmc --output-format csv -- python3 bench.py {argv}",
                f"python3 bench.py --traffic-from {fetch_dir} {write_dir} {argv}"]
@@ -470,6 +475,8 @@ def main():
         names = {"batch_step_kernel": "traffic_batch_64_boxes.json", "adjoint_kernel": "traffic_adjoint_96.json",
                  "adjoint_near_kernel": "traffic_adjoint_near_96.json"}
         for fam, rec in out.items():
+            if fam.startswith("pull_kernel/"):
+                names[fam] = f"traffic_pull_{args.storage}_{args.n}.json"
             dst = ROOT / "profiles" / (names[fam] if fam in names else f"traffic_step_{fam.split('/')[1]}_{args.n}.json")
             dst.write_text(json.dumps(rec, indent=1) + "\n")
         print(json.dumps(out))
