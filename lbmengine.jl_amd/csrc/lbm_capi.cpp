@@ -344,8 +344,7 @@ Launch step_launch(lbm_handle *h, const StepArgs &a, int mode) {
   return l;
 }
 
-hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int pin) {
-  StepArgs a = step_args(h, fin, fout, pin);
+hipError_t launch_args(lbm_handle *h, int mode, const StepArgs &a) {
   Launch l = step_launch(h, a, mode);
   switch (h->cfg.storage) {
     case LBM_STORAGE_F64: return launch_step<kF64>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
@@ -353,6 +352,10 @@ hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int
     case LBM_STORAGE_F16C: return launch_step<kF16C>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
     default: return launch_step<kF16>(h->Q, h->cfg.collision, h->cfg.force, mode, a, l);
   }
+}
+
+hipError_t launch_mode(lbm_handle *h, int mode, const void *fin, void *fout, int pin) {
+  return launch_args(h, mode, step_args(h, fin, fout, pin));
 }
 
 // The move kernel after a gathering kernel that read records `pin` (more
@@ -1491,6 +1494,18 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
           cands.push_back({bx, by, bz, 0, occ});
         }
       }
+  // One-row tiles wider than the lattice: the lanes past the row's end
+  // return at once (whole waves of them without issuing a load), and the
+  // box blocks keep the block's thread count (96^3 FP32 with a force field
+  // and a squirmer: 192 x 1 x 1 runs a tick in 29.8 us, the best shape that
+  // fits the rows 32.6 us)
+  for (int32_t bx = (h->n[0] + 63) / 64 * 64, k = 0; k < 2 && bx <= 512; bx += 64, k++) {
+    if (bx == h->n[0] || h->n[0] < 16) continue;
+    for (int occ = 0; occ < nocc; occ++) {
+      cands.push_back({bx, 1, 1, 1, occ});
+      cands.push_back({bx, 1, 1, 0, occ});
+    }
+  }
   if (cands.empty()) return fail(LBM_EINVAL, "no block shape fits this lattice");
   // The trial kernels step a copy of the populations back and forth between
   // two scratch buffers, as a run does: stepping the same input again and
@@ -1504,6 +1519,31 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     hipFree(scratch);
     return hip_fail(ce, "autotune copy");
   }
+  // The trials move the particles as a run does: each trial kernel applies
+  // the move the one before it left pending, so the box blocks' whole chain
+  // is timed.  Their records alternate parity from the handle's own, which
+  // are kept aside and put back before every candidate and at the end; the
+  // trials write nothing to the per-tick log.  (With more than
+  // kLocalParticles particles the moves belong to the move kernel, and every
+  // trial reads the handle's records.)
+  const size_t rec_bytes = h->parts_h.size() * sizeof(DevParticle);
+  const bool moving = rec_bytes > 0 && !many_particles(h);
+  DevParticle *recs_kept = nullptr;
+  if (moving) {
+    hipError_t ce = hipMalloc(&recs_kept, rec_bytes);
+    if (ce == hipSuccess)
+      ce = hipMemcpyAsync(recs_kept, parts_of(h, h->pcur), rec_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (ce != hipSuccess) {
+      if (recs_kept) hipFree(recs_kept);
+      hipFree(scratch);
+      return hip_fail(ce, "autotune record copy");
+    }
+  }
+  auto put_back = [&]() {
+    hipError_t ce = hipSuccess;
+    if (moving) ce = hipMemcpyAsync(parts_of(h, h->pcur), recs_kept, rec_bytes, hipMemcpyDeviceToDevice, h->stream);
+    return ce;
+  };
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -1511,9 +1551,14 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   double best_t = 1e300;
   Cand best_c = cands[0];
   hipError_t e = hipSuccess;
+  auto release = [&]() {
+    hipStreamSynchronize(h->stream);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (recs_kept) hipFree(recs_kept);
+    hipFree(scratch);
+  };
   for (const Cand &c : cands) {
-    // the trial steps also move the particles, into the records of the other
-    // parity, which the next tick rewrites
     Tiling t = make_tiling(h, c.bx, c.by, c.bz);
     t.remap = c.remap;
     t.occ = c.occ;
@@ -1521,19 +1566,22 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
     if (rc) {
       h->tiling = saved;
       plan_boxes(h, tile_threads(saved));
-      hipEventDestroy(e0);
-      hipEventDestroy(e1);
-      hipFree(scratch);
+      put_back();
+      release();
       return rc;
     }
     h->tiling = t;
     // fused kernels between the two scratch buffers (an even count of
-    // warm-up steps, so the timed ones start from the copy's buffer)
+    // warm-up steps, so the timed ones start from the copy's buffer), the
+    // first from the handle's records
+    e = put_back();
     int side = 0;
     auto trial = [&]() {
       char *from = scratch + side * bytes;
+      StepArgs a = step_args(h, from, scratch + (side ^ 1) * bytes, moving ? h->pcur ^ side : h->pcur);
+      a.sp.traj = nullptr;
       side ^= 1;
-      return launch_mode(h, kFused, from, scratch + side * bytes, h->pcur);
+      return launch_args(h, kFused, a);
     };
     for (int r = 0; r < 2 && e == hipSuccess; r++) e = trial();  // warm-up
     if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
@@ -1548,9 +1596,9 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
       best_c = c;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(scratch);
+  if (e == hipSuccess) e = put_back();
+  else put_back();
+  release();
   if (e != hipSuccess) {
     h->tiling = saved;
     plan_boxes(h, tile_threads(saved));
@@ -2219,6 +2267,36 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
       return hip_fail(ce, "batch autotune copy");
     }
   }
+  // The trials move the particles as runs do (records alternating parity
+  // from each box's own, kept aside and put back before every candidate and
+  // at the end; nothing goes to the per-tick logs), so every box's whole
+  // chain of box steps, the pending move included, is timed.
+  std::vector<size_t> rec_off(n + 1, 0);
+  for (int i = 0; i < n; i++) rec_off[i + 1] = rec_off[i] + b->boxes[i]->parts_h.size();
+  DevParticle *recs_kept = nullptr;
+  if (rec_off[n] > 0) {
+    hipError_t ce = hipMalloc(&recs_kept, rec_off[n] * sizeof(DevParticle));
+    for (int i = 0; i < n && ce == hipSuccess; i++) {
+      lbm_handle *h = b->boxes[i];
+      const size_t nb = h->parts_h.size() * sizeof(DevParticle);
+      if (nb) ce = hipMemcpyAsync(recs_kept + rec_off[i], parts_of(h, h->pcur), nb, hipMemcpyDeviceToDevice, b->stream);
+    }
+    if (ce != hipSuccess) {
+      if (recs_kept) hipFree(recs_kept);
+      hipFree(trial_d);
+      hipFree(scratch);
+      return hip_fail(ce, "batch autotune record copy");
+    }
+  }
+  auto put_back = [&]() {
+    hipError_t ce = hipSuccess;
+    for (int i = 0; i < n && ce == hipSuccess; i++) {
+      lbm_handle *h = b->boxes[i];
+      const size_t nb = h->parts_h.size() * sizeof(DevParticle);
+      if (nb) ce = hipMemcpyAsync(parts_of(h, h->pcur), recs_kept + rec_off[i], nb, hipMemcpyDeviceToDevice, b->stream);
+    }
+    return ce;
+  };
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -2238,9 +2316,12 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
       lbm_handle *h = b->boxes[i];
       char *s0 = scratch + off[i], *s1 = s0 + scratch_bytes;
       trial[i] = step_args(h, s0, s1, h->pcur);
-      trial[n + i] = step_args(h, s1, s0, h->pcur);
+      trial[n + i] = step_args(h, s1, s0, h->pcur ^ 1);
+      trial[i].sp.traj = nullptr;
+      trial[n + i].sp.traj = nullptr;
     }
     e = hipMemcpyAsync(trial_d, trial.data(), 2 * (size_t)n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = put_back();
     // (an even count of warm-up steps, so the timed ones start from buffer 0)
     int side = 0;
     auto step = [&]() {
@@ -2261,9 +2342,11 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
       best_c = c;
     }
   }
+  if (const hipError_t pe = put_back(); e == hipSuccess) e = pe;
   hipStreamSynchronize(b->stream);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (recs_kept) hipFree(recs_kept);
   hipFree(trial_d);
   hipFree(scratch);
   if (!rc && e != hipSuccess) rc = hip_fail(e, "batch autotune");

@@ -119,3 +119,54 @@ def test_low_occupancy_adjoint_is_bit_identical(gpu):
     assert a.loss == b.loss
     assert np.array_equal(a.dB1, b.dB1) and np.array_equal(a.dB2, b.dB2)
     assert np.array_equal(a.dF, b.dF)
+
+
+def test_autotune_leaves_the_state_as_it_was(gpu):
+    """The autotuner's trials step copies of the populations and move the
+    particles through both record parities: afterwards a run from the tuned
+    handle equals a run of a twin given the same shape, bit for bit, and the
+    per-tick log holds only the run's ticks."""
+    from lbmengine_hip import _capi
+    a = squirmer_model("f32")
+    b = squirmer_model("f32")
+    for m in (a, b):
+        _capi.check(m._fn["lbm_set_trajectory_log"](m._h, 64))
+        m.run(7)
+    a.autotune(3)
+    b.set_block(*a.get_block())
+    b.set_tile_order(a.get_tile_order())
+    b.set_occupancy(a.get_occupancy())
+    a.run(30)
+    b.run(30)
+    fa, pa = state(a)
+    fb, pb = state(b)
+    assert np.array_equal(fa, fb) and np.array_equal(pa, pb)
+    la, lb = np.zeros((37, 9)), np.zeros((37, 9))
+    _capi.check(a._fn["lbm_get_trajectory"](a._h, 0, 1, 37, _capi.as_dptr(la)))
+    _capi.check(b._fn["lbm_get_trajectory"](b._h, 0, 1, 37, _capi.as_dptr(lb)))
+    assert np.array_equal(la, lb)
+
+
+def test_batch_autotune_leaves_the_state_as_it_was(gpu):
+    def boxes():
+        return [squirmer_model("f32", n=n, r=r, seed=s) for n, r, s in ((32, 5.0, 1), (40, 6.0, 2))]
+    xa, xb = boxes(), boxes()
+    ba, bb = lh.LBMbatch(xa), lh.LBMbatch(xb)
+    ba.run(5)
+    bb.run(5)
+    ba.autotune(2)
+    rows = ba.get_row_tiles()
+    if rows:
+        bb.set_row_tiles(rows)
+    else:
+        bb.set_block(*ba.get_block())
+    bb.set_tile_order(ba.get_tile_order())
+    bb.set_occupancy(ba.get_occupancy())
+    ba.run(20)
+    bb.run(20)
+    ba.sync()
+    bb.sync()
+    for ma, mb in zip(xa, xb):
+        fa, pa = state(ma)
+        fb, pb = state(mb)
+        assert np.array_equal(fa, fb) and np.array_equal(pa, pb)
