@@ -300,6 +300,18 @@ int lbm_sync(lbm_handle *h);
 /* model.tick and model.time. */
 int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time);
 
+/* State snapshots, to resume a run: lbm_save_state writes the populations in
+ * the storage's own encoding, every particle's state record (kinematics, the
+ * node-velocity snapshot, the move count) and the tick and time into `buf`
+ * (lbm_state_size bytes); lbm_load_state puts them into a handle created
+ * with the same configuration and particles, which then runs on bit for bit
+ * as the saved one would have (LBM_EINVAL for another lattice, storage or
+ * particle set; the per-tick log then refuses the ticks before the restored
+ * one).  No counterpart in the reference, which cannot resume a run. */
+int lbm_state_size(lbm_handle *h, int64_t *bytes);
+int lbm_save_state(lbm_handle *h, void *buf, int64_t bytes);
+int lbm_load_state(lbm_handle *h, const void *buf, int64_t bytes);
+
 /* Block shape of the step kernel (threads along x, y, z; product 64..512, a
  * multiple of 64), normally chosen by the autotuner; 0 resets to the
  * default. */

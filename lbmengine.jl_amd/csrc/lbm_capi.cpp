@@ -1405,6 +1405,88 @@ int lbm_get_tick(lbm_handle *h, int64_t *tick, double *time) {
   return LBM_OK;
 }
 
+// Snapshot layout: StateHeader, the populations of the current buffer in the
+// storage's own encoding and the row-blocked layout, then the particles'
+// device records of the current parity.
+struct StateHeader {
+  char magic[8];  // "LBMSTAT1"
+  int32_t Q, storage, np, record_bytes;
+  int64_t N, pop_bytes, tick, tick0;
+  double time;
+};
+constexpr char kStateMagic[8] = {'L', 'B', 'M', 'S', 'T', 'A', 'T', '1'};
+
+static int64_t state_bytes(const lbm_handle *h) {
+  return (int64_t)sizeof(StateHeader) + (int64_t)h->N * h->Q * (int64_t)elem_size(h->cfg.storage) +
+         (int64_t)h->parts_h.size() * (int64_t)sizeof(DevParticle);
+}
+
+int lbm_state_size(lbm_handle *h, int64_t *bytes) {
+  if (!h || !bytes) return fail(LBM_EINVAL, "null argument");
+  *bytes = state_bytes(h);
+  return LBM_OK;
+}
+
+int lbm_save_state(lbm_handle *h, void *buf, int64_t bytes) {
+  if (!h || !buf) return fail(LBM_EINVAL, "null argument");
+  if (bytes < state_bytes(h)) return fail(LBM_EINVAL, "state buffer too small (see lbm_state_size)");
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  StateHeader hd{};
+  std::memcpy(hd.magic, kStateMagic, sizeof hd.magic);
+  hd.Q = h->Q;
+  hd.storage = h->cfg.storage;
+  hd.np = (int32_t)h->parts_h.size();
+  hd.record_bytes = (int32_t)sizeof(DevParticle);
+  hd.N = h->N;
+  hd.pop_bytes = (int64_t)h->N * h->Q * (int64_t)elem_size(h->cfg.storage);
+  hd.tick = h->tick;
+  hd.tick0 = h->tick0;
+  hd.time = h->time;
+  char *out = static_cast<char *>(buf);
+  std::memcpy(out, &hd, sizeof hd);
+  HIP_TRY(hipMemcpy(out + sizeof hd, h->f[h->cur], (size_t)hd.pop_bytes, hipMemcpyDeviceToHost), "copy populations");
+  if (hd.np)
+    HIP_TRY(hipMemcpy(out + sizeof hd + hd.pop_bytes, parts_of(h, h->pcur), (size_t)hd.np * sizeof(DevParticle),
+                      hipMemcpyDeviceToHost),
+            "copy particles");
+  return LBM_OK;
+}
+
+int lbm_load_state(lbm_handle *h, const void *buf, int64_t bytes) {
+  if (!h || !buf) return fail(LBM_EINVAL, "null argument");
+  if (bytes < (int64_t)sizeof(StateHeader)) return fail(LBM_EINVAL, "not a state snapshot");
+  StateHeader hd;
+  const char *in = static_cast<const char *>(buf);
+  std::memcpy(&hd, in, sizeof hd);
+  if (std::memcmp(hd.magic, kStateMagic, sizeof hd.magic) != 0) return fail(LBM_EINVAL, "not a state snapshot");
+  const int64_t pop_bytes = (int64_t)h->N * h->Q * (int64_t)elem_size(h->cfg.storage);
+  if (hd.Q != h->Q || hd.storage != h->cfg.storage || hd.N != h->N || hd.pop_bytes != pop_bytes ||
+      hd.record_bytes != (int32_t)sizeof(DevParticle) || hd.np != (int32_t)h->parts_h.size())
+    return fail(LBM_EINVAL, "the snapshot was taken of a lattice or particle set other than this handle's");
+  if (bytes < state_bytes(h)) return fail(LBM_EINVAL, "state snapshot truncated");
+  std::vector<DevParticle> recs((size_t)hd.np);
+  if (hd.np) std::memcpy(recs.data(), in + sizeof hd + pop_bytes, (size_t)hd.np * sizeof(DevParticle));
+  // the box plan follows the handle's particles (their radii): the
+  // snapshot's must be the same particles, with no move left pending
+  for (size_t p = 0; p < recs.size(); p++) {
+    const DevParticle &a = recs[p], &b = h->parts_h[p];
+    if (a.kind != b.kind || a.radius != b.radius || a.whole != b.whole || a.pending)
+      return fail(LBM_EINVAL, "particle " + std::to_string(p) + " of the snapshot is not this handle's");
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+  HIP_TRY(hipMemcpy(h->f[h->cur], in + sizeof hd, (size_t)pop_bytes, hipMemcpyHostToDevice), "copy populations");
+  if (hd.np)
+    HIP_TRY(hipMemcpy(parts_of(h, h->pcur), recs.data(), recs.size() * sizeof(DevParticle), hipMemcpyHostToDevice),
+            "copy particles");
+  h->parts_h = recs;
+  h->tick = hd.tick;
+  h->tick0 = hd.tick0;
+  h->time = hd.time;
+  h->populated = true;
+  h->traj_from = h->tick;  // the log holds no tick of the restored past
+  return LBM_OK;
+}
+
 int lbm_set_block(lbm_handle *h, int32_t bx, int32_t by, int32_t bz) {
   if (!h) return fail(LBM_EINVAL, "null handle");
   Tiling t;

@@ -57,6 +57,16 @@ function lbm_get_tick(h)
     lbmcheck(ccall((:lbm_get_tick, libhip), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Float64}), h, t, tm))
     return t[], tm[]
 end
+# state snapshots (no counterpart in the reference): the bytes a run resumes from
+function lbm_save_state(h)
+    n = Ref{Int64}(0)
+    lbmcheck(ccall((:lbm_state_size, libhip), Cint, (Ptr{Cvoid}, Ref{Int64}), h, n))
+    buf = Vector{UInt8}(undef, n[])
+    lbmcheck(ccall((:lbm_save_state, libhip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64), h, buf, n[]))
+    return buf
+end
+lbm_load_state(h, buf::Vector{UInt8}) =
+    lbmcheck(ccall((:lbm_load_state, libhip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64), h, buf, length(buf)))
 function lbm_check_populations(h)
     bad, sat = Ref{Int64}(0), Ref{Int64}(0)
     lbmcheck(ccall((:lbm_check_populations, libhip), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}), h, bad, sat))

@@ -136,6 +136,21 @@ function LBMpropagate!(models::Vector{LBMmodel}; simulationTime = :default, tick
     return nothing
 end
 
+"""
+    saveState(model, path)
+    loadState!(model, path)
+
+Write the state a run resumes from (populations in the storage's encoding,
+every particle's state record, the tick and the time) to `path`, and resume
+a model built the same way from it, bit for bit.  No counterpart in the
+reference, which cannot resume a run.
+"""
+saveState(model::LBMmodel, path::AbstractString) = write(path, lbm_save_state(getfield(model, :handle)))
+function loadState!(model::LBMmodel, path::AbstractString)
+    lbm_load_state(getfield(model, :handle), read(path))
+    return model
+end
+
 # viscousStressTensor (cpu/fluids.jl:507-520) evaluated on the device: a
 # dims x dims matrix of fields, as the reference returns it.
 function viscousStressTensor(model::LBMmodel)

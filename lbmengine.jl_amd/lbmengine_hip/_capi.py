@@ -108,6 +108,9 @@ SIGNATURES = {
     "lbm_run_noalloc": [_vp, C.c_int64],
     "lbm_sync": [_vp],
     "lbm_get_tick": [_vp, C.POINTER(C.c_int64), _dp],
+    "lbm_state_size": [_vp, C.POINTER(C.c_int64)],
+    "lbm_save_state": [_vp, _vp, C.c_int64],
+    "lbm_load_state": [_vp, _vp, C.c_int64],
     "lbm_set_block": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "lbm_get_block": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "lbm_set_tile_order": [_vp, C.c_int32],

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field
 from fractions import Fraction
 from typing import Any, Callable, Optional
@@ -595,6 +596,28 @@ class LBMmodel:
     @property
     def fluidVelocity(self) -> np.ndarray:
         return self._hydro()[1]
+
+    # --- state snapshots (no counterpart in the reference) -----------------
+    def save_state(self, path=None) -> np.ndarray:
+        """The state a run continues from: the populations in the storage's
+        own encoding, every particle's state record, the tick and the time,
+        as a byte array (also written to `path` when given).  `load_state`
+        on a model built the same way resumes from it bit for bit."""
+        n = C.c_int64()
+        _capi.check(self._fn["lbm_state_size"](self._h, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _capi.check(self._fn["lbm_save_state"](self._h, buf.ctypes.data_as(C.c_void_p), n.value))
+        if path is not None:
+            buf.tofile(path)
+        return buf
+
+    def load_state(self, src) -> None:
+        """Resume from a `save_state` snapshot: a byte array or a file path."""
+        if isinstance(src, (str, os.PathLike)):
+            buf = np.fromfile(src, np.uint8)
+        else:
+            buf = np.ascontiguousarray(np.frombuffer(src, np.uint8))
+        _capi.check(self._fn["lbm_load_state"](self._h, buf.ctypes.data_as(C.c_void_p), buf.size))
 
     # --- backend controls ------------------------------------------------
     def set_block(self, bx: int, by: int, bz: int = 1) -> None:
