@@ -1596,8 +1596,15 @@ int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms) 
   const size_t bytes = (size_t)h->N * h->Q * elem_size(h->cfg.storage);
   char *scratch = nullptr;
   HIP_TRY(hipMalloc(&scratch, 2 * bytes), "hipMalloc autotune scratch");
-  if (const hipError_t ce = hipMemcpyAsync(scratch, h->f[h->cur], bytes, hipMemcpyDeviceToDevice, h->stream);
-      ce != hipSuccess) {
+  // A run's first kernel collides without gathering and records the rho
+  // that the first gathering kernel's links read (rho_side); the trials
+  // start the same way, from a copy of the populations in the second
+  // scratch buffer into the first, so that they read no rho that no kernel
+  // has written (a handle that never ran holds none).
+  hipError_t ce = hipMemcpyAsync(scratch + bytes, h->f[h->cur], bytes, hipMemcpyDeviceToDevice, h->stream);
+  if (ce == hipSuccess) ce = launch_mode(h, kCollide, scratch + bytes, scratch, h->pcur);
+  if (ce != hipSuccess) {
+    hipStreamSynchronize(h->stream);
     hipFree(scratch);
     return hip_fail(ce, "autotune copy");
   }
@@ -2338,12 +2345,25 @@ int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best
     hipFree(scratch);
     return fail(LBM_ENOMEM, "hipMalloc autotune table");
   }
-  for (int i = 0; i < n; i++) {
-    const lbm_handle *h = b->boxes[i];
-    const hipError_t ce = hipMemcpyAsync(scratch + off[i], h->f[h->cur],
-                                         (size_t)h->N * h->Q * elem_size(b->storage), hipMemcpyDeviceToDevice,
-                                         b->stream);
+  // As a run does, the trials start with a collide-only kernel, from a copy
+  // of each box's populations in the second scratch buffer into the first,
+  // which records the rho the first gathering kernel's links read.
+  {
+    hipError_t ce = hipSuccess;
+    std::vector<StepArgs> col((size_t)n);
+    for (int i = 0; i < n && ce == hipSuccess; i++) {
+      lbm_handle *h = b->boxes[i];
+      char *s0 = scratch + off[i], *s1 = s0 + scratch_bytes;
+      ce = hipMemcpyAsync(s1, h->f[h->cur], (size_t)h->N * h->Q * elem_size(b->storage), hipMemcpyDeviceToDevice,
+                          b->stream);
+      col[i] = step_args(h, s1, s0, h->pcur);
+    }
+    if (ce == hipSuccess)
+      ce = hipMemcpyAsync(trial_d, col.data(), (size_t)n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream);
+    if (ce == hipSuccess) ce = launch_batch_mode(b, kCollide, col.data(), trial_d);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(b->stream);
     if (ce != hipSuccess) {
+      hipStreamSynchronize(b->stream);
       hipFree(trial_d);
       hipFree(scratch);
       return hip_fail(ce, "batch autotune copy");

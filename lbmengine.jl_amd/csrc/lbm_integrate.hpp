@@ -116,7 +116,11 @@ struct StepParticles {
 constexpr int kBoxMargin = 2;
 
 // Link box of a sphere of radius r at Rg (node indices, inclusive): every
-// solid node and every fluid node with a solid neighbour lies inside.
+// solid node and every fluid node with a solid neighbour lies inside.  A box
+// that leaves the lattice raises *err and is clamped into it; the bounds are
+// formed and clamped in double precision, so that any centre (huge,
+// infinite or NaN, as a diverging run may produce) gives indices inside the
+// lattice rather than a saturated or wrapped integer.
 __host__ __device__ inline void link_box(const double Rg[3], double r, const double x0[3],
                                          double dx, const int32_t n[3], int dims,
                                          int32_t lo[3], int32_t hi[3], int32_t *err) {
@@ -127,15 +131,16 @@ __host__ __device__ inline void link_box(const double Rg[3], double r, const dou
       continue;
     }
     const double a = (Rg[k] - r - x0[k]) / dx, b = (Rg[k] + r - x0[k]) / dx;
-    int32_t l = (int32_t)floor(a) - 2, h = (int32_t)ceil(b) + 2;
-    if (!(a == a) || !(b == b) || l < 0 || h > n[k] - 1) {
+    const double top = (double)(n[k] - 1);
+    double l = floor(a) - 2.0, h = ceil(b) + 2.0;
+    if (!(l >= 0.0) || !(h <= top)) {  // (NaN fails both)
       *err = 1;
-      l = l < 0 ? 0 : (l > n[k] - 1 ? n[k] - 1 : l);
-      h = h > n[k] - 1 ? n[k] - 1 : (h < 0 ? 0 : h);
-      if (!(a == a) || !(b == b)) l = 0, h = 0;
+      l = l >= 0.0 ? (l <= top ? l : top) : 0.0;
+      h = h <= top ? (h >= 0.0 ? h : 0.0) : top;
+      if (!(a == a) || !(b == b)) l = 0.0, h = 0.0;
     }
-    lo[k] = l;
-    hi[k] = h;
+    lo[k] = (int32_t)l;
+    hi[k] = (int32_t)h;
   }
 }
 

@@ -735,21 +735,27 @@ __host__ __device__ inline void band_row(const Band &b, int y, int z, int &a0, i
   const double dy = y - b.c[1], dz = z - b.c[2];
   const double rho2 = dy * dy + dz * dz;
   if (rho2 > b.rout2) return;
+  // (the ends are clamped to the owned row in double precision before they
+  // become integers, so that a centre of any value, NaN included, gives a
+  // range inside the row)
   const double ao = sqrt(b.rout2 - rho2);
-  int lo = (int)ceil(b.c[0] - ao), hi = (int)floor(b.c[0] + ao);
-  lo = lo < b.lo[0] ? b.lo[0] : lo;
-  hi = hi > b.hi[0] ? b.hi[0] : hi;
+  const double rl = b.lo[0], rh = b.hi[0];
+  double lo = ceil(b.c[0] - ao), hi = floor(b.c[0] + ao);
+  lo = lo >= rl ? (lo <= rh + 1.0 ? lo : rh + 1.0) : rl;
+  hi = hi <= rh ? (hi >= rl - 1.0 ? hi : rl - 1.0) : rh;
   if (rho2 < b.rin2) {
     // minus the hole |x - c| < ai
     const double ai = sqrt(b.rin2 - rho2);
-    const int l1 = (int)floor(b.c[0] - ai), h0 = (int)ceil(b.c[0] + ai);
-    a0 = lo;
-    na = (l1 < hi ? l1 : hi) - lo + 1;
-    b0 = h0 > lo ? h0 : lo;
-    nb = hi - b0 + 1;  // (l1 < c0 < h0: the two never overlap)
+    double l1 = floor(b.c[0] - ai), h0 = ceil(b.c[0] + ai);
+    l1 = l1 <= hi ? (l1 >= lo - 1.0 ? l1 : lo - 1.0) : hi;
+    h0 = h0 >= lo ? (h0 <= hi + 1.0 ? h0 : hi + 1.0) : lo;
+    a0 = (int)lo;
+    na = (int)(l1 - lo) + 1;
+    b0 = (int)h0;
+    nb = (int)(hi - h0) + 1;  // (l1 < c0 < h0: the two never overlap)
   } else {
-    a0 = lo;
-    na = hi - lo + 1;
+    a0 = (int)lo;
+    na = (int)(hi - lo) + 1;
   }
   na = na > 0 ? na : 0;
   nb = nb > 0 ? nb : 0;
