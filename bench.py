@@ -431,8 +431,10 @@ def small_configs(lh, args):
                            "MLUPS": round(n2 * 2000 / w / 1e6, 1), "ms_per_step": round(w / 2000 * 1e3, 5),
                            "kernel_ms": round(k, 5), "step_ms_device": round(s, 5), "block": list(m.get_block()),
                            "hbm_frac_algorithmic": round(2 * 19 * BYTES[st] * n2 / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                           "note": f"{2 * 19 * BYTES[st] * n2 / 1e6:.0f} MB per launch: within the 256 MB "
-                                   "last-level cache for f32, so the algorithmic rate can exceed what HBM delivers"}
+                           "note": f"{2 * 19 * BYTES[st] * n2 / 1e6:.0f} MB per launch"
+                                   + (": within the 256 MB last-level cache, so the algorithmic rate can exceed "
+                                      "what HBM delivers" if 2 * 19 * BYTES[st] * n2 <= 256e6 else
+                                      ": more than the 256 MB last-level cache holds")}
         del m
     return out
 
