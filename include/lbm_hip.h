@@ -338,10 +338,12 @@ int lbm_get_tile_order(lbm_handle *h, int32_t *xcd_remap);
 int lbm_set_occupancy(lbm_handle *h, int32_t low);
 int lbm_get_occupancy(lbm_handle *h, int32_t *low);
 
-/* Autotuner: times the fused step kernel over candidate block shapes and
- * tile orders (reading the current populations, writing a scratch buffer,
- * so the state is untouched), keeps the fastest and returns it with its time
- * per launch.
+/* Autotuner: times the fused step kernel over candidate block shapes, tile
+ * orders and both occupancy builds, `reps` kernels each, then the best few
+ * again three times with more kernels, and keeps the lowest median; returns
+ * it with its time per launch.  The trials run as a run does (a
+ * collide-only kernel, then fused kernels that move the particles) on a
+ * copy of the populations and records, so the state is untouched.
  * No counterpart in the reference (CUDA.jl launches 8x8x8 blocks,
  * gpu/aux.jl:12-21). */
 int lbm_autotune(lbm_handle *h, int32_t reps, int32_t best[3], double *best_ms);
@@ -423,8 +425,8 @@ int lbm_batch_get_occupancy(lbm_batch *b, int32_t *low);
 
 /* Autotuner of the batched fused kernel over the block shapes that fit every
  * box, row tiles of 64..512 threads, both tile orders and both occupancy
- * builds (the boxes' state is untouched).  best: the block shape, or
- * {threads, 0, 0} when row tiles are fastest. */
+ * builds, ranked as lbm_autotune ranks (the boxes' state is untouched).
+ * best: the block shape, or {threads, 0, 0} when row tiles are fastest. */
 int lbm_batch_autotune(lbm_batch *b, int32_t reps, int32_t best[3], double *best_ms);
 
 /* Ticks per captured graph (0 or even; default 32), timing and the averages
