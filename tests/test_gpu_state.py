@@ -1,8 +1,9 @@
 """State snapshots (lbm_save_state / lbm_load_state): a model built the same
 way and given a snapshot runs on bit for bit as the saved one does, in
-memory or through a file; snapshots of another lattice, storage or particle
-set are refused; the per-tick log refuses the ticks before the restored one.
-The reference has no resume, so the uninterrupted run is the check."""
+memory or through a file, with walls, with more than 8 particles and with
+bonds; snapshots of another lattice, storage or particle set are refused;
+the per-tick log refuses the ticks before the restored one.  The reference
+has no resume, so the uninterrupted run is the check."""
 import numpy as np
 import pytest
 
@@ -80,3 +81,38 @@ def test_log_after_resume(gpu):
     b.sync()
     _capi.check(b._fn["lbm_get_trajectory"](b._h, 0, 15, 3, _capi.as_dptr(out)))
     assert np.array_equal(out[-1, :3], b.particles[0].position)
+
+
+def test_resume_with_many_particles_and_bonds(gpu):
+    """Twelve particles (the move kernel's path) tied by linear bonds: the
+    resumed run equals the saved one's continuation bit for bit."""
+    from lbmengine_hip import interactions as it
+
+    def build():
+        rng = np.random.default_rng(31)
+        m = lh.modelInit(dims=3, velocities="D3Q19", latticeParameter=1.0, relaxationTimeRatio=0.8,
+                         kineticParameter=3 / 16, xlims=(-32, 31), ylims=(-24, 23), zlims=(-24, 23),
+                         storage="f32", forceDensity=[1e-6, -5e-7, 0.0])
+        for i in range(3):
+            for j in range(2):
+                for k in range(2):
+                    R = (np.array([i - 1.0, j - 0.5, k - 0.5]) * 20.0) + rng.uniform(-0.4, 0.4, 3)
+                    if (i + j + k) % 3 == 0:
+                        lh.addBead(m, radius=3.5, position=R, coupleForces=True, coupleTorques=True)
+                    else:
+                        lh.addSquirmer(m, radius=3.5, position=R, swimmingSpeed=2e-3 / 3, beta=rng.uniform(-4, 4),
+                                       swimmingDirection=rng.standard_normal(3), coupleForces=True,
+                                       coupleTorques=True)
+        it.addLinearBond(m, 1, 2, equilibriumDisplacement=20.0, stiffness=0.01)
+        it.addLinearBond(m, 5, 6, equilibriumDisplacement=20.0, stiffness=0.02)
+        return m
+
+    a = build()
+    assert len(a.particles) == 12
+    a.run(9)
+    snap = a.save_state()
+    a.run(40)
+    b = build()
+    b.load_state(snap)
+    b.run(40)
+    assert same(a, b)
